@@ -1,0 +1,194 @@
+/*
+ * smp_hip.h — C ABI of the MI355X stable-marriage (BambooSMP-shaped) proposal engine.
+ *
+ * Plain pointers and sizes only; no C++ or framework types cross this boundary.
+ * Every call returns an int status (SMP_OK == 0) and never throws; the text of
+ * the last error on the calling thread is available from smp_last_error().
+ *
+ * Data layout (same as the reference's flat lists, include/utils/types.h:33):
+ *   pref_m[m*n + k] = man m's k-th choice of woman   (int32, row-major n x n)
+ *   pref_w[w*n + k] = woman w's k-th choice of man   (int32, row-major n x n)
+ *   result match[m] = woman matched to man m         (int32[n]; Matching, types.h:44)
+ * The result is the man-optimal stable matching, which is unique.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference repository root):
+ *   smp_create         SmpEngineBamboo::SmpEngineBamboo(const SmpObj&, size_t n)
+ *                      src/smp/smp_engine_bamboo.cu:6-31 (device allocation + H2D of
+ *                      both lists; SmpObj::CreateFromFlatLists include/smp/smp.h:25-29)
+ *   smp_create_device  same constructor, for lists already resident in HBM (borrowed)
+ *   smp_exec           AbsSmpEngine::FindStableMatching include/smp/smp_engine_abs.h:10-36
+ *                      (precheck IsPerfect smp_engine_bamboo.h:25-33, InitProc
+ *                      smp_engine_bamboo.cu:33-106, CoreProc :108-287, PostProc :289-306)
+ *   smp_result         the Matching returned by FindStableMatching (abs.h:35)
+ *   smp_profile        AbsSmpEngine::PrintProfilingInfo abs.h:38-52 (precheck/init/core/total ms)
+ *   smp_destroy        ~SmpEngineBamboo smp_engine_bamboo.h:18-19 (RAII device arrays)
+ *   smp_generate       GenerateWorkload include/utils/generate_workloads_flat.h:250-280
+ *                      (seeded here; the reference seeds from std::random_device)
+ *   smp_batch_*        no reference counterpart: the reference runs one instance per
+ *                      SmpEngineExecutor call (benchmark/smp_engine_exec.h:6-32); this is
+ *                      that executor looped over many instances, in one launch.
+ *   smp_server_*       no reference counterpart: request-serving loop around the engine.
+ */
+#ifndef PACTAE_SMP_HIP_H
+#define PACTAE_SMP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SMP_OK 0
+#define SMP_ERR_INVALID 1  /* bad argument or malformed preference lists      */
+#define SMP_ERR_HIP 2      /* HIP runtime error (no device, launch failure...) */
+#define SMP_ERR_NOMEM 3    /* device or pinned host allocation failed          */
+#define SMP_ERR_STATE 4    /* call out of order (e.g. result before exec)      */
+#define SMP_ERR_TIMEOUT 5  /* a bounded wait expired                          */
+#define SMP_PENDING 6      /* server: request not finished yet                */
+
+/* Largest n accepted: women, men and list positions are packed into 16-bit
+ * fields of the men-major PR entries. */
+#define SMP_MAX_N 65535
+
+/* ---- workloads ---------------------------------------------------------- */
+enum smp_workload {
+  SMP_WL_UNIFORM = 0,     /* every row an independent uniform permutation          */
+  SMP_WL_CONGESTED = 1,   /* all men share one permutation L; women uniform         */
+  SMP_WL_SHIFTED = 2,     /* man i's list is L rotated left by i; women uniform     */
+  SMP_WL_CORRELATED = 3,  /* men: one base permutation + param transpositions each;
+                             param <= 0 means n/10; women uniform                   */
+  SMP_WL_REF_RANDOM = 4,  /* reference RANDOM: shuffled id order, rows shuffled in
+                             groups of param (default 5); both sides
+                             (generate_workloads_flat.h:85-137)                     */
+  SMP_WL_REF_CONGESTED = 5, /* reference CONGESTED: men share L, women share P (:14-47) */
+  SMP_WL_REF_PERFECT = 6, /* reference PERFECT: distinct first choices (:50-82)     */
+  SMP_WL_REF_SOLO = 7     /* reference SOLO: long displacement chain (:140-247)     */
+};
+
+/* Fill pref_m and pref_w (each n*n int32, host memory) with workload `kind`.
+ * Deterministic in (kind, n, seed, param): every row is drawn from its own
+ * xoshiro256** stream seeded through SplitMix64 from (seed, side, row), with
+ * Fisher-Yates and unbiased bounded draws.  threads <= 0 picks a default. */
+int smp_generate(int32_t kind, int32_t n, uint64_t seed, int32_t param, int32_t* pref_m,
+                 int32_t* pref_w, int32_t threads);
+
+/* ---- single-instance engine ---------------------------------------------- */
+typedef struct smp_engine smp_engine;
+
+enum smp_tail_mode {
+  SMP_TAIL_AUTO = 0, /* hand the last free men to the host when that is faster     */
+  SMP_TAIL_GPU = 1,  /* finish every chain in the persistent proposal kernel       */
+  SMP_TAIL_HOST = 2  /* always hand off at tail_threshold free men                 */
+};
+
+typedef struct smp_opts {
+  int32_t prefix_cols;    /* PR columns gathered at init: 0 = auto, >= n = full n x n
+                             (the reference's layout); beyond it the kernel reads
+                             pref_m and the rank matrix directly                  */
+  int32_t tail_mode;      /* enum smp_tail_mode                                    */
+  int32_t tail_threshold; /* free-man count at which the host takes over; 0 = auto */
+  int32_t device;         /* HIP device ordinal                                    */
+  int32_t precheck;       /* 1 = IsPerfect fast path (default), 0 = skip           */
+  int32_t waves_per_cu;   /* proposal-kernel occupancy; 0 = auto                   */
+  int32_t validate;       /* 1 = reject rows that are not permutations (default)   */
+  int32_t reserved[9];
+} smp_opts;
+
+typedef struct smp_times {
+  /* Wall clock of the timed phases.  precheck/init/core are the three phases
+   * AbsSmpEngine reports; post (result assembly and the copy of the matching to
+   * the host) is untimed in the reference and is timed here.  total is all four. */
+  double precheck_ms, init_ms, core_ms, post_ms, total_ms;
+  /* Device time of the main kernels (HIP events on the engine's stream). */
+  double rank_kernel_ms, gather_kernel_ms, propose_kernel_ms;
+  /* Host time of the hand-off (copies) and of the host tail loop. */
+  double handoff_ms, host_tail_ms;
+  int64_t proposals;       /* sum over men of (final list position + 1)           */
+  int64_t atomics;         /* atomicMin attempts made by the proposal kernel      */
+  int64_t host_proposals;  /* proposals made by the host tail                     */
+  int32_t is_perfect;      /* 1 if the first-choice fast path answered            */
+  int32_t handoff_free;    /* free men handed to the host (0 = GPU finished)      */
+  int32_t prefix_cols;     /* PR columns actually gathered                        */
+  int32_t tail_threshold;  /* threshold used                                      */
+  int32_t host_row_fetches;/* PR rows the host tail fetched beyond the prefix     */
+  int32_t reserved[7];
+} smp_times;
+
+void smp_default_opts(smp_opts* opts);
+
+/* Copies both host lists into HBM (untimed, like the reference constructor). */
+int smp_create(const int32_t* pref_m, const int32_t* pref_w, int32_t n, const smp_opts* opts,
+               smp_engine** out);
+/* Borrows lists already in HBM; they must outlive the engine and stay unchanged. */
+int smp_create_device(const int32_t* d_pref_m, const int32_t* d_pref_w, int32_t n,
+                      const smp_opts* opts, smp_engine** out);
+/* One full solve: precheck + init + core + post.  Synchronous: every device and
+ * host-tail step has finished when it returns.  May be called repeatedly. */
+int smp_exec(smp_engine* e);
+/* Copy the last result (int32[n], man -> woman) to host memory. */
+int smp_result(smp_engine* e, int32_t* match_m2w);
+/* Device address of the last result (int32[n] in HBM, owned by the engine). */
+int smp_result_device(smp_engine* e, const int32_t** d_match);
+int smp_profile(const smp_engine* e, smp_times* t);
+void smp_destroy(smp_engine* e);
+
+/* ---- batched: many independent instances in one persistent launch -------- */
+typedef struct smp_batch smp_batch;
+
+typedef struct smp_batch_opts {
+  int32_t device;
+  int32_t use_graph;      /* 1 = capture the launch sequence in a hipGraph (default) */
+  int32_t block_threads;  /* workgroup size of the batched kernel; 0 = autotuned     */
+  int32_t wgs_per_cu;     /* persistent workgroups per CU; 0 = autotuned            */
+  int32_t reserved[12];
+} smp_batch_opts;
+
+typedef struct smp_batch_times {
+  double exec_ms;         /* wall time of the last smp_batch_exec / _looped         */
+  double kernel_ms;       /* device time of the persistent kernel(s)                */
+  double solves_per_s;
+  int64_t proposals;
+  int32_t count;
+  int32_t block_threads, wgs_per_cu;
+  int32_t reserved[9];
+} smp_batch_times;
+
+void smp_batch_default_opts(smp_batch_opts* opts);
+/* sizes[i] = n_i; instance i's lists start at element offset offsets[i] of
+ * pref_m_all / pref_w_all (each list n_i*n_i int32).  Host buffers are copied. */
+int smp_batch_create(int32_t count, const int32_t* sizes, const int64_t* offsets,
+                     const int32_t* pref_m_all, const int32_t* pref_w_all,
+                     const smp_batch_opts* opts, smp_batch** out);
+/* Solve every instance in one persistent launch (work-stealing over instances). */
+int smp_batch_exec(smp_batch* b);
+/* Baseline: solve the same instances one by one with plain single-instance launches. */
+int smp_batch_exec_looped(smp_batch* b);
+/* Results concatenated in instance order: match_all[moff[i] + m], moff = prefix sums of sizes. */
+int smp_batch_result(smp_batch* b, int32_t* match_all);
+int smp_batch_profile(const smp_batch* b, smp_batch_times* t);
+void smp_batch_destroy(smp_batch* b);
+
+/* Deterministic batch plan: sizes log-uniform in [n_min, n_max], kinds drawn with
+ * weights mix[0..2] over {UNIFORM, CONGESTED, CORRELATED}. */
+int smp_batch_plan(int32_t count, int32_t n_min, int32_t n_max, uint64_t seed,
+                   const double* mix3, int32_t* sizes, int32_t* kinds);
+
+/* ---- device memory helpers (so hosts need no other GPU runtime) ----------- */
+int smp_device_count(int32_t* count);
+int smp_device_alloc(size_t bytes, void** d_ptr);
+int smp_device_free(void* d_ptr);
+int smp_copy_h2d(void* d_dst, const void* h_src, size_t bytes);
+int smp_copy_d2h(void* h_dst, const void* d_src, size_t bytes);
+int smp_device_sync(void);
+
+const char* smp_last_error(void);
+const char* smp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PACTAE_SMP_HIP_H */

@@ -1,0 +1,144 @@
+"""ctypes declarations of the C ABI in include/smp_hip.h.
+
+Loads pactae_amd/_lib/libsmp_hip.so (built in-tree by pactae_amd.build.build()).
+There is no fallback: if the library is missing or fails to load, importing the
+package raises, so nothing can silently run without the HIP code.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsmp_hip.so")
+
+SMP_OK = 0
+SMP_ERR_INVALID = 1
+SMP_ERR_HIP = 2
+SMP_ERR_NOMEM = 3
+SMP_ERR_STATE = 4
+SMP_ERR_TIMEOUT = 5
+SMP_PENDING = 6
+SMP_MAX_N = 65535
+
+SMP_TAIL_AUTO = 0
+SMP_TAIL_GPU = 1
+SMP_TAIL_HOST = 2
+
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+u64 = ctypes.c_uint64
+i32p = ctypes.POINTER(ctypes.c_int32)
+i64p = ctypes.POINTER(ctypes.c_int64)
+vp = ctypes.c_void_p
+
+
+class SmpOpts(ctypes.Structure):
+    _fields_ = [
+        ("prefix_cols", i32),
+        ("tail_mode", i32),
+        ("tail_threshold", i32),
+        ("device", i32),
+        ("precheck", i32),
+        ("waves_per_cu", i32),
+        ("validate", i32),
+        ("reserved", i32 * 9),
+    ]
+
+
+class SmpTimes(ctypes.Structure):
+    _fields_ = [
+        ("precheck_ms", ctypes.c_double),
+        ("init_ms", ctypes.c_double),
+        ("core_ms", ctypes.c_double),
+        ("post_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("rank_kernel_ms", ctypes.c_double),
+        ("gather_kernel_ms", ctypes.c_double),
+        ("propose_kernel_ms", ctypes.c_double),
+        ("handoff_ms", ctypes.c_double),
+        ("host_tail_ms", ctypes.c_double),
+        ("proposals", i64),
+        ("atomics", i64),
+        ("host_proposals", i64),
+        ("is_perfect", i32),
+        ("handoff_free", i32),
+        ("prefix_cols", i32),
+        ("tail_threshold", i32),
+        ("host_row_fetches", i32),
+        ("reserved", i32 * 7),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class SmpBatchOpts(ctypes.Structure):
+    _fields_ = [
+        ("device", i32),
+        ("use_graph", i32),
+        ("block_threads", i32),
+        ("wgs_per_cu", i32),
+        ("reserved", i32 * 12),
+    ]
+
+
+class SmpBatchTimes(ctypes.Structure):
+    _fields_ = [
+        ("exec_ms", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("solves_per_s", ctypes.c_double),
+        ("proposals", i64),
+        ("count", i32),
+        ("block_threads", i32),
+        ("wgs_per_cu", i32),
+        ("reserved", i32 * 9),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+# (name, restype, argtypes) for every entry point declared in include/smp_hip.h.
+SIGNATURES = [
+    ("smp_generate", ctypes.c_int, [i32, i32, u64, i32, i32p, i32p, i32]),
+    ("smp_default_opts", None, [ctypes.POINTER(SmpOpts)]),
+    ("smp_create", ctypes.c_int, [i32p, i32p, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
+    ("smp_create_device", ctypes.c_int, [vp, vp, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
+    ("smp_exec", ctypes.c_int, [vp]),
+    ("smp_result", ctypes.c_int, [vp, i32p]),
+    ("smp_result_device", ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+    ("smp_profile", ctypes.c_int, [vp, ctypes.POINTER(SmpTimes)]),
+    ("smp_destroy", None, [vp]),
+    ("smp_batch_default_opts", None, [ctypes.POINTER(SmpBatchOpts)]),
+    ("smp_batch_create", ctypes.c_int,
+     [i32, i32p, i64p, i32p, i32p, ctypes.POINTER(SmpBatchOpts), ctypes.POINTER(vp)]),
+    ("smp_batch_exec", ctypes.c_int, [vp]),
+    ("smp_batch_exec_looped", ctypes.c_int, [vp]),
+    ("smp_batch_result", ctypes.c_int, [vp, i32p]),
+    ("smp_batch_profile", ctypes.c_int, [vp, ctypes.POINTER(SmpBatchTimes)]),
+    ("smp_batch_destroy", None, [vp]),
+    ("smp_batch_plan", ctypes.c_int,
+     [i32, i32, i32, u64, ctypes.POINTER(ctypes.c_double), i32p, i32p]),
+    ("smp_device_count", ctypes.c_int, [i32p]),
+    ("smp_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(vp)]),
+    ("smp_device_free", ctypes.c_int, [vp]),
+    ("smp_copy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    ("smp_copy_d2h", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    ("smp_device_sync", ctypes.c_int, []),
+    ("smp_last_error", ctypes.c_char_p, []),
+    ("smp_version", ctypes.c_char_p, []),
+]
+
+
+def load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"pactae_amd: HIP library {LIB_PATH} is not built; run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (or pactae_amd.build.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(lib, name)  # AttributeError here means the library lacks a declared symbol
+        fn.restype = restype
+        fn.argtypes = argtypes
+    return lib

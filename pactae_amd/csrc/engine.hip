@@ -1,0 +1,209 @@
+// engine.hip — C ABI of the single-instance engine (declared in include/smp_hip.h);
+// the engine itself is in engine.hpp.
+
+#include "engine.hpp"
+
+namespace pactae {
+
+namespace {
+thread_local std::string g_last_error;
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+}  // namespace pactae
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+void smp_default_opts(smp_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->prefix_cols = 0;
+  o->tail_mode = SMP_TAIL_AUTO;
+  o->tail_threshold = 0;
+  o->device = 0;
+  o->precheck = 1;
+  o->waves_per_cu = 0;
+  o->validate = 1;
+}
+
+static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
+  if (!out) {
+    pactae::set_error("out is null");
+    return SMP_ERR_INVALID;
+  }
+  *out = nullptr;
+  if (n <= 0 || n > SMP_MAX_N) {
+    pactae::set_error("n must be in [1, " + std::to_string(SMP_MAX_N) + "], got " +
+                      std::to_string(n));
+    return SMP_ERR_INVALID;
+  }
+  auto e = std::make_unique<smp_engine>();
+  e->n = n;
+  if (opts)
+    e->opts = *opts;
+  else
+    smp_default_opts(&e->opts);
+  if (int st = e->setup()) return st;
+  *out = e.release();
+  return SMP_OK;
+}
+
+int smp_create(const int32_t* pref_m, const int32_t* pref_w, int32_t n, const smp_opts* opts,
+               smp_engine** out) {
+  if (!pref_m || !pref_w) {
+    pactae::set_error("null preference list");
+    return SMP_ERR_INVALID;
+  }
+  smp_engine* e = nullptr;
+  if (int st = make_engine(n, opts, &e)) return st;
+  const size_t bytes = (size_t)n * n * sizeof(int32_t);
+  int32_t *dm = nullptr, *dw = nullptr;
+  if (hipMalloc(&dm, bytes) != hipSuccess || hipMalloc(&dw, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(dm);
+    pactae::set_error("device allocation of the preference lists failed");
+    delete e;
+    return SMP_ERR_NOMEM;
+  }
+  e->d_pref_m = dm;
+  e->d_pref_w = dw;
+  e->owns_lists = true;
+  if (hipMemcpy(dm, pref_m, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dw, pref_w, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    pactae::set_error("H2D copy of the preference lists failed");
+    delete e;
+    return SMP_ERR_HIP;
+  }
+  *out = e;
+  return SMP_OK;
+}
+
+int smp_create_device(const int32_t* d_pref_m, const int32_t* d_pref_w, int32_t n,
+                      const smp_opts* opts, smp_engine** out) {
+  if (!d_pref_m || !d_pref_w) {
+    pactae::set_error("null device preference list");
+    return SMP_ERR_INVALID;
+  }
+  smp_engine* e = nullptr;
+  if (int st = make_engine(n, opts, &e)) return st;
+  e->d_pref_m = d_pref_m;
+  e->d_pref_w = d_pref_w;
+  e->owns_lists = false;
+  *out = e;
+  return SMP_OK;
+}
+
+int smp_exec(smp_engine* e) {
+  if (!e) {
+    pactae::set_error("null engine");
+    return SMP_ERR_INVALID;
+  }
+  if (hipSetDevice(e->opts.device) != hipSuccess) {
+    pactae::set_error("hipSetDevice failed");
+    return SMP_ERR_HIP;
+  }
+  return e->exec();
+}
+
+int smp_result(smp_engine* e, int32_t* match) {
+  if (!e || !match) {
+    pactae::set_error("null argument");
+    return SMP_ERR_INVALID;
+  }
+  if (!e->have_result) {
+    pactae::set_error("smp_result before a successful smp_exec");
+    return SMP_ERR_STATE;
+  }
+  std::memcpy(match, e->h_match.data(), sizeof(int32_t) * e->n);
+  return SMP_OK;
+}
+
+int smp_result_device(smp_engine* e, const int32_t** d_match) {
+  if (!e || !d_match) {
+    pactae::set_error("null argument");
+    return SMP_ERR_INVALID;
+  }
+  if (!e->have_result) {
+    pactae::set_error("smp_result_device before a successful smp_exec");
+    return SMP_ERR_STATE;
+  }
+  *d_match = e->d_match;
+  return SMP_OK;
+}
+
+int smp_profile(const smp_engine* e, smp_times* t) {
+  if (!e || !t) {
+    pactae::set_error("null argument");
+    return SMP_ERR_INVALID;
+  }
+  *t = e->times;
+  return SMP_OK;
+}
+
+void smp_destroy(smp_engine* e) { delete e; }
+
+int smp_generate(int32_t kind, int32_t n, uint64_t seed, int32_t param, int32_t* pref_m,
+                 int32_t* pref_w, int32_t threads) {
+  int st = pactae::generate(kind, n, seed, param, pref_m, pref_w, threads);
+  if (st) pactae::set_error("smp_generate: bad kind or size");
+  return st;
+}
+
+int smp_batch_plan(int32_t count, int32_t n_min, int32_t n_max, uint64_t seed, const double* mix3,
+                   int32_t* sizes, int32_t* kinds) {
+  int st = pactae::batch_plan(count, n_min, n_max, seed, mix3, sizes, kinds);
+  if (st) pactae::set_error("smp_batch_plan: bad arguments");
+  return st;
+}
+
+int smp_device_count(int32_t* count) {
+  int c = 0;
+  if (!count) return SMP_ERR_INVALID;
+  hipError_t st = hipGetDeviceCount(&c);
+  if (st != hipSuccess) {
+    *count = 0;
+    pactae::set_error(std::string("hipGetDeviceCount: ") + hipGetErrorString(st));
+    return SMP_ERR_HIP;
+  }
+  *count = c;
+  return SMP_OK;
+}
+
+int smp_device_alloc(size_t bytes, void** p) {
+  if (!p) return SMP_ERR_INVALID;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    pactae::set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    return SMP_ERR_NOMEM;
+  }
+  return SMP_OK;
+}
+
+int smp_device_free(void* p) {
+  SMP_HIP_TRY(hipFree(p));
+  return SMP_OK;
+}
+
+int smp_copy_h2d(void* d, const void* h, size_t bytes) {
+  SMP_HIP_TRY(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+  return SMP_OK;
+}
+
+int smp_copy_d2h(void* h, const void* d, size_t bytes) {
+  SMP_HIP_TRY(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+  return SMP_OK;
+}
+
+int smp_device_sync(void) {
+  SMP_HIP_TRY(hipDeviceSynchronize());
+  return SMP_OK;
+}
+
+const char* smp_last_error(void) { return pactae::last_error(); }
+const char* smp_version(void) { return "pactae_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,395 @@
+// kernels.hip — gfx950 kernels of the single-instance proposal engine.
+//
+// Phases (reference counterparts, paths relative to the reference repository root):
+//   k_first_choices / k_first_match   IsPerfect + perfect PostProc
+//                                     include/smp/smp_engine_bamboo.h:25-33,
+//                                     src/smp/smp_engine_bamboo.cu:291-297
+//   k_invert_rows                     F9RankMatrixInitKernel src/smp/kernel.cu:672-686
+//   k_gather_pr                       F9PRMatrixInitKernel   src/smp/kernel.cu:688-709
+//   k_propose                         La3F9CoreKernel        src/smp/kernel.cu:633-670
+//   k_post                            PostProc               src/smp/smp_engine_bamboo.cu:299-303
+//   k_gather_row                      rows the host tail needs beyond the gathered prefix
+//                                     (the reference copies the whole PR matrix to the host
+//                                     instead, smp_engine_bamboo.h:53-61)
+//
+// All arithmetic is integer.  Every loop is bounded by n (list length) or by the
+// grid size; preference entries are range-checked before they index anything.
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace pactae {
+
+// ---------------------------------------------------------------------------
+// Fills (used instead of hipMemsetAsync so the same sequence can be captured
+// into a hipGraph).
+__global__ void k_fill_u64(unsigned long long* p, size_t count, unsigned long long v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_i32(int* p, size_t count, int v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_reset_scalars(Scalars* sc, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->cursor = 0;
+    sc->free_count = n;
+    sc->parked_count = 0;
+    sc->error = 0;
+    sc->dup_first = 0;
+    sc->atomics = 0;
+    sc->steps = 0;
+    sc->proposals = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Precheck: do all men have distinct first choices?  seen[] must be zero.
+__global__ void k_first_choices(const int32_t* __restrict__ pref_m, int n, int* seen, Scalars* sc) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  int w = pref_m[(size_t)m * n];
+  if ((unsigned)w >= (unsigned)n) {
+    atomicOr(&sc->error, kErrEntryRange);
+    atomicOr(&sc->dup_first, 1);
+    return;
+  }
+  if (atomicAdd(&seen[w], 1) != 0) atomicOr(&sc->dup_first, 1);
+}
+
+// Perfect case: every man keeps his first choice.
+__global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t* match) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) match[m] = pref_m[(size_t)m * n];
+}
+
+// ---------------------------------------------------------------------------
+// Rank inversion, one workgroup per row: rank[w][pref[w][r]] = r.
+// The row's inverse is assembled in LDS (`chunk` columns at a time; one pass when
+// the row fits) so that both the HBM read of pref and the HBM write of rank are
+// fully coalesced; only the LDS scatter is random.  With validate != 0 every LDS
+// slot starts at -1 and a slot left at -1 marks a row that is not a permutation.
+template <int VEC>
+__global__ __launch_bounds__(1024) void k_invert_rows(const int32_t* __restrict__ pref,
+                                                      int32_t* __restrict__ rank, int n, int chunk,
+                                                      int validate, Scalars* sc) {
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, bs = blockDim.x;
+  int bad = 0;
+  for (int row = blockIdx.x; row < n; row += gridDim.x) {
+    const int32_t* src = pref + (size_t)row * n;
+    int32_t* dst = rank + (size_t)row * n;
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+      const int clen = min(chunk, n - c0);
+      if (validate)
+        for (int c = tid; c < clen; c += bs) lds[c] = -1;
+      __syncthreads();
+      if constexpr (VEC == 4) {
+        const int4* src4 = reinterpret_cast<const int4*>(src);
+        for (int q = tid; q < (n >> 2); q += bs) {
+          int4 v = src4[q];
+          const int r = q << 2;
+          unsigned u;
+          u = (unsigned)(v.x - c0); if (u < (unsigned)clen) lds[u] = r;     else bad |= (unsigned)v.x >= (unsigned)n;
+          u = (unsigned)(v.y - c0); if (u < (unsigned)clen) lds[u] = r + 1; else bad |= (unsigned)v.y >= (unsigned)n;
+          u = (unsigned)(v.z - c0); if (u < (unsigned)clen) lds[u] = r + 2; else bad |= (unsigned)v.z >= (unsigned)n;
+          u = (unsigned)(v.w - c0); if (u < (unsigned)clen) lds[u] = r + 3; else bad |= (unsigned)v.w >= (unsigned)n;
+        }
+      } else {
+        for (int r = tid; r < n; r += bs) {
+          int v = src[r];
+          unsigned u = (unsigned)(v - c0);
+          if (u < (unsigned)clen) lds[u] = r;
+          else bad |= (unsigned)v >= (unsigned)n;
+        }
+      }
+      __syncthreads();
+      if constexpr (VEC == 4) {
+        int4* dst4 = reinterpret_cast<int4*>(dst + c0);
+        for (int q = tid; q < (clen >> 2); q += bs) {
+          int4 v = make_int4(lds[4 * q], lds[4 * q + 1], lds[4 * q + 2], lds[4 * q + 3]);
+          if (validate && (v.x < 0 || v.y < 0 || v.z < 0 || v.w < 0)) bad |= 2;
+          dst4[q] = v;
+        }
+      } else {
+        for (int c = tid; c < clen; c += bs) {
+          int v = lds[c];
+          if (validate && v < 0) bad |= 2;
+          dst[c0 + c] = v;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (bad & 1) atomicOr(&sc->error, kErrEntryRange);
+  if (bad & 2) atomicOr(&sc->error, kErrNotPerm);
+}
+
+// ---------------------------------------------------------------------------
+// PR gather: pr[m*K + k] = (w = pref_m[m][k], rank_w[w][m]) for k < K.
+// Tiles are 32 men x 256 columns.  The rank reads rank_w[w][m0..m0+32) of one
+// men block share 128-byte lines, so tiles of the same men block are kept on one
+// XCD (workgroup b runs on XCD b % 8) and consecutive in its dispatch order,
+// which keeps those lines in that XCD's L2 while the block is processed.
+constexpr int kGatherMen = 32;
+constexpr int kGatherCols = 256;
+
+__global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ pref_m,
+                                                   const int32_t* __restrict__ rank_w,
+                                                   uint32_t* __restrict__ pr, int n, int K,
+                                                   int tiles, int per_xcd, Scalars* sc) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= tiles) return;
+  const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
+  const int mb = t / col_tiles, ct = t - mb * col_tiles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int chunks = kGatherCols / 64;
+  int bad = 0;
+  for (int i = wave; i < kGatherMen * chunks; i += 4) {
+    const int m = mb * kGatherMen + (i / chunks);
+    const int k = ct * kGatherCols + (i % chunks) * 64 + lane;
+    if (m >= n || k >= K) continue;
+    const int w = pref_m[(size_t)m * n + k];
+    uint32_t e;
+    if ((unsigned)w < (unsigned)n) {
+      e = pr_pack((uint32_t)w, (uint32_t)rank_w[(size_t)w * n + m]);
+    } else {
+      bad = 1;
+      e = pr_pack(0, 0xFFFFu);
+    }
+    pr[(size_t)m * K + k] = e;
+  }
+  if (bad) atomicOr(&sc->error, kErrEntryRange);
+}
+
+// Entries [k0, k0 + count) of man m's PR row, for the host tail.
+__global__ void k_gather_row(const int32_t* __restrict__ pref_m, const int32_t* __restrict__ rank_w,
+                             int n, int m, int k0, int count, uint32_t* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  int k = k0 + i;
+  uint32_t e = pr_pack(0, 0xFFFFu);
+  if (k < n) {
+    int w = pref_m[(size_t)m * n + k];
+    if ((unsigned)w < (unsigned)n) e = pr_pack((uint32_t)w, (uint32_t)rank_w[(size_t)w * n + m]);
+  }
+  out[i] = e;
+}
+
+// ---------------------------------------------------------------------------
+// Proposal kernel.  Persistent: each wave owns one free man ("chain") at a time.
+// A step loads the next 64 entries of the man's list (one coalesced PR load, or
+// pref_m + rank_w beyond the gathered prefix), reads the 64 women's partner words
+// (plain, coherent loads: words only ever decrease, so a stale read only makes
+// the wave try an atomic that then fails), and ballots the lanes where the man
+// beats the current partner.  Candidates are tried in list order with a 64-bit
+// atomicMin; the first success engages the man.  If the woman was taken, the wave
+// continues as the displaced man at his stored position + 1; if she was free the
+// chain ends and the wave takes a never-started man from the cursor.
+//
+// Hand-off: with T > 0 a wave that sees free_count <= T after a step parks its
+// chain (man, next position) for the host and exits; waves do not start new men.
+
+
+__global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int n = a.n, K = a.K, T = a.T;
+  int m = -1, k = 0;
+  unsigned long long n_atomics = 0, n_steps = 0;
+  for (;;) {
+    if (m < 0) {
+      int got = 0;
+      if (lane == 0) got = atomicAdd(&a.sc->cursor, 1);
+      got = __shfl(got, 0);
+      if (got >= n) break;
+      m = got;
+      k = 0;
+    }
+    if (k >= n) {  // only possible when rows are not permutations
+      if (lane == 0) atomicOr(&a.sc->error, kErrListEnd);
+      break;
+    }
+    int fc = 0;
+    if (T > 0 && lane == 0)
+      fc = __hip_atomic_load(&a.sc->free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ++n_steps;
+    const int kk = k + lane;
+    bool valid = kk < n;
+    uint32_t w = 0, r = 0;
+    if (valid) {
+      if (kk < K) {
+        const uint32_t e = a.pr[(size_t)m * K + kk];
+        w = pr_woman(e);
+        r = pr_rank(e);
+      } else {
+        const int wi = a.pref_m[(size_t)m * n + kk];
+        if ((unsigned)wi < (unsigned)n) {
+          w = (uint32_t)wi;
+          r = (uint32_t)a.rank_w[(size_t)w * n + m] & 0xFFFFu;
+        } else {
+          valid = false;
+          atomicOr(&a.sc->error, kErrEntryRange);
+        }
+      }
+      valid = valid && w < (uint32_t)n;
+    }
+    const unsigned long long mine = pw_pack(r, (uint32_t)m, (uint32_t)kk);
+    unsigned long long hint = 0;
+    if (valid) hint = __hip_atomic_load(&a.partner[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long cand = __ballot(valid && mine < hint);
+    bool moved = false;
+    while (cand) {
+      const int j = __ffsll((long long)cand) - 1;
+      unsigned long long old = 0;
+      if (lane == j) old = atomicMin(&a.partner[w], mine);
+      old = __shfl(old, j);
+      const unsigned long long mj = __shfl(mine, j);
+      ++n_atomics;
+      if (old > mj) {
+        if (old == kEmpty) {
+          if (lane == 0) atomicSub(&a.sc->free_count, 1);
+          m = -1;
+        } else {
+          m = (int)pw_man(old);
+          k = (int)pw_pos(old) + 1;
+        }
+        moved = true;
+        break;
+      }
+      cand &= cand - 1;
+    }
+    if (!moved) k += 64;
+    if (T > 0) {
+      fc = __shfl(fc, 0);
+      if (fc <= T) {
+        if (m >= 0 && lane == 0) {
+          const int slot = atomicAdd(&a.sc->parked_count, 1);
+          a.parked[slot] = make_int2(m, k);
+        }
+        break;
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&a.sc->atomics, n_atomics);
+    atomicAdd(&a.sc->steps, n_steps);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Post: match[man] = woman from the partner words; also sums (position + 1)
+// over all women, which equals the total number of proposals of any
+// Gale-Shapley run (each man proposes down to his final partner exactly once).
+__global__ __launch_bounds__(256) void k_post(const unsigned long long* __restrict__ partner, int n,
+                                              int32_t* __restrict__ match, Scalars* sc) {
+  __shared__ unsigned long long red[4];
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long props = 0;
+  if (w < n) {
+    const unsigned long long v = partner[w];
+    if (v == kEmpty) {
+      atomicOr(&sc->error, kErrUnmatched);
+    } else {
+      const uint32_t man = pw_man(v);
+      if (man < (uint32_t)n) match[man] = w;
+      props = pw_pos(v) + 1ull;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) props += __shfl_down(props, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = props;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&sc->proposals, red[0] + red[1] + red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Host-callable launchers.
+
+hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long long v, hipStream_t s) {
+  int grid = (int)std::min<size_t>((count + 255) / 256, 4096);
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, s, p, count, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s) {
+  int grid = (int)std::min<size_t>((count + 255) / 256, 4096);
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid), dim3(256), 0, s, p, count, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_scalars(Scalars* sc, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_first_choices, dim3((n + 255) / 256), dim3(256), 0, s, pref_m, n, seen, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s) {
+  hipLaunchKernelGGL(k_first_match, dim3((n + 255) / 256), dim3(256), 0, s, pref_m, n, match);
+  return hipGetLastError();
+}
+
+// chunk: columns of one LDS pass (<= max LDS bytes / 4).
+hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
+                              int threads, Scalars* sc, hipStream_t s) {
+  const size_t lds = (size_t)chunk * sizeof(int32_t);
+  const bool vec = (n % 4 == 0) && (chunk % 4 == 0);
+  if (vec)
+    hipLaunchKernelGGL(k_invert_rows<4>, dim3(n), dim3(threads), lds, s, pref, rank, n, chunk,
+                       validate, sc);
+  else
+    hipLaunchKernelGGL(k_invert_rows<1>, dim3(n), dim3(threads), lds, s, pref, rank, n, chunk,
+                       validate, sc);
+  return hipGetLastError();
+}
+
+hipError_t set_invert_lds_limit(int bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_invert_rows<4>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_invert_rows<1>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
+                            Scalars* sc, hipStream_t s) {
+  const int men_blocks = (n + kGatherMen - 1) / kGatherMen;
+  const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
+  const long long tiles = (long long)men_blocks * col_tiles;
+  if (tiles > (1ll << 30)) return hipErrorInvalidValue;
+  const int per_xcd = (int)((tiles + 7) / 8);
+  hipLaunchKernelGGL(k_gather_pr, dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
+                     (int)tiles, per_xcd, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_row(const int32_t* pref_m, const int32_t* rank_w, int n, int m, int k0,
+                             int count, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_row, dim3((count + 255) / 256), dim3(256), 0, s, pref_m, rank_w, n, m,
+                     k0, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, partner, n, match, sc);
+  return hipGetLastError();
+}
+
+}  // namespace pactae

@@ -1,0 +1,36 @@
+// kernels.hpp — host-side declarations of the kernel launchers in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "smp_internal.hpp"
+
+namespace pactae {
+
+struct ProposeArgs {
+  const uint32_t* pr;
+  const int32_t* pref_m;
+  const int32_t* rank_w;
+  unsigned long long* partner;
+  int2* parked;
+  Scalars* sc;
+  int n, K, T;
+};
+
+hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s);
+hipError_t launch_reset_scalars(Scalars* sc, int n, hipStream_t s);
+hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars* sc, hipStream_t s);
+hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s);
+hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
+                              int threads, Scalars* sc, hipStream_t s);
+hipError_t set_invert_lds_limit(int bytes);
+hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
+                            Scalars* sc, hipStream_t s);
+hipError_t launch_gather_row(const int32_t* pref_m, const int32_t* rank_w, int n, int m, int k0,
+                             int count, uint32_t* out, hipStream_t s);
+hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
+hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
+                       hipStream_t s);
+
+}  // namespace pactae

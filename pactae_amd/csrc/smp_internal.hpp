@@ -1,0 +1,81 @@
+// smp_internal.hpp — definitions shared by the kernels, the engines and the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <stdint.h>
+#include <cstdio>
+#include <string>
+
+#include "../../include/smp_hip.h"
+
+#if defined(__HIPCC__)
+#define SMP_HD __host__ __device__ inline
+#else
+#define SMP_HD inline
+#endif
+
+namespace pactae {
+
+// ---------------------------------------------------------------------------
+// Men-major PR entry (the reference's PRNode {idx_, rank_}, include/utils/types.h:58-61,
+// packed into 32 bits): woman in the high half, her rank of this man in the low half.
+SMP_HD uint32_t pr_pack(uint32_t w, uint32_t r) { return (w << 16) | (r & 0xFFFFu); }
+SMP_HD uint32_t pr_woman(uint32_t e) { return e >> 16; }
+SMP_HD uint32_t pr_rank(uint32_t e) { return e & 0xFFFFu; }
+
+// Woman's partner word, claimed with a 64-bit atomicMin:
+//   bits 42..62 rank of the man in her list (decides the min: ranks are distinct)
+//   bits 21..41 the man
+//   bits  0..20 the man's list position of this woman (so a displaced man
+//               resumes at position + 1 without any per-man side array)
+// An unclaimed woman holds all ones, which is larger than every real word.
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kFieldBits = 21;
+constexpr uint64_t kFieldMask = (1ull << kFieldBits) - 1;
+SMP_HD uint64_t pw_pack(uint32_t rank, uint32_t man, uint32_t pos) {
+  return ((uint64_t)rank << 42) | ((uint64_t)man << 21) | (uint64_t)pos;
+}
+SMP_HD uint32_t pw_rank(uint64_t v) { return (uint32_t)(v >> 42); }
+SMP_HD uint32_t pw_man(uint64_t v) { return (uint32_t)((v >> 21) & kFieldMask); }
+SMP_HD uint32_t pw_pos(uint64_t v) { return (uint32_t)(v & kFieldMask); }
+
+// Device-side status flags (bit set in Scalars::error).
+enum : int {
+  kErrEntryRange = 1,   // a preference entry outside [0, n)
+  kErrNotPerm = 2,      // a woman's row is not a permutation
+  kErrListEnd = 4,      // a man ran past the end of his list (rows not permutations)
+  kErrUnmatched = 8,    // a woman left unclaimed at the end
+};
+
+// Small device-resident control block of one solve.
+struct Scalars {
+  int cursor;        // next never-started man (work queue over 0..n-1)
+  int free_count;    // men not held by any woman (started or not)
+  int parked_count;  // chains parked for the host tail
+  int error;         // kErr* bits
+  int dup_first;     // precheck: 1 if two men share a first choice
+  int pad[3];
+  unsigned long long atomics;    // atomicMin attempts
+  unsigned long long steps;      // segment scans
+  unsigned long long proposals;  // sum over women of (partner's list position + 1)
+};
+
+// Thread-local last error text for the C ABI.
+void set_error(const std::string& msg);
+const char* last_error();
+
+int generate(int kind, int n, uint64_t seed, int param, int32_t* pm, int32_t* pw, int threads);
+int batch_plan(int count, int n_min, int n_max, uint64_t seed, const double* mix3, int32_t* sizes,
+               int32_t* kinds);
+
+}  // namespace pactae
+
+#define SMP_HIP_TRY(expr)                                                                   \
+  do {                                                                                      \
+    hipError_t _st = (expr);                                                                \
+    if (_st != hipSuccess) {                                                                \
+      ::pactae::set_error(std::string(#expr) + " failed: " + hipGetErrorString(_st) + " (" + \
+                          __FILE__ + ":" + std::to_string(__LINE__) + ")");                 \
+      return SMP_ERR_HIP;                                                                   \
+    }                                                                                       \
+  } while (0)

@@ -1,0 +1,311 @@
+"""Python mirror of the reference's SMP engine interface over the C ABI.
+
+Reference shapes (paths relative to the reference repository root):
+  WorkloadType                 include/utils/types.h:85-90
+  GenerateWorkload             include/utils/generate_workloads_flat.h:250-280
+  SmpObj::CreateFromFlatLists  include/smp/smp.h:25-29
+  SmpEngineBamboo(smp, n)      include/smp/smp_engine_bamboo.h:13-119
+  FindStableMatching()         include/smp/smp_engine_abs.h:10-36
+  PrintProfilingInfo()         include/smp/smp_engine_abs.h:38-52
+  SmpEngineExecutor<E>(...)    benchmark/smp_engine_exec.h:6-32
+
+Errors from the library raise SmpError carrying the C status and message.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+
+_lib = _abi.load()
+
+
+class SmpError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        msg = _lib.smp_last_error().decode(errors="replace")
+        super().__init__(f"{where}: status {status}: {msg}")
+        self.status = status
+
+
+def _check(st: int, where: str) -> None:
+    if st != _abi.SMP_OK:
+        raise SmpError(st, where)
+
+
+def _i32p(a: np.ndarray):
+    return a.ctypes.data_as(_abi.i32p)
+
+
+class Workload(enum.IntEnum):
+    """Generators of include/smp_hip.h (enum smp_workload)."""
+    UNIFORM = 0
+    CONGESTED = 1
+    SHIFTED = 2
+    CORRELATED = 3
+    REF_RANDOM = 4
+    REF_CONGESTED = 5
+    REF_PERFECT = 6
+    REF_SOLO = 7
+
+
+class WorkloadType(enum.IntEnum):
+    """The reference's WorkloadType (types.h:85-90), mapped to generators below."""
+    PERFECT = 0
+    RANDOM = 1
+    CONGESTED = 2
+    SOLO = 3
+
+
+_REF_KIND = {
+    WorkloadType.PERFECT: Workload.REF_PERFECT,
+    WorkloadType.RANDOM: Workload.REF_RANDOM,
+    WorkloadType.CONGESTED: Workload.REF_CONGESTED,
+    WorkloadType.SOLO: Workload.REF_SOLO,
+}
+
+
+def generate(kind: int, n: int, seed: int = 0, param: int = 0, threads: int = 0):
+    """Seeded generator -> (pref_m, pref_w), each int32[n*n] (flat, row-major)."""
+    pm = np.empty(n * n, dtype=np.int32)
+    pw = np.empty(n * n, dtype=np.int32)
+    _check(_lib.smp_generate(int(kind), n, seed & (2**64 - 1), param, _i32p(pm), _i32p(pw), threads),
+           "smp_generate")
+    return pm, pw
+
+
+def generate_workload(workload_type: WorkloadType, n: int, group_size: int = 5, seed: int = 0):
+    """GenerateWorkload(type, n, M, W, group) with the reference's four workload types."""
+    return generate(_REF_KIND[WorkloadType(workload_type)], n, seed, group_size)
+
+
+def batch_plan(count: int, n_min: int, n_max: int, seed: int, mix=(0.6, 0.2, 0.2)):
+    sizes = np.empty(count, dtype=np.int32)
+    kinds = np.empty(count, dtype=np.int32)
+    mix_arr = (ctypes.c_double * 3)(*mix)
+    _check(_lib.smp_batch_plan(count, n_min, n_max, seed & (2**64 - 1), mix_arr, _i32p(sizes),
+                               _i32p(kinds)), "smp_batch_plan")
+    return sizes, kinds
+
+
+@dataclass
+class SmpObj:
+    """Owns the flat lists (SmpObj, smp.h:8-52)."""
+    flatten_pref_lists_m: np.ndarray
+    flatten_pref_lists_w: np.ndarray
+    n: int
+
+    @staticmethod
+    def create_from_flat_lists(flat_m, flat_w, n: int) -> "SmpObj":
+        m = np.ascontiguousarray(flat_m, dtype=np.int32).reshape(-1).copy()
+        w = np.ascontiguousarray(flat_w, dtype=np.int32).reshape(-1).copy()
+        if m.size != n * n or w.size != n * n:
+            raise ValueError(f"flat lists must hold n*n = {n * n} entries")
+        return SmpObj(m, w, n)
+
+
+class DeviceArray:
+    """int32 buffer in HBM allocated through the library (no other GPU runtime)."""
+
+    def __init__(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, dtype=np.int32)
+        self.nbytes = host.nbytes
+        self.ptr = ctypes.c_void_p()
+        _check(_lib.smp_device_alloc(self.nbytes, ctypes.byref(self.ptr)), "smp_device_alloc")
+        _check(_lib.smp_copy_h2d(self.ptr, host.ctypes.data_as(ctypes.c_void_p), self.nbytes),
+               "smp_copy_h2d")
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.nbytes // 4, dtype=np.int32)
+        _check(_lib.smp_copy_d2h(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes),
+               "smp_copy_d2h")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.smp_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def default_opts(**overrides) -> _abi.SmpOpts:
+    o = _abi.SmpOpts()
+    _lib.smp_default_opts(ctypes.byref(o))
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, int(v))
+    return o
+
+
+class SmpEngineBamboo:
+    """MI355X BambooSMP engine (SmpEngineBamboo, smp_engine_bamboo.h:13-119).
+
+    ``SmpEngineBamboo(smp, n)`` copies the host lists into HBM like the reference
+    constructor.  ``SmpEngineBamboo.from_device(d_m, d_w, n)`` borrows lists that
+    are already resident.
+    """
+
+    name = "BambooSMP-MI355X"
+
+    def __init__(self, smp: SmpObj | None, n: int, **opts):
+        self._h = ctypes.c_void_p()
+        self.n = n
+        self._keep = None
+        if smp is None:
+            return
+        o = default_opts(**opts)
+        _check(_lib.smp_create(_i32p(smp.flatten_pref_lists_m), _i32p(smp.flatten_pref_lists_w), n,
+                               ctypes.byref(o), ctypes.byref(self._h)), "smp_create")
+
+    @classmethod
+    def from_device(cls, d_pref_m: DeviceArray, d_pref_w: DeviceArray, n: int, **opts):
+        e = cls(None, n)
+        o = default_opts(**opts)
+        _check(_lib.smp_create_device(d_pref_m.ptr, d_pref_w.ptr, n, ctypes.byref(o),
+                                      ctypes.byref(e._h)), "smp_create_device")
+        e._keep = (d_pref_m, d_pref_w)
+        return e
+
+    def exec(self) -> None:
+        _check(_lib.smp_exec(self._h), "smp_exec")
+
+    def result(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        _check(_lib.smp_result(self._h, _i32p(out)), "smp_result")
+        return out
+
+    def find_stable_matching(self) -> np.ndarray:
+        """FindStableMatching(): one full solve, returns the man -> woman Matching."""
+        self.exec()
+        return self.result()
+
+    def profile(self) -> dict:
+        t = _abi.SmpTimes()
+        _check(_lib.smp_profile(self._h, ctypes.byref(t)), "smp_profile")
+        return t.as_dict()
+
+    def print_profiling_info(self) -> None:
+        """Same lines as AbsSmpEngine::PrintProfilingInfo (smp_engine_abs.h:38-44)."""
+        p = self.profile()
+        print(f"{self.name} Profiling Info:")
+        print(f"{self.name} Precheck Time: {p['precheck_ms']} ms")
+        print(f"{self.name} Init Time: {p['init_ms']} ms")
+        print(f"{self.name} Core Time: {p['core_ms']} ms")
+        print(f"{self.name} Total Time: {p['precheck_ms'] + p['init_ms'] + p['core_ms']} ms")
+        print("=" * 66)
+
+    def close(self) -> None:
+        if self._h:
+            _lib.smp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def smp_engine_executor(workload_type: WorkloadType, workload_size: int, group_size: int = 5,
+                        seed: int = 0, **opts) -> tuple[np.ndarray, dict]:
+    """SmpEngineExecutor<SmpEngineBamboo> (smp_engine_exec.h:6-32): generate, solve, report."""
+    pm, pw = generate_workload(workload_type, workload_size, group_size, seed)
+    smp = SmpObj.create_from_flat_lists(pm, pw, workload_size)
+    with SmpEngineBamboo(smp, workload_size, **opts) as eng:
+        extra = f", group_size: {group_size}" if workload_type == WorkloadType.RANDOM else ""
+        print(f"Executing {eng.name} with workload: {WorkloadType(workload_type).name.title()}, "
+              f"size: {workload_size}{extra}")
+        match = eng.find_stable_matching()
+        eng.print_profiling_info()
+        return match, eng.profile()
+
+
+class SmpBatch:
+    """Many independent instances solved in one persistent launch (smp_batch_*)."""
+
+    def __init__(self, sizes, pref_m_list, pref_w_list, **opts):
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        self.count = int(sizes.size)
+        self.sizes = sizes
+        sq = sizes.astype(np.int64) ** 2
+        self.offsets = np.zeros(self.count, dtype=np.int64)
+        if self.count > 1:
+            self.offsets[1:] = np.cumsum(sq)[:-1]
+        self.moffsets = np.zeros(self.count, dtype=np.int64)
+        if self.count > 1:
+            self.moffsets[1:] = np.cumsum(sizes.astype(np.int64))[:-1]
+        if isinstance(pref_m_list, np.ndarray) and pref_m_list.ndim == 1:
+            pm_all = np.ascontiguousarray(pref_m_list, dtype=np.int32)
+            pw_all = np.ascontiguousarray(pref_w_list, dtype=np.int32)
+        else:
+            pm_all = np.concatenate([np.asarray(a, dtype=np.int32).reshape(-1) for a in pref_m_list])
+            pw_all = np.concatenate([np.asarray(a, dtype=np.int32).reshape(-1) for a in pref_w_list])
+        if pm_all.size != int(sq.sum()) or pw_all.size != int(sq.sum()):
+            raise ValueError("concatenated lists do not match sizes")
+        o = _abi.SmpBatchOpts()
+        _lib.smp_batch_default_opts(ctypes.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, int(v))
+        self._h = ctypes.c_void_p()
+        _check(_lib.smp_batch_create(self.count, _i32p(sizes),
+                                     self.offsets.ctypes.data_as(_abi.i64p), _i32p(pm_all),
+                                     _i32p(pw_all), ctypes.byref(o), ctypes.byref(self._h)),
+               "smp_batch_create")
+
+    def exec(self) -> None:
+        _check(_lib.smp_batch_exec(self._h), "smp_batch_exec")
+
+    def exec_looped(self) -> None:
+        _check(_lib.smp_batch_exec_looped(self._h), "smp_batch_exec_looped")
+
+    def result(self) -> list[np.ndarray]:
+        out = np.empty(int(self.sizes.sum()), dtype=np.int32)
+        _check(_lib.smp_batch_result(self._h, _i32p(out)), "smp_batch_result")
+        return [out[o:o + n] for o, n in zip(self.moffsets, self.sizes)]
+
+    def profile(self) -> dict:
+        t = _abi.SmpBatchTimes()
+        _check(_lib.smp_batch_profile(self._h, ctypes.byref(t)), "smp_batch_profile")
+        return t.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            _lib.smp_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    c = ctypes.c_int32(0)
+    st = _lib.smp_device_count(ctypes.byref(c))
+    return int(c.value) if st == _abi.SMP_OK else 0
+
+
+def version() -> str:
+    return _lib.smp_version().decode()

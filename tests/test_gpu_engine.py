@@ -1,0 +1,118 @@
+"""GPU parity of the single-instance engine, through the C ABI, against the oracle.
+
+Bar: bit-exact man -> woman arrays (integer work), plus zero blocking pairs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+TAIL_CONFIGS = [
+    dict(tail_mode=1),                          # GPU finishes every chain
+    dict(tail_mode=2, tail_threshold=1),        # host takes the last free man
+    dict(tail_mode=2, tail_threshold=37),       # host takes the last 37
+    dict(tail_mode=1, prefix_cols=1),           # almost everything via the direct path
+    dict(tail_mode=2, tail_threshold=8, prefix_cols=64),
+    dict(tail_mode=1, prefix_cols=100000),      # full n x n PR matrix (reference layout)
+]
+
+
+def solve(smp, n, pm, pw, **opts):
+    obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+    with smp.SmpEngineBamboo(obj, n, **opts) as e:
+        m = e.find_stable_matching()
+        return m, e.profile()
+
+
+def test_golden_matchings(gpu, oracle):
+    smp = gpu
+    with open(os.path.join(HERE, "golden", "matchings.json")) as f:
+        rows = json.load(f)
+    for r in rows:
+        pm, pw = smp.generate(r["kind"], r["n"], seed=r["seed"])
+        for cfg in (TAIL_CONFIGS[0], TAIL_CONFIGS[2]):
+            got, prof = solve(smp, r["n"], pm, pw, **cfg)
+            assert got.tolist() == r["match"], (r["kind"], r["n"], r["seed"], cfg)
+            assert prof["proposals"] == r["proposals"]
+
+
+@pytest.mark.parametrize("cfg", TAIL_CONFIGS)
+@pytest.mark.parametrize("kind", range(8))
+def test_parity_mid_sizes(gpu, oracle, kind, cfg):
+    smp = gpu
+    for n in (1, 2, 3, 5, 63, 64, 65, 257, 1000):
+        if kind == 7 and n < 4:
+            continue
+        pm, pw = smp.generate(kind, n, seed=1000 + n)
+        want, t = oracle.gs_seq(n, pm, pw)
+        got, prof = solve(smp, n, pm, pw, **cfg)
+        assert np.array_equal(got, want), (kind, n, cfg)
+        assert prof["proposals"] == t["proposals"]
+
+
+def test_config1_uniform_2000(gpu, oracle):
+    smp = gpu
+    n = 2000
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=0x5EED0001)
+    want, _ = oracle.gs_seq(n, pm, pw)
+    for cfg in TAIL_CONFIGS:
+        got, _ = solve(smp, n, pm, pw, **cfg)
+        assert np.array_equal(got, want), cfg
+    assert oracle.count_blocking_pairs(n, pm, pw, want) == 0
+
+
+def test_repeated_exec_and_device_resident(gpu, oracle):
+    smp = gpu
+    n = 1500
+    pm, pw = smp.generate(smp.Workload.CORRELATED, n, seed=3)
+    want, _ = oracle.mw_seq(n, pm, pw)
+    dm, dw = smp.DeviceArray(pm), smp.DeviceArray(pw)
+    with smp.SmpEngineBamboo.from_device(dm, dw, n, tail_mode=2, tail_threshold=16) as e:
+        for _ in range(4):
+            assert np.array_equal(e.find_stable_matching(), want)
+
+
+def test_config2_uniform_20000(gpu, oracle):
+    smp = gpu
+    n = 20000
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=0x5EED0002)
+    want, t = oracle.gs_seq(n, pm, pw)
+    for cfg in (dict(tail_mode=1), dict(tail_mode=2, tail_threshold=16), dict(tail_mode=0)):
+        got, prof = solve(smp, n, pm, pw, **cfg)
+        assert np.array_equal(got, want), cfg
+        assert prof["proposals"] == t["proposals"]
+    assert oracle.count_blocking_pairs(n, pm, pw, want) == 0
+
+
+def test_config3_congested_and_shifted_20000(gpu, oracle):
+    smp = gpu
+    n = 20000
+    pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=0x5EED0003)
+    want = oracle.serial_dictatorship(n, pm[:n], pw)
+    got, prof = solve(smp, n, pm, pw, tail_mode=1)
+    assert np.array_equal(got, want)
+    assert prof["proposals"] == n * (n + 1) // 2
+    pm, pw = smp.generate(smp.Workload.SHIFTED, n, seed=0x5EED0003)
+    got, prof = solve(smp, n, pm, pw)
+    assert prof["is_perfect"] == 1
+    assert np.array_equal(got, pm.reshape(n, n)[:, 0])
+
+
+def test_invalid_lists_are_rejected(gpu):
+    smp = gpu
+    n = 50
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=1)
+    bad_w = pw.copy()
+    bad_w[7 * n + 3] = bad_w[7 * n + 4]  # woman 7's row is not a permutation
+    with pytest.raises(smp.SmpError) as ei:
+        solve(smp, n, pm, bad_w)
+    assert ei.value.status == 1
+    bad_m = pm.copy()
+    bad_m[5 * n + 2] = n + 3  # out of range entry
+    with pytest.raises(smp.SmpError) as ei:
+        solve(smp, n, bad_m, pw, precheck=0)
+    assert ei.value.status == 1
