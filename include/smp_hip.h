@@ -176,6 +176,65 @@ void smp_batch_destroy(smp_batch* b);
 int smp_batch_plan(int32_t count, int32_t n_min, int32_t n_max, uint64_t seed,
                    const double* mix3, int32_t* sizes, int32_t* kinds);
 
+/* ---- serving: a request loop with continuous batching -------------------- */
+typedef struct smp_server smp_server;
+
+enum smp_server_mode {
+  SMP_SERVE_BATCHED = 0, /* admitted requests are launched together, several launches
+                            in flight at once; new requests join the next free lane */
+  SMP_SERVE_PLAIN = 1    /* baseline: one request at a time through the plain
+                            single-instance pipeline                            */
+};
+
+typedef struct smp_server_opts {
+  int32_t device;
+  int32_t mode;             /* enum smp_server_mode                                   */
+  int32_t lanes;            /* launches in flight at once (streams); 0 = 3           */
+  int32_t max_batch;        /* requests per launch; 0 = 64                           */
+  int32_t max_n;            /* largest request accepted; 0 = 8192 (int16 ranks)      */
+  int32_t block_threads;    /* workgroup size of the serving kernel; 0 = auto        */
+  int64_t hbm_budget_bytes; /* device arena for request state; 0 = 4 GiB             */
+  int32_t reserved[8];
+} smp_server_opts;
+
+typedef struct smp_request_info {
+  double queue_ms;    /* submit -> launch                                        */
+  double service_ms;  /* launch -> completion seen by the server                 */
+  double latency_ms;  /* submit -> completion                                    */
+  int64_t proposals;
+  int32_t n;
+  int32_t batch_size; /* requests in the launch that carried this one            */
+  int32_t lane;
+  int32_t reserved[5];
+} smp_request_info;
+
+typedef struct smp_server_stats {
+  int64_t submitted, completed, launches;
+  int64_t arena_bytes, peak_arena_bytes;
+  int32_t peak_inflight;  /* admitted and not completed, maximum seen            */
+  int32_t peak_lanes_busy;
+  int32_t reserved[8];
+} smp_server_stats;
+
+void smp_server_default_opts(smp_server_opts* opts);
+int smp_server_create(const smp_server_opts* opts, smp_server** out);
+/* Submit one instance.  on_device = 0: host lists, copied into the arena at
+ * admission (the caller may reuse them once this returns).  on_device = 1: lists
+ * already in HBM, borrowed until the request completes.  Never blocks on the GPU;
+ * admission happens when the arena and a lane have room.  The result must be
+ * collected with smp_server_poll / smp_server_wait. */
+int smp_server_submit(smp_server* s, const int32_t* pref_m, const int32_t* pref_w, int32_t n,
+                      int32_t on_device, uint64_t* ticket);
+/* SMP_OK and the matching if the request finished (its ticket is then retired),
+ * SMP_PENDING otherwise. */
+int smp_server_poll(smp_server* s, uint64_t ticket, int32_t* match, smp_request_info* info);
+/* Wait up to timeout_ms (< 0: no limit) for a request. SMP_ERR_TIMEOUT on expiry. */
+int smp_server_wait(smp_server* s, uint64_t ticket, int32_t* match, smp_request_info* info,
+                    int32_t timeout_ms);
+int smp_server_stats_get(smp_server* s, smp_server_stats* st);
+/* Finishes in-flight work, then frees everything. */
+void smp_server_destroy(smp_server* s);
+
 /* ---- device memory helpers (so hosts need no other GPU runtime) ----------- */
 int smp_device_count(int32_t* count);
 int smp_device_alloc(size_t bytes, void** d_ptr);

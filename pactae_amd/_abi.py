@@ -99,6 +99,54 @@ class SmpBatchTimes(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class SmpServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("device", i32),
+        ("mode", i32),
+        ("lanes", i32),
+        ("max_batch", i32),
+        ("max_n", i32),
+        ("block_threads", i32),
+        ("hbm_budget_bytes", i64),
+        ("reserved", i32 * 8),
+    ]
+
+
+class SmpRequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("queue_ms", ctypes.c_double),
+        ("service_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+        ("proposals", i64),
+        ("n", i32),
+        ("batch_size", i32),
+        ("lane", i32),
+        ("reserved", i32 * 5),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class SmpServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", i64),
+        ("completed", i64),
+        ("launches", i64),
+        ("arena_bytes", i64),
+        ("peak_arena_bytes", i64),
+        ("peak_inflight", i32),
+        ("peak_lanes_busy", i32),
+        ("reserved", i32 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+SMP_SERVE_BATCHED = 0
+SMP_SERVE_PLAIN = 1
+
 # (name, restype, argtypes) for every entry point declared in include/smp_hip.h.
 SIGNATURES = [
     ("smp_generate", ctypes.c_int, [i32, i32, u64, i32, i32p, i32p, i32]),
@@ -120,6 +168,13 @@ SIGNATURES = [
     ("smp_batch_destroy", None, [vp]),
     ("smp_batch_plan", ctypes.c_int,
      [i32, i32, i32, u64, ctypes.POINTER(ctypes.c_double), i32p, i32p]),
+    ("smp_server_default_opts", None, [ctypes.POINTER(SmpServerOpts)]),
+    ("smp_server_create", ctypes.c_int, [ctypes.POINTER(SmpServerOpts), ctypes.POINTER(vp)]),
+    ("smp_server_submit", ctypes.c_int, [vp, vp, vp, i32, i32, ctypes.POINTER(u64)]),
+    ("smp_server_poll", ctypes.c_int, [vp, u64, i32p, ctypes.POINTER(SmpRequestInfo)]),
+    ("smp_server_wait", ctypes.c_int, [vp, u64, i32p, ctypes.POINTER(SmpRequestInfo), i32]),
+    ("smp_server_stats_get", ctypes.c_int, [vp, ctypes.POINTER(SmpServerStats)]),
+    ("smp_server_destroy", None, [vp]),
     ("smp_device_count", ctypes.c_int, [i32p]),
     ("smp_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(vp)]),
     ("smp_device_free", ctypes.c_int, [vp]),

@@ -7,8 +7,8 @@
 //     first, so the long instances start early and the short ones fill the end;
 //   * a workgroup solves its instance alone: it inverts the women's rows
 //     (rank[w][pref_w[w][r]] = r, the F9RankMatrixInitKernel step,
-//     src/smp/kernel.cu:672-686) into an int16 rank matrix, one row per wave staged
-//     in LDS; then its waves run proposal chains (the La3F9CoreKernel walk,
+//     src/smp/kernel.cu:672-686) into an int16 rank matrix, rows staged in LDS;
+//     then its waves run proposal chains (the La3F9CoreKernel walk,
 //     src/smp/kernel.cu:633-670) with the women's partner words in LDS and
 //     64-bit LDS atomicMin; then it writes the matching (PostProc,
 //     src/smp/smp_engine_bamboo.cu:299-303);
@@ -20,6 +20,7 @@
 #include <numeric>
 
 #include "engine.hpp"
+#include "wg_solver.hpp"
 
 namespace pactae {
 
@@ -37,135 +38,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchDesc* __restri
                                                        int16_t* __restrict__ rank_all,
                                                        int32_t* __restrict__ match_all,
                                                        int* next_inst, unsigned long long* proposals,
-                                                       int* error, int max_n) {
-  constexpr int NW = BLOCK / 64;
-  extern __shared__ unsigned long long smem_u64[];
-  unsigned long long* partner = smem_u64;                    // max_n words
-  int16_t* rowbuf = reinterpret_cast<int16_t*>(partner + max_n);  // NW rows of max_n
-  int* ctl = reinterpret_cast<int*>(rowbuf + NW * (size_t)max_n);
-  __shared__ unsigned long long red[NW];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                                       int* error, int max_n, int rows) {
+  extern __shared__ unsigned char smem[];
+  __shared__ int ctl[2];
+  __shared__ unsigned long long red[BLOCK / 64];
   int bad = 0;
-
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      ctl[0] = atomicAdd(next_inst, 1);
-      ctl[1] = 0;
-    }
+    if (threadIdx.x == 0) ctl[0] = atomicAdd(next_inst, 1);
     __syncthreads();
     const int inst = ctl[0];
     if (inst >= count) break;
     const BatchDesc d = desc[inst];
-    const int n = d.n;
-    const int32_t* pm = pm_all + d.off;
-    const int32_t* pw = pw_all + d.off;
-    int16_t* rk = rank_all + d.off;
-
-    // 1. women's rank rows, NW rows per round (one per wave).
-    int16_t* buf = rowbuf + (size_t)wave * max_n;
-    for (int base = 0; base < n; base += NW) {
-      const int row = base + wave;
-      if (row < n) {
-        for (int c = lane; c < n; c += 64) buf[c] = -1;
-      }
-      __syncthreads();
-      if (row < n) {
-        const int32_t* src = pw + (size_t)row * n;
-        for (int r = lane; r < n; r += 64) {
-          const int m = src[r];
-          if ((unsigned)m < (unsigned)n) buf[m] = (int16_t)r;
-          else bad |= kErrEntryRange;
-        }
-      }
-      __syncthreads();
-      if (row < n) {
-        int16_t* dst = rk + (size_t)row * n;
-        for (int c = lane; c < n; c += 64) {
-          const int16_t v = buf[c];
-          if (v < 0) bad |= kErrNotPerm;
-          dst[c] = v;
-        }
-      }
-    }
-    for (int w = threadIdx.x; w < n; w += BLOCK) partner[w] = kEmpty;
-    __syncthreads();
-
-    // 2. proposal chains, one per wave at a time.
-    int m = -1, k = 0;
-    for (;;) {
-      if (m < 0) {
-        int got = 0;
-        if (lane == 0) got = atomicAdd(&ctl[1], 1);
-        got = __shfl(got, 0);
-        if (got >= n) break;
-        m = got;
-        k = 0;
-      }
-      if (k >= n) {
-        bad |= kErrListEnd;
-        break;
-      }
-      const int kk = k + lane;
-      bool valid = kk < n;
-      uint32_t w = 0, r = 0;
-      if (valid) {
-        const int wi = pm[(size_t)m * n + kk];
-        if ((unsigned)wi < (unsigned)n) {
-          w = (uint32_t)wi;
-          r = (uint32_t)(uint16_t)rk[(size_t)w * n + m];
-        } else {
-          valid = false;
-          bad |= kErrEntryRange;
-        }
-      }
-      const unsigned long long mine = pw_pack(r, (uint32_t)m, (uint32_t)kk);
-      unsigned long long hint = 0;
-      if (valid) hint = __hip_atomic_load(&partner[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      unsigned long long cand = __ballot(valid && mine < hint);
-      bool moved = false;
-      while (cand) {
-        const int j = __ffsll((long long)cand) - 1;
-        unsigned long long old = 0;
-        if (lane == j) old = atomicMin(&partner[w], mine);
-        old = __shfl(old, j);
-        const unsigned long long mj = __shfl(mine, j);
-        if (old > mj) {
-          if (old == kEmpty) {
-            m = -1;
-          } else {
-            m = (int)pw_man(old);
-            k = (int)pw_pos(old) + 1;
-          }
-          moved = true;
-          break;
-        }
-        cand &= cand - 1;
-      }
-      if (!moved) k += 64;
-    }
-    __syncthreads();
-
-    // 3. result and proposal count.
+    InstanceRef ref{pm_all + d.off, pw_all + d.off, rank_all + d.off, match_all + d.moff, d.n};
     unsigned long long props = 0;
-    for (int w = threadIdx.x; w < n; w += BLOCK) {
-      const unsigned long long v = partner[w];
-      if (v == kEmpty) {
-        bad |= kErrUnmatched;
-      } else {
-        const uint32_t man = pw_man(v);
-        if (man < (uint32_t)n) match_all[d.moff + man] = w;
-        props += pw_pos(v) + 1ull;
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) props += __shfl_down(props, off);
-    if (lane == 0) red[wave] = props;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long s = 0;
-      for (int i = 0; i < NW; ++i) s += red[i];
-      atomicAdd(proposals, s);
-    }
+    bad |= wg_solve<BLOCK>(ref, smem, rows, max_n, ctl, red, &props);
+    if (threadIdx.x == 0) atomicAdd(proposals, props);
   }
   if (bad) atomicOr(error, bad);
 }
@@ -178,20 +66,16 @@ __global__ void k_batch_reset(int* next_inst, unsigned long long* proposals, int
   }
 }
 
-static size_t batch_lds_bytes(int block, int max_n) {
-  const int nw = block / 64;
-  return (size_t)max_n * 8 + (size_t)nw * max_n * 2 + 16;
-}
-
 template <int BLOCK>
 static hipError_t launch_batch_t(int grid, size_t lds, hipStream_t s, const BatchDesc* desc, int count,
                                  const int32_t* pm, const int32_t* pw, int16_t* rank, int32_t* match,
-                                 int* next_inst, unsigned long long* props, int* err, int max_n) {
+                                 int* next_inst, unsigned long long* props, int* err, int max_n,
+                                 int rows) {
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_solve<BLOCK>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_batch_solve<BLOCK>, dim3(grid), dim3(BLOCK), lds, s, desc, count, pm, pw, rank,
-                     match, next_inst, props, err, max_n);
+                     match, next_inst, props, err, max_n, rows);
   return hipGetLastError();
 }
 
@@ -220,7 +104,7 @@ struct smp_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  int block = 0, wpc = 0;
+  int block = 0, wpc = 0, rows = 0;
   bool have_result = false;
   smp_batch_times times{};
   std::unique_ptr<smp_engine> looped;  // single-instance engine for the looped baseline
@@ -242,34 +126,39 @@ struct smp_batch {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  hipError_t enqueue(int blk, int wgs_per_cu) {
+  hipError_t enqueue(int blk, int wgs_per_cu, int rows) {
     hipLaunchKernelGGL(pactae::k_batch_reset, dim3(1), dim3(64), 0, stream, d_ctl, d_props, d_ctl + 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int grid = std::max(1, std::min(count, dev.cus * wgs_per_cu));
-    const size_t lds = pactae::batch_lds_bytes(blk, max_n);
+    const size_t lds = pactae::wg_lds_bytes(rows, max_n);
     switch (blk) {
       case 256:
         return pactae::launch_batch_t<256>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank, d_match,
-                                           d_ctl, d_props, d_ctl + 1, max_n);
+                                           d_ctl, d_props, d_ctl + 1, max_n, rows);
       case 512:
         return pactae::launch_batch_t<512>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank, d_match,
-                                           d_ctl, d_props, d_ctl + 1, max_n);
+                                           d_ctl, d_props, d_ctl + 1, max_n, rows);
       default:
         return pactae::launch_batch_t<1024>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank,
-                                            d_match, d_ctl, d_props, d_ctl + 1, max_n);
+                                            d_match, d_ctl, d_props, d_ctl + 1, max_n, rows);
     }
   }
 
-  bool fits(int blk, int wgs_per_cu) const {
-    return pactae::batch_lds_bytes(blk, max_n) * wgs_per_cu <= 160 * 1024 &&
-           (int)pactae::batch_lds_bytes(blk, max_n) <= dev.max_lds;
+  // Most rows in flight (one per wave at most, a power of two) whose LDS fits
+  // `wgs_per_cu` workgroups in a CU's 160 KiB; 0 if none fits.
+  int rows_for(int blk, int wgs_per_cu) const {
+    for (int rows = blk / 64; rows >= 1; rows >>= 1) {
+      const size_t lds = pactae::wg_lds_bytes(rows, max_n) + 8 * (blk / 64) + 64;
+      if (lds * wgs_per_cu <= 160 * 1024 && (int)lds <= dev.max_lds) return rows;
+    }
+    return 0;
   }
 
   // One timed run of a configuration (events around the enqueued work).
-  int time_config(int blk, int wgs, float* ms) {
+  int time_config(int blk, int wgs, int rows, float* ms) {
     SMP_HIP_TRY(hipEventRecord(ev0, stream));
-    SMP_HIP_TRY(enqueue(blk, wgs));
+    SMP_HIP_TRY(enqueue(blk, wgs, rows));
     SMP_HIP_TRY(hipEventRecord(ev1, stream));
     SMP_HIP_TRY(hipStreamSynchronize(stream));
     SMP_HIP_TRY(hipEventElapsedTime(ms, ev0, ev1));
@@ -282,17 +171,18 @@ struct smp_batch {
     const int blocks[] = {256, 512, 1024};
     const int wgss[] = {1, 2, 4, 8};
     float best = 1e30f;
-    int bb = 0, bw = 0;
+    int bb = 0, bw = 0, br = 0;
     for (int blk : blocks) {
       if (opts.block_threads > 0 && blk != opts.block_threads) continue;
       for (int w : wgss) {
         if (opts.wgs_per_cu > 0 && w != opts.wgs_per_cu) continue;
-        if (blk * w > 2048 || !fits(blk, w)) continue;
+        const int rows = rows_for(blk, w);
+        if (blk * w > 2048 || rows == 0) continue;
         float t1 = 0, t2 = 0;
-        if (int st = time_config(blk, w, &t1)) return st;
-        if (int st = time_config(blk, w, &t2)) return st;
+        if (int st = time_config(blk, w, rows, &t1)) return st;
+        if (int st = time_config(blk, w, rows, &t2)) return st;
         const float t = std::min(t1, t2);
-        if (t < best) best = t, bb = blk, bw = w;
+        if (t < best) best = t, bb = blk, bw = w, br = rows;
       }
     }
     if (!bb) {
@@ -302,13 +192,14 @@ struct smp_batch {
     }
     block = bb;
     wpc = bw;
+    rows = br;
     return SMP_OK;
   }
 
   int build_graph() {
     if (!opts.use_graph) return SMP_OK;
     SMP_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue(block, wpc);
+    hipError_t e = enqueue(block, wpc, rows);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -422,7 +313,7 @@ int smp_batch_exec(smp_batch* b) {
   if (b->graph_exec)
     SMP_HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
   else
-    SMP_HIP_TRY(b->enqueue(b->block, b->wpc));
+    SMP_HIP_TRY(b->enqueue(b->block, b->wpc, b->rows));
   SMP_HIP_TRY(hipEventRecord(b->ev1, b->stream));
   SMP_HIP_TRY(hipStreamSynchronize(b->stream));
   b->times = smp_batch_times{};

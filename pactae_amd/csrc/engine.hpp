@@ -325,7 +325,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (int st = read_scalars()) return st;
     if (int st = check_device_error("proposal kernel")) return st;
     times.atomics = (int64_t)h_sc->atomics;
-    const int started = std::min(h_sc->cursor, n);
+    // Men [0, waves) were taken by the waves' first iterations, the rest from the cursor.
+    const int started = (int)std::min<long long>(n, (long long)grid * 4 + h_sc->cursor);
     const int handoff = h_sc->parked_count + (n - started);
     times.handoff_free = handoff;
     if (handoff > 0) {

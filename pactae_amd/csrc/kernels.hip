@@ -112,8 +112,9 @@ __global__ __launch_bounds__(1024) void k_invert_rows(const int32_t* __restrict_
       __syncthreads();
       if constexpr (VEC == 4) {
         int4* dst4 = reinterpret_cast<int4*>(dst + c0);
+        const int4* lds4 = reinterpret_cast<const int4*>(lds);
         for (int q = tid; q < (clen >> 2); q += bs) {
-          int4 v = make_int4(lds[4 * q], lds[4 * q + 1], lds[4 * q + 2], lds[4 * q + 3]);
+          int4 v = lds4[q];
           if (validate && (v.x < 0 || v.y < 0 || v.z < 0 || v.w < 0)) bad |= 2;
           dst4[q] = v;
         }
@@ -201,13 +202,21 @@ __global__ void k_gather_row(const int32_t* __restrict__ pref_m, const int32_t* 
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   const int lane = threadIdx.x & 63;
   const int n = a.n, K = a.K, T = a.T;
+  // Wave g starts with man g; later men come from the cursor, offset by the
+  // number of waves, so the first round needs no atomic.
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  const int gwave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   int m = -1, k = 0;
+  bool first = true;
   unsigned long long n_atomics = 0, n_steps = 0;
   for (;;) {
     if (m < 0) {
-      int got = 0;
-      if (lane == 0) got = atomicAdd(&a.sc->cursor, 1);
-      got = __shfl(got, 0);
+      int got = gwave;
+      if (!first) {
+        if (lane == 0) got = waves + atomicAdd(&a.sc->cursor, 1);
+        got = __shfl(got, 0);
+      }
+      first = false;
       if (got >= n) break;
       m = got;
       k = 0;

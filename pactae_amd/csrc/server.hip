@@ -1,0 +1,617 @@
+// server.hip — request-serving loop with continuous batching (no reference
+// counterpart; the reference runs one instance per executor call,
+// benchmark/smp_engine_exec.h:6-32).
+//
+// Requests are admitted in arrival order when the HBM arena has room for their
+// state (int16 rank matrix, plus a copy of the lists for host-submitted
+// requests).  Admitted requests are launched together on whichever launch lane
+// (stream) is free; with several lanes, a request that arrives while others are on
+// the GPU starts as soon as a lane and memory free up, and its kernels overlap
+// with the ones already running.  One launch is two kernels:
+//   k_serve_rank    all women's rank rows of the launch, one row per workgroup
+//                   over a flat row index (so a large request is spread over the
+//                   whole GPU), staged in LDS, written as int16;
+//   k_serve_chains  one workgroup per request: proposal chains with the partner
+//                   words in LDS, then the matching, written straight into pinned
+//                   host memory.
+// The plain mode is the baseline: one request at a time through the plain
+// single-instance pipeline (engine.hpp).
+
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "engine.hpp"
+#include "wg_solver.hpp"
+
+namespace pactae {
+
+struct ServeReq {
+  const int32_t* pm;
+  const int32_t* pw;
+  int16_t* rank;
+  int32_t* match;  // pinned host memory (device-visible)
+  int n;
+  int row0;        // first flat row index of this request in the launch
+};
+
+__global__ __launch_bounds__(256) void k_serve_rank(const ServeReq* __restrict__ reqs, int count,
+                                                    int total_rows, int* errs) {
+  extern __shared__ int16_t rbuf[];
+  for (int g = blockIdx.x; g < total_rows; g += gridDim.x) {
+    int lo = 0, hi = count - 1;  // last request with row0 <= g
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (reqs[mid].row0 <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const ServeReq q = reqs[lo];
+    const int n = q.n, row = g - q.row0;
+    int bad = 0;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) rbuf[c] = -1;
+    __syncthreads();
+    const int32_t* src = q.pw + (size_t)row * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const int m = src[r];
+      if ((unsigned)m < (unsigned)n) rbuf[m] = (int16_t)r;
+      else bad |= kErrEntryRange;
+    }
+    __syncthreads();
+    int16_t* dst = q.rank + (size_t)row * n;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      const int16_t v = rbuf[c];
+      if (v < 0) bad |= kErrNotPerm;
+      dst[c] = v;
+    }
+    __syncthreads();
+    if (bad) atomicOr(&errs[lo], bad);
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_serve_chains(const ServeReq* __restrict__ reqs, int count,
+                                                        unsigned long long* props, int* errs) {
+  extern __shared__ unsigned long long partner[];
+  __shared__ int cursor;
+  __shared__ unsigned long long red[BLOCK / 64];
+  const int i = blockIdx.x;
+  if (i >= count) return;
+  const ServeReq q = reqs[i];
+  InstanceRef ref{q.pm, q.pw, q.rank, q.match, q.n};
+  unsigned long long p = 0;
+  const int bad = wg_chains<BLOCK>(ref, partner, &cursor, red, &p);
+  if (bad) atomicOr(&errs[i], bad);
+  if (threadIdx.x == 0) props[i] = p;
+}
+
+// First-fit arena over one device allocation, 256-byte granules.
+class Arena {
+ public:
+  void init(size_t bytes) {
+    size_ = bytes;
+    free_.clear();
+    free_[0] = bytes;
+    used_ = peak_ = 0;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = (bytes + 255) & ~size_t(255);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes, start = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[start] = rest;
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+  size_t size() const { return size_; }
+
+ private:
+  std::map<size_t, size_t> free_;
+  size_t size_ = 0, used_ = 0, peak_ = 0;
+};
+
+}  // namespace pactae
+
+using pactae::Clock;
+
+struct smp_server {
+  enum State { kPending, kReady, kLaunched, kDone, kFailed };
+  struct Request {
+    uint64_t ticket = 0;
+    int n = 0;
+    bool on_device = false;
+    std::vector<int32_t> host_pm, host_pw;
+    const int32_t* d_pm = nullptr;
+    const int32_t* d_pw = nullptr;
+    size_t off = 0, bytes = 0;
+    bool has_block = false;
+    State state = kPending;
+    int status = SMP_OK;
+    std::string error;
+    Clock::time_point t_submit, t_launch, t_done;
+    std::vector<int32_t> result;
+    int64_t proposals = 0;
+    int batch_size = 0, lane = -1;
+  };
+  struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<Request*> reqs;
+    pactae::ServeReq* h_reqs = nullptr;  // pinned
+    pactae::ServeReq* d_reqs = nullptr;
+    int32_t* h_match = nullptr;          // pinned results, max_batch * max_n
+    unsigned long long* d_props = nullptr;
+    int* d_errs = nullptr;
+    unsigned long long* h_props = nullptr;
+    int* h_errs = nullptr;
+  };
+
+  smp_server_opts opts{};
+  pactae::DeviceInfo dev;
+  pactae::Arena arena;
+  char* d_arena = nullptr;
+  std::vector<Lane> lanes;
+  hipStream_t copy_stream = nullptr;
+  std::unique_ptr<smp_engine> plain;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> pending, ready;
+  std::map<uint64_t, std::unique_ptr<Request>> table;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  bool fatal = false;
+  std::string fatal_msg;
+  std::thread worker;
+  smp_server_stats stats{};
+  int inflight = 0;
+
+  size_t footprint(const Request& r) const {
+    const size_t nn = (size_t)r.n * r.n;
+    size_t b = ((nn * 2 + 255) & ~size_t(255));            // int16 rank matrix
+    if (!r.on_device) b += 2 * ((nn * 4 + 255) & ~size_t(255));  // copies of both lists
+    return b;
+  }
+
+  int setup() {
+    if (int st = pactae::query_device(opts.device, &dev)) return st;
+    if (hipMalloc(&d_arena, (size_t)opts.hbm_budget_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      pactae::set_error("server: arena allocation of " + std::to_string(opts.hbm_budget_bytes) +
+                        " bytes failed");
+      return SMP_ERR_NOMEM;
+    }
+    arena.init((size_t)opts.hbm_budget_bytes);
+    SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+    if (opts.mode == SMP_SERVE_PLAIN) {
+      smp_opts o;
+      smp_default_opts(&o);
+      o.device = opts.device;
+      o.tail_mode = SMP_TAIL_GPU;
+      auto e = std::make_unique<smp_engine>();
+      e->n = opts.max_n;
+      e->opts = o;
+      if (int st = e->setup()) return st;
+      plain = std::move(e);
+    } else {
+      lanes.resize(opts.lanes);
+      const size_t mb = (size_t)opts.max_batch;
+      for (auto& L : lanes) {
+        SMP_HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        SMP_HIP_TRY(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+        SMP_HIP_TRY(hipHostMalloc(&L.h_reqs, mb * sizeof(pactae::ServeReq), hipHostMallocDefault));
+        SMP_HIP_TRY(hipHostMalloc(&L.h_match, mb * opts.max_n * sizeof(int32_t), hipHostMallocDefault));
+        SMP_HIP_TRY(hipHostMalloc(&L.h_props, mb * sizeof(unsigned long long), hipHostMallocDefault));
+        SMP_HIP_TRY(hipHostMalloc(&L.h_errs, mb * sizeof(int), hipHostMallocDefault));
+        SMP_HIP_TRY(hipMalloc(&L.d_reqs, mb * sizeof(pactae::ServeReq)));
+        SMP_HIP_TRY(hipMalloc(&L.d_props, mb * sizeof(unsigned long long)));
+        SMP_HIP_TRY(hipMalloc(&L.d_errs, mb * sizeof(int)));
+      }
+      const int lds = (int)std::min<size_t>((size_t)opts.max_n * 8, 160 * 1024);
+      SMP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&pactae::k_serve_chains<256>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      SMP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&pactae::k_serve_chains<512>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    }
+    worker = std::thread([this] { loop(); });
+    return SMP_OK;
+  }
+
+  ~smp_server() {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop = true;
+    }
+    cv_work.notify_all();
+    if (worker.joinable()) worker.join();
+    for (auto& L : lanes) {
+      if (L.stream) (void)hipStreamSynchronize(L.stream);
+      (void)hipHostFree(L.h_reqs);
+      (void)hipHostFree(L.h_match);
+      (void)hipHostFree(L.h_props);
+      (void)hipHostFree(L.h_errs);
+      (void)hipFree(L.d_reqs);
+      (void)hipFree(L.d_props);
+      (void)hipFree(L.d_errs);
+      if (L.done) (void)hipEventDestroy(L.done);
+      if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    plain.reset();
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    (void)hipFree(d_arena);
+  }
+
+  void fail_all(const std::string& msg) {  // with mu held
+    fatal = true;
+    fatal_msg = msg;
+    for (auto& kv : table) {
+      Request* r = kv.second.get();
+      if (r->state != kDone && r->state != kFailed) {
+        r->state = kFailed;
+        r->status = SMP_ERR_HIP;
+        r->error = msg;
+      }
+    }
+    pending.clear();
+    ready.clear();
+    cv_done.notify_all();
+  }
+
+  void finish(Request* r, int status, const std::string& err) {  // with mu held
+    r->t_done = Clock::now();
+    r->status = status;
+    r->error = err;
+    r->state = status == SMP_OK ? kDone : kFailed;
+    if (r->has_block) arena.release(r->off, r->bytes), r->has_block = false;
+    --inflight;
+    ++stats.completed;
+    cv_done.notify_all();
+  }
+
+  // Admit pending requests in arrival order while the arena has room.  Copies of
+  // host lists are made here (outside the lock).  Returns a HIP status.
+  hipError_t admit(std::unique_lock<std::mutex>& lk, bool* progressed) {
+    while (!pending.empty()) {
+      Request* r = pending.front();
+      r->bytes = footprint(*r);
+      if (!arena.alloc(r->bytes, &r->off)) break;
+      r->has_block = true;
+      pending.pop_front();
+      ++inflight;
+      stats.peak_inflight = std::max<int32_t>(stats.peak_inflight, inflight);
+      *progressed = true;
+      if (!r->on_device) {
+        const size_t nn = (size_t)r->n * r->n;
+        const size_t rank_b = (nn * 2 + 255) & ~size_t(255);
+        const size_t list_b = (nn * 4 + 255) & ~size_t(255);
+        int32_t* dm = reinterpret_cast<int32_t*>(d_arena + r->off + rank_b);
+        int32_t* dw = reinterpret_cast<int32_t*>(d_arena + r->off + rank_b + list_b);
+        lk.unlock();
+        hipError_t e = hipMemcpyAsync(dm, r->host_pm.data(), nn * 4, hipMemcpyHostToDevice, copy_stream);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(dw, r->host_pw.data(), nn * 4, hipMemcpyHostToDevice, copy_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(copy_stream);
+        lk.lock();
+        if (e != hipSuccess) return e;
+        r->d_pm = dm;
+        r->d_pw = dw;
+        std::vector<int32_t>().swap(r->host_pm);
+        std::vector<int32_t>().swap(r->host_pw);
+      }
+      r->state = kReady;
+      ready.push_back(r);
+    }
+    return hipSuccess;
+  }
+
+  hipError_t launch(Lane& L, int lane_id) {  // with mu held
+    const int count = std::min<int>((int)ready.size(), opts.max_batch);
+    L.reqs.assign(ready.begin(), ready.begin() + count);
+    ready.erase(ready.begin(), ready.begin() + count);
+    int rows = 0, bmax = 0;
+    const auto now = Clock::now();
+    for (int i = 0; i < count; ++i) {
+      Request* r = L.reqs[i];
+      r->state = kLaunched;
+      r->t_launch = now;
+      r->batch_size = count;
+      r->lane = lane_id;
+      int32_t* hm = L.h_match + (size_t)i * opts.max_n;
+      int32_t* dm = nullptr;
+      hipError_t e = hipHostGetDevicePointer(reinterpret_cast<void**>(&dm), hm, 0);
+      if (e != hipSuccess) return e;
+      L.h_reqs[i] = pactae::ServeReq{r->d_pm, r->d_pw, reinterpret_cast<int16_t*>(d_arena + r->off), dm,
+                                     r->n, rows};
+      rows += r->n;
+      bmax = std::max(bmax, r->n);
+    }
+    hipError_t e = hipMemcpyAsync(L.d_reqs, L.h_reqs, count * sizeof(pactae::ServeReq),
+                                  hipMemcpyHostToDevice, L.stream);
+    if (e != hipSuccess) return e;
+    if ((e = pactae::launch_fill_i32(L.d_errs, count, 0, L.stream)) != hipSuccess) return e;
+    const int rank_grid = std::min(rows, dev.cus * 8);
+    hipLaunchKernelGGL(pactae::k_serve_rank, dim3(rank_grid), dim3(256), (size_t)bmax * sizeof(int16_t),
+                       L.stream, L.d_reqs, count, rows, L.d_errs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const size_t lds = (size_t)bmax * 8;
+    if (opts.block_threads == 256)
+      hipLaunchKernelGGL(pactae::k_serve_chains<256>, dim3(count), dim3(256), lds, L.stream, L.d_reqs,
+                         count, L.d_props, L.d_errs);
+    else
+      hipLaunchKernelGGL(pactae::k_serve_chains<512>, dim3(count), dim3(512), lds, L.stream, L.d_reqs,
+                         count, L.d_props, L.d_errs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(L.h_props, L.d_props, count * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, L.stream)) != hipSuccess)
+      return e;
+    if ((e = hipMemcpyAsync(L.h_errs, L.d_errs, count * sizeof(int), hipMemcpyDeviceToHost, L.stream)) !=
+        hipSuccess)
+      return e;
+    if ((e = hipEventRecord(L.done, L.stream)) != hipSuccess) return e;
+    L.busy = true;
+    ++stats.launches;
+    return hipSuccess;
+  }
+
+  void complete(Lane& L) {  // with mu held; the lane's event has fired
+    for (size_t i = 0; i < L.reqs.size(); ++i) {
+      Request* r = L.reqs[i];
+      const int32_t* hm = L.h_match + i * opts.max_n;
+      r->result.assign(hm, hm + r->n);
+      r->proposals = (int64_t)L.h_props[i];
+      if (L.h_errs[i])
+        finish(r, SMP_ERR_INVALID, "invalid preference lists (device flags " + std::to_string(L.h_errs[i]) + ")");
+      else
+        finish(r, SMP_OK, "");
+    }
+    L.reqs.clear();
+    L.busy = false;
+  }
+
+  void loop() {
+    (void)hipSetDevice(opts.device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      bool progressed = false;
+      if (!fatal && opts.mode == SMP_SERVE_PLAIN) {
+        if (hipError_t e = admit(lk, &progressed); e != hipSuccess) {
+          fail_all(std::string("server: ") + hipGetErrorString(e));
+          continue;
+        }
+        if (!ready.empty()) {
+          Request* r = ready.front();
+          ready.pop_front();
+          r->state = kLaunched;
+          r->t_launch = Clock::now();
+          r->batch_size = 1;
+          r->lane = 0;
+          ++stats.launches;
+          lk.unlock();
+          int st = plain->run_view(r->d_pm, r->d_pw, r->n, nullptr);
+          std::vector<int32_t> res;
+          int64_t props = 0;
+          std::string err;
+          if (st == SMP_OK) {
+            res.assign(plain->h_match.begin(), plain->h_match.begin() + r->n);
+            props = plain->times.proposals;
+          } else {
+            err = pactae::last_error();
+          }
+          lk.lock();
+          r->result = std::move(res);
+          r->proposals = props;
+          finish(r, st, err);
+          progressed = true;
+        }
+      } else if (!fatal) {
+        int busy = 0;
+        for (auto& L : lanes) {
+          if (L.busy) {
+            hipError_t q = hipEventQuery(L.done);
+            if (q == hipSuccess) {
+              complete(L);
+              progressed = true;
+            } else if (q != hipErrorNotReady) {
+              fail_all(std::string("server: ") + hipGetErrorString(q));
+              break;
+            }
+          }
+        }
+        if (hipError_t e = admit(lk, &progressed); e != hipSuccess) {
+          fail_all(std::string("server: ") + hipGetErrorString(e));
+          continue;
+        }
+        for (size_t li = 0; li < lanes.size() && !ready.empty(); ++li) {
+          if (lanes[li].busy) continue;
+          if (hipError_t e = launch(lanes[li], (int)li); e != hipSuccess) {
+            fail_all(std::string("server launch: ") + hipGetErrorString(e));
+            break;
+          }
+          progressed = true;
+        }
+        for (auto& L : lanes) busy += L.busy;
+        stats.peak_lanes_busy = std::max<int32_t>(stats.peak_lanes_busy, busy);
+        if (busy && !progressed) {
+          lk.unlock();
+          std::this_thread::sleep_for(std::chrono::microseconds(20));
+          lk.lock();
+          continue;
+        }
+      }
+      stats.arena_bytes = (int64_t)arena.used();
+      stats.peak_arena_bytes = (int64_t)arena.peak();
+      const bool idle = pending.empty() && ready.empty() && inflight == 0;
+      if (stop && (idle || fatal)) break;
+      if (!progressed) cv_work.wait_for(lk, std::chrono::milliseconds(2));
+    }
+  }
+};
+
+extern "C" {
+
+void smp_server_default_opts(smp_server_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->device = 0;
+  o->mode = SMP_SERVE_BATCHED;
+  o->lanes = 3;
+  o->max_batch = 64;
+  o->max_n = 8192;
+  o->block_threads = 0;
+  o->hbm_budget_bytes = 4ll << 30;
+}
+
+int smp_server_create(const smp_server_opts* opts, smp_server** out) {
+  if (!out) return SMP_ERR_INVALID;
+  *out = nullptr;
+  auto s = std::make_unique<smp_server>();
+  if (opts)
+    s->opts = *opts;
+  else
+    smp_server_default_opts(&s->opts);
+  auto& o = s->opts;
+  if (o.lanes <= 0) o.lanes = 3;
+  if (o.max_batch <= 0) o.max_batch = 64;
+  if (o.max_n <= 0) o.max_n = 8192;
+  if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = 4ll << 30;
+  if (o.block_threads != 256 && o.block_threads != 512) o.block_threads = 512;
+  if (o.max_n > 20000 || o.lanes > 16 || o.max_batch > 4096 ||
+      (o.mode != SMP_SERVE_BATCHED && o.mode != SMP_SERVE_PLAIN)) {
+    pactae::set_error("server: max_n must be <= 20000 (LDS partner words), lanes <= 16, max_batch <= 4096");
+    return SMP_ERR_INVALID;
+  }
+  if (int st = s->setup()) return st;
+  *out = s.release();
+  return SMP_OK;
+}
+
+int smp_server_submit(smp_server* s, const int32_t* pm, const int32_t* pw, int32_t n, int32_t on_device,
+                      uint64_t* ticket) {
+  if (!s || !pm || !pw || !ticket || n <= 0 || n > s->opts.max_n) {
+    pactae::set_error("server submit: bad arguments (n must be in [1, max_n])");
+    return SMP_ERR_INVALID;
+  }
+  auto r = std::make_unique<smp_server::Request>();
+  r->n = n;
+  r->on_device = on_device != 0;
+  r->t_submit = Clock::now();
+  if (r->on_device) {
+    r->d_pm = pm;
+    r->d_pw = pw;
+  } else {
+    r->host_pm.assign(pm, pm + (size_t)n * n);
+    r->host_pw.assign(pw, pw + (size_t)n * n);
+  }
+  if (s->footprint(*r) > s->arena.size()) {
+    pactae::set_error("server submit: request footprint exceeds the HBM budget");
+    return SMP_ERR_NOMEM;
+  }
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->fatal) {
+      pactae::set_error(s->fatal_msg);
+      return SMP_ERR_HIP;
+    }
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    s->pending.push_back(r.get());
+    s->table.emplace(r->ticket, std::move(r));
+    ++s->stats.submitted;
+  }
+  s->cv_work.notify_one();
+  return SMP_OK;
+}
+
+static int collect(smp_server* s, std::map<uint64_t, std::unique_ptr<smp_server::Request>>::iterator it,
+                   int32_t* match, smp_request_info* info) {
+  smp_server::Request* r = it->second.get();
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    info->queue_ms = std::chrono::duration<double, std::milli>(r->t_launch - r->t_submit).count();
+    info->service_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_launch).count();
+    info->latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    info->proposals = r->proposals;
+    info->n = r->n;
+    info->batch_size = r->batch_size;
+    info->lane = r->lane;
+  }
+  const int st = r->status;
+  if (st == SMP_OK && match) std::memcpy(match, r->result.data(), sizeof(int32_t) * r->n);
+  if (st != SMP_OK) pactae::set_error(r->error);
+  s->table.erase(it);
+  return st;
+}
+
+int smp_server_poll(smp_server* s, uint64_t ticket, int32_t* match, smp_request_info* info) {
+  if (!s) return SMP_ERR_INVALID;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->table.find(ticket);
+  if (it == s->table.end()) {
+    pactae::set_error("server poll: unknown ticket");
+    return SMP_ERR_INVALID;
+  }
+  const auto st = it->second->state;
+  if (st != smp_server::kDone && st != smp_server::kFailed) return SMP_PENDING;
+  return collect(s, it, match, info);
+}
+
+int smp_server_wait(smp_server* s, uint64_t ticket, int32_t* match, smp_request_info* info,
+                    int32_t timeout_ms) {
+  if (!s) return SMP_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->table.find(ticket);
+  if (it == s->table.end()) {
+    pactae::set_error("server wait: unknown ticket");
+    return SMP_ERR_INVALID;
+  }
+  smp_server::Request* r = it->second.get();
+  auto done = [&] { return r->state == smp_server::kDone || r->state == smp_server::kFailed; };
+  if (timeout_ms < 0) {
+    s->cv_done.wait(lk, done);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) {
+    pactae::set_error("server wait: timed out");
+    return SMP_ERR_TIMEOUT;
+  }
+  return collect(s, it, match, info);
+}
+
+int smp_server_stats_get(smp_server* s, smp_server_stats* st) {
+  if (!s || !st) return SMP_ERR_INVALID;
+  std::lock_guard<std::mutex> g(s->mu);
+  *st = s->stats;
+  st->arena_bytes = (int64_t)s->arena.used();
+  st->peak_arena_bytes = (int64_t)s->arena.peak();
+  return SMP_OK;
+}
+
+void smp_server_destroy(smp_server* s) { delete s; }
+
+}  // extern "C"

@@ -301,6 +301,87 @@ class SmpBatch:
             pass
 
 
+class SmpServer:
+    """Request-serving loop (smp_server_*): continuous batching or the plain baseline.
+
+    submit() never blocks on the GPU; wait()/poll() return (match, info)."""
+
+    BATCHED = _abi.SMP_SERVE_BATCHED
+    PLAIN = _abi.SMP_SERVE_PLAIN
+
+    def __init__(self, **opts):
+        o = _abi.SmpServerOpts()
+        _lib.smp_server_default_opts(ctypes.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, int(v))
+        self.max_n = o.max_n
+        self._h = ctypes.c_void_p()
+        _check(_lib.smp_server_create(ctypes.byref(o), ctypes.byref(self._h)), "smp_server_create")
+        self._n = {}
+        self._keep = {}
+
+    def submit(self, pref_m, pref_w, n: int) -> int:
+        """Host lists (numpy int32, copied) or DeviceArray pairs (borrowed)."""
+        t = ctypes.c_uint64()
+        if isinstance(pref_m, DeviceArray):
+            st = _lib.smp_server_submit(self._h, pref_m.ptr, pref_w.ptr, n, 1, ctypes.byref(t))
+            keep = (pref_m, pref_w)
+        else:
+            pm = np.ascontiguousarray(pref_m, dtype=np.int32).reshape(-1)
+            pw = np.ascontiguousarray(pref_w, dtype=np.int32).reshape(-1)
+            st = _lib.smp_server_submit(self._h, pm.ctypes.data_as(ctypes.c_void_p),
+                                        pw.ctypes.data_as(ctypes.c_void_p), n, 0, ctypes.byref(t))
+            keep = None
+        _check(st, "smp_server_submit")
+        self._n[t.value] = n
+        if keep is not None:
+            self._keep[t.value] = keep
+        return t.value
+
+    def _finish(self, ticket, st, out, info):
+        self._n.pop(ticket, None)
+        self._keep.pop(ticket, None)
+        _check(st, "smp_server request")
+        return out, info.as_dict()
+
+    def poll(self, ticket: int):
+        out = np.empty(self._n[ticket], dtype=np.int32)
+        info = _abi.SmpRequestInfo()
+        st = _lib.smp_server_poll(self._h, ticket, _i32p(out), ctypes.byref(info))
+        if st == _abi.SMP_PENDING:
+            return None
+        return self._finish(ticket, st, out, info)
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        out = np.empty(self._n[ticket], dtype=np.int32)
+        info = _abi.SmpRequestInfo()
+        st = _lib.smp_server_wait(self._h, ticket, _i32p(out), ctypes.byref(info), timeout_ms)
+        return self._finish(ticket, st, out, info)
+
+    def stats(self) -> dict:
+        s = _abi.SmpServerStats()
+        _check(_lib.smp_server_stats_get(self._h, ctypes.byref(s)), "smp_server_stats_get")
+        return s.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            _lib.smp_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_count() -> int:
     c = ctypes.c_int32(0)
     st = _lib.smp_device_count(ctypes.byref(c))

@@ -57,3 +57,16 @@ def test_errors_are_reported_not_thrown(smp):
 
 def test_version(smp):
     assert "gfx950" in smp.version()
+
+
+def test_server_struct_layouts(tmp_path):
+    from pactae_amd import _abi
+    src = tmp_path / "sz2.c"
+    src.write_text('#include "smp_hip.h"\n#include <stdio.h>\n'
+                   'int main(void){printf("%zu %zu %zu\\n", sizeof(smp_server_opts), sizeof(smp_request_info),'
+                   ' sizeof(smp_server_stats)); return 0;}\n')
+    exe = tmp_path / "sz2"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert [int(x) for x in out] == [ctypes.sizeof(_abi.SmpServerOpts), ctypes.sizeof(_abi.SmpRequestInfo),
+                                     ctypes.sizeof(_abi.SmpServerStats)]

@@ -1,0 +1,79 @@
+"""Serving loop: each request's result is bit-exact against the oracle, and the
+same alone as in a concurrent mix (tolerance 0); the plain baseline agrees."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def requests(smp, count, seed, n_min=16, n_max=1500):
+    sizes, kinds = smp.batch_plan(count, n_min, n_max, seed)
+    return [(int(n), smp.generate(int(k), int(n), seed=seed * 7919 + i)) for i, (n, k) in
+            enumerate(zip(sizes, kinds))]
+
+
+def test_alone_equals_mix_equals_oracle(gpu, oracle):
+    smp = gpu
+    reqs = requests(smp, 48, seed=3)
+    want = [oracle.gs_seq(n, pm, pw)[0] for n, (pm, pw) in reqs]
+    with smp.SmpServer(lanes=3, max_batch=8) as srv:
+        alone = []
+        for n, (pm, pw) in reqs:
+            m, info = srv.wait(srv.submit(pm, pw, n), timeout_ms=60000)
+            assert info["batch_size"] == 1
+            alone.append(m)
+        tickets = [srv.submit(pm, pw, n) for n, (pm, pw) in reqs]
+        mix, infos = [], []
+        for t in tickets:
+            m, info = srv.wait(t, timeout_ms=60000)
+            mix.append(m)
+            infos.append(info)
+        st = srv.stats()
+    for i in range(len(reqs)):
+        assert np.array_equal(alone[i], want[i]), i
+        assert np.array_equal(mix[i], want[i]), i
+    assert max(i["batch_size"] for i in infos) > 1  # requests really were batched
+    assert st["completed"] == 2 * len(reqs)
+    assert st["arena_bytes"] == 0
+
+
+def test_device_resident_requests_and_admission(gpu, oracle):
+    smp = gpu
+    reqs = requests(smp, 24, seed=5, n_min=256, n_max=2048)
+    want = [oracle.mw_seq(n, pm, pw)[0] for n, (pm, pw) in reqs]
+    dev = [(smp.DeviceArray(pm), smp.DeviceArray(pw)) for _, (pm, pw) in reqs]
+    # A budget that holds only a few of these requests at once forces admission to wait
+    # for memory to free up.
+    budget = 3 * 2048 * 2048 * 2 + (1 << 20)
+    with smp.SmpServer(lanes=2, max_batch=4, hbm_budget_bytes=budget) as srv:
+        tickets = [srv.submit(dm, dw, n) for (n, _), (dm, dw) in zip(reqs, dev)]
+        got = [srv.wait(t, timeout_ms=60000)[0] for t in tickets]
+        st = srv.stats()
+    for i, g in enumerate(got):
+        assert np.array_equal(g, want[i]), i
+    assert st["peak_arena_bytes"] <= budget
+
+
+def test_plain_mode_matches(gpu, oracle):
+    smp = gpu
+    reqs = requests(smp, 12, seed=9)
+    with smp.SmpServer(mode=smp.SmpServer.PLAIN) as srv:
+        tickets = [srv.submit(pm, pw, n) for n, (pm, pw) in reqs]
+        got = [srv.wait(t, timeout_ms=60000)[0] for t in tickets]
+    for (n, (pm, pw)), g in zip(reqs, got):
+        assert np.array_equal(g, oracle.gs_seq(n, pm, pw)[0])
+
+
+def test_invalid_request_fails_alone(gpu, oracle):
+    smp = gpu
+    n = 64
+    pm, pw = smp.generate(0, n, seed=1)
+    bad = pw.copy()
+    bad[3 * n] = bad[3 * n + 1]
+    with smp.SmpServer(lanes=2, max_batch=4) as srv:
+        t_bad = srv.submit(pm, bad, n)
+        t_ok = srv.submit(pm, pw, n)
+        with pytest.raises(smp.SmpError):
+            srv.wait(t_bad, timeout_ms=60000)
+        got, _ = srv.wait(t_ok, timeout_ms=60000)
+    assert np.array_equal(got, oracle.gs_seq(n, pm, pw)[0])
