@@ -143,6 +143,76 @@ def batched(smp, args):
             "bit_exact_batched_vs_looped": bool(same)}, (sizes, kinds, pm_all, pw_all, res_batched)
 
 
+def serving(smp, args):
+    """Config 5: Poisson arrivals, n log-uniform in [128, 8192], same generator mix as
+    the batched config, lists resident in HBM (a pool the requests draw from)."""
+    import threading
+    rng = np.random.default_rng(args.seed)
+    sizes, kinds = smp.batch_plan(args.serve_pool, 128, 8192, seed=args.seed + 5)
+    t0 = time.perf_counter()
+    pool = []
+    for i, (n, k) in enumerate(zip(sizes, kinds)):
+        pm, pw = smp.generate(int(k), int(n), seed=args.seed * 7 + i, threads=16)
+        pool.append((int(n), smp.DeviceArray(pm), smp.DeviceArray(pw), pm, pw))
+    log(f"serving pool of {len(pool)} instances ready in {time.perf_counter() - t0:.1f}s")
+    picks = rng.integers(0, len(pool), size=args.serve_requests)
+    gaps = rng.exponential(1.0 / args.serve_rate, size=args.serve_requests)
+    arrivals = np.cumsum(gaps)
+
+    def run(mode):
+        results = [None] * len(picks)
+        with smp.SmpServer(mode=mode, lanes=3, max_batch=64, max_n=8192) as srv:
+            # warm-up request (kernel load, first allocations)
+            n0, d0m, d0w, _, _ = pool[0]
+            srv.wait(srv.submit(d0m, d0w, n0), timeout_ms=120000)
+            tickets = [None] * len(picks)
+            start = time.perf_counter()
+
+            def producer():
+                for i, p in enumerate(picks):
+                    delay = start + arrivals[i] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    n, dm, dw, _, _ = pool[p]
+                    tickets[i] = srv.submit(dm, dw, n)
+
+            th = threading.Thread(target=producer)
+            th.start()
+            infos = []
+            for i in range(len(picks)):
+                while tickets[i] is None:
+                    time.sleep(0.0002)
+                m, info = srv.wait(tickets[i], timeout_ms=600000)
+                results[i] = m
+                infos.append(info)
+            th.join()
+            end = time.perf_counter()
+            st = srv.stats()
+        lat = np.array([x["latency_ms"] for x in infos])
+        return results, {
+            "solves_per_s": round(len(picks) / (end - start), 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3),
+            "launches": int(st["launches"]),
+            "peak_inflight": int(st["peak_inflight"]),
+            "peak_lanes_busy": int(st["peak_lanes_busy"]),
+            "peak_arena_mb": round(st["peak_arena_bytes"] / 2**20, 1),
+            "wall_s": round(end - start, 3)}
+
+    res_b, batched_stats = run(smp.SmpServer.BATCHED)
+    log(f"serving batched: {batched_stats}")
+    res_p, plain_stats = run(smp.SmpServer.PLAIN)
+    log(f"serving plain: {plain_stats}")
+    same = all(np.array_equal(a, b) for a, b in zip(res_b, res_p))
+    for dm_pair in pool:
+        dm_pair[1].free()
+        dm_pair[2].free()
+    return {"arrival_rate_per_s": args.serve_rate, "requests": int(args.serve_requests),
+            "pool": len(pool), "n_range": [128, 8192], "batched": batched_stats,
+            "plain": plain_stats, "bit_exact_batched_vs_plain": bool(same)}, (pool, picks, res_b)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,7 +224,14 @@ def main() -> int:
     ap.add_argument("--tail-threshold", type=int, default=None)
     ap.add_argument("--prefix-cols", type=int, default=None)
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--serve-pool", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--serve-rate", type=float, default=1500.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run the 16-byte streaming copy kernel over this many MiB "
+                         "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -166,6 +243,14 @@ def main() -> int:
     if smp.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src = smp.DeviceArray(np.zeros(nbytes // 4, np.int32))
+        dst = smp.DeviceArray(np.zeros(nbytes // 4, np.int32))
+        for _ in range(3):
+            smp.device_copy(dst, src)
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
+        return 0
 
     pm, pw, match, walls, profs, elapsed = single_instance(smp, args)
     n = args.n
@@ -214,6 +299,9 @@ def main() -> int:
     if not args.no_batch:
         bres, _ = batched(smp, args)
         out["batched"] = bres
+    if not args.no_serve:
+        sres, _ = serving(smp, args)
+        out["serving"] = sres
     print(json.dumps(out), flush=True)
     return 0
 

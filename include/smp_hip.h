@@ -75,6 +75,17 @@ enum smp_workload {
 int smp_generate(int32_t kind, int32_t n, uint64_t seed, int32_t param, int32_t* pref_m,
                  int32_t* pref_w, int32_t threads);
 
+/* The reference's on-disk workload cache (include/utils/generate_workloads_flat.h:
+ * 283-376): text, first line n, then n rows of pref_m and n rows of pref_w, each
+ * value followed by a space.  smp_workload_load with pref_m == NULL only reads n;
+ * otherwise capacity (elements of each output) must be >= n*n. */
+int smp_workload_save(const char* path, int32_t n, const int32_t* pref_m, const int32_t* pref_w);
+int smp_workload_load(const char* path, int32_t* n_out, int32_t* pref_m, int32_t* pref_w,
+                      size_t capacity);
+/* GetWorkloadFilename (generate_workloads_flat.h:285-300) with the reference's
+ * WorkloadType numbering: PERFECT 0, RANDOM 1, CONGESTED 2, SOLO 3. */
+int smp_workload_filename(int32_t ref_type, int32_t n, int32_t group, char* buf, size_t len);
+
 /* ---- single-instance engine ---------------------------------------------- */
 typedef struct smp_engine smp_engine;
 
@@ -241,6 +252,9 @@ int smp_device_alloc(size_t bytes, void** d_ptr);
 int smp_device_free(void* d_ptr);
 int smp_copy_h2d(void* d_dst, const void* h_src, size_t bytes);
 int smp_copy_d2h(void* h_dst, const void* d_src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and stores;
+ * bytes must be a multiple of 16).  Synchronous. */
+int smp_device_copy(void* d_dst, const void* d_src, size_t bytes);
 int smp_device_sync(void);
 
 const char* smp_last_error(void);

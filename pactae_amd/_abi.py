@@ -150,6 +150,9 @@ SMP_SERVE_PLAIN = 1
 # (name, restype, argtypes) for every entry point declared in include/smp_hip.h.
 SIGNATURES = [
     ("smp_generate", ctypes.c_int, [i32, i32, u64, i32, i32p, i32p, i32]),
+    ("smp_workload_save", ctypes.c_int, [ctypes.c_char_p, i32, i32p, i32p]),
+    ("smp_workload_load", ctypes.c_int, [ctypes.c_char_p, i32p, i32p, i32p, ctypes.c_size_t]),
+    ("smp_workload_filename", ctypes.c_int, [i32, i32, i32, ctypes.c_char_p, ctypes.c_size_t]),
     ("smp_default_opts", None, [ctypes.POINTER(SmpOpts)]),
     ("smp_create", ctypes.c_int, [i32p, i32p, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
     ("smp_create_device", ctypes.c_int, [vp, vp, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
@@ -180,6 +183,7 @@ SIGNATURES = [
     ("smp_device_free", ctypes.c_int, [vp]),
     ("smp_copy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
     ("smp_copy_d2h", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    ("smp_device_copy", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
     ("smp_device_sync", ctypes.c_int, []),
     ("smp_last_error", ctypes.c_char_p, []),
     ("smp_version", ctypes.c_char_p, []),

@@ -198,6 +198,20 @@ int smp_copy_d2h(void* h, const void* d, size_t bytes) {
   return SMP_OK;
 }
 
+int smp_device_copy(void* d, const void* src, size_t bytes) {
+  if (!d || !src || bytes % 16 != 0) {
+    pactae::set_error("smp_device_copy: null pointer or size not a multiple of 16");
+    return SMP_ERR_INVALID;
+  }
+  int dev = 0;
+  SMP_HIP_TRY(hipGetDevice(&dev));
+  int cus = 0;
+  SMP_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  SMP_HIP_TRY(pactae::launch_copy16(d, src, bytes, cus, nullptr));
+  SMP_HIP_TRY(hipDeviceSynchronize());
+  return SMP_OK;
+}
+
 int smp_device_sync(void) {
   SMP_HIP_TRY(hipDeviceSynchronize());
   return SMP_OK;

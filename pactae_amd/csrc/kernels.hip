@@ -317,8 +317,24 @@ __global__ __launch_bounds__(256) void k_post(const unsigned long long* __restri
   if (threadIdx.x == 0) atomicAdd(&sc->proposals, red[0] + red[1] + red[2] + red[3]);
 }
 
+// Streaming copy, 16 bytes per thread per iteration.
+__global__ void k_copy16(int4* __restrict__ dst, const int4* __restrict__ src, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host-callable launchers.
+
+hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s) {
+  const size_t count = bytes / 16;
+  int grid = (int)std::min<size_t>((count + 255) / 256, (size_t)cus * 16);
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, reinterpret_cast<int4*>(dst),
+                     reinterpret_cast<const int4*>(src), count);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long long v, hipStream_t s) {
   int grid = (int)std::min<size_t>((count + 255) / 256, 4096);

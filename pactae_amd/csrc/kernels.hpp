@@ -17,6 +17,7 @@ struct ProposeArgs {
   int n, K, T;
 };
 
+hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s);
 hipError_t launch_reset_scalars(Scalars* sc, int n, hipStream_t s);

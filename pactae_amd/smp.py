@@ -82,6 +82,50 @@ def generate_workload(workload_type: WorkloadType, n: int, group_size: int = 5, 
     return generate(_REF_KIND[WorkloadType(workload_type)], n, seed, group_size)
 
 
+def workload_filename(workload_type: WorkloadType, n: int, group_size: int = 5) -> str:
+    """GetWorkloadFilename (generate_workloads_flat.h:285-300)."""
+    buf = ctypes.create_string_buffer(128)
+    _check(_lib.smp_workload_filename(int(workload_type), n, group_size, buf, len(buf)),
+           "smp_workload_filename")
+    return buf.value.decode()
+
+
+def save_workload(path: str, n: int, pm, pw) -> None:
+    """SaveSmpWorkload (generate_workloads_flat.h:302-323)."""
+    pm = np.ascontiguousarray(pm, dtype=np.int32).reshape(-1)
+    pw = np.ascontiguousarray(pw, dtype=np.int32).reshape(-1)
+    _check(_lib.smp_workload_save(path.encode(), n, _i32p(pm), _i32p(pw)), "smp_workload_save")
+
+
+def load_workload(path: str):
+    """LoadSmpWorkload (generate_workloads_flat.h:325-352) -> (n, pref_m, pref_w)."""
+    n = ctypes.c_int32(0)
+    _check(_lib.smp_workload_load(path.encode(), ctypes.byref(n), None, None, 0), "smp_workload_load")
+    nn = n.value * n.value
+    pm = np.empty(nn, dtype=np.int32)
+    pw = np.empty(nn, dtype=np.int32)
+    _check(_lib.smp_workload_load(path.encode(), ctypes.byref(n), _i32p(pm), _i32p(pw), nn),
+           "smp_workload_load")
+    return n.value, pm, pw
+
+
+def generate_workload_cached(workload_type: WorkloadType, n: int, group_size: int = 5, seed: int = 0,
+                             directory: str = "data/workloads"):
+    """GenerateWorkloadCached (generate_workloads_flat.h:355-376): load the cached file if
+    present, else generate, save and return.  Returns (pref_m, pref_w)."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, workload_filename(workload_type, n, group_size))
+    if os.path.exists(path):
+        n_file, pm, pw = load_workload(path)
+        if n_file != n:
+            raise ValueError(f"cached workload {path} holds n={n_file}, asked for n={n}")
+        return pm, pw
+    pm, pw = generate_workload(workload_type, n, group_size, seed)
+    save_workload(path, n, pm, pw)
+    return pm, pw
+
+
 def batch_plan(count: int, n_min: int, n_max: int, seed: int, mix=(0.6, 0.2, 0.2)):
     sizes = np.empty(count, dtype=np.int32)
     kinds = np.empty(count, dtype=np.int32)
@@ -380,6 +424,11 @@ class SmpServer:
             self.close()
         except Exception:
             pass
+
+
+def device_copy(dst: DeviceArray, src: DeviceArray) -> None:
+    """Streaming device-to-device copy by the library's own kernel."""
+    _check(_lib.smp_device_copy(dst.ptr, src.ptr, min(dst.nbytes, src.nbytes) & ~15), "smp_device_copy")
 
 
 def device_count() -> int:

@@ -92,3 +92,49 @@ def test_batch_plan_is_deterministic_and_log_uniform(smp):
     assert 300 < np.median(s1) < 440
     frac = [(k1 == k).mean() for k in (0, 1, 3)]
     assert abs(frac[0] - 0.6) < 0.04 and abs(frac[1] - 0.2) < 0.03 and abs(frac[2] - 0.2) < 0.03
+
+
+def test_workload_text_cache_round_trip(smp, tmp_path):
+    # Format of SaveSmpWorkload (generate_workloads_flat.h:302-323): "n\n", n rows of
+    # M then n rows of W, every value followed by a space.
+    n = 5
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=1)
+    path = str(tmp_path / smp.workload_filename(smp.WorkloadType.RANDOM, n, 5))
+    smp.save_workload(path, n, pm, pw)
+    text = open(path).read().split("\n")
+    assert text[0] == "5"
+    M, W = pm.reshape(n, n), pw.reshape(n, n)
+    for r in range(n):
+        assert text[1 + r] == "".join(f"{v} " for v in M[r])
+        assert text[1 + n + r] == "".join(f"{v} " for v in W[r])
+    n2, pm2, pw2 = smp.load_workload(path)
+    assert n2 == n and np.array_equal(pm, pm2) and np.array_equal(pw, pw2)
+
+
+def test_workload_filenames_and_cache(smp, tmp_path):
+    WT = smp.WorkloadType
+    assert smp.workload_filename(WT.RANDOM, 30000, 5) == "smp_workload_random_30000_5.txt"
+    assert smp.workload_filename(WT.SOLO, 30000) == "smp_workload_solo_30000.txt"
+    assert smp.workload_filename(WT.CONGESTED, 7) == "smp_workload_congested_7.txt"
+    assert smp.workload_filename(WT.PERFECT, 9) == "smp_workload_perfect_9.txt"
+    d = str(tmp_path / "wl")
+    a = smp.generate_workload_cached(WT.SOLO, 300, seed=4, directory=d)
+    b = smp.generate_workload_cached(WT.SOLO, 300, seed=99, directory=d)  # cache hit: seed unused
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    import shutil
+    shutil.copy(os.path.join(d, smp.workload_filename(WT.SOLO, 300)),
+                os.path.join(d, smp.workload_filename(WT.SOLO, 301)))
+    with pytest.raises(ValueError):
+        smp.generate_workload_cached(WT.SOLO, 301, directory=d)
+    with pytest.raises(smp.SmpError):
+        smp.load_workload(os.path.join(d, "missing.txt"))
+
+
+def test_workload_load_rejects_bad_files(smp, tmp_path):
+    p = tmp_path / "bad.txt"
+    p.write_text("3\n0 1 2 \n1 2 \n")
+    with pytest.raises(smp.SmpError):
+        smp.load_workload(str(p))
+    p.write_text("x\n")
+    with pytest.raises(smp.SmpError):
+        smp.load_workload(str(p))
