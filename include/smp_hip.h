@@ -105,7 +105,10 @@ typedef struct smp_opts {
   int32_t precheck;       /* 1 = IsPerfect fast path (default), 0 = skip           */
   int32_t waves_per_cu;   /* proposal-kernel occupancy; 0 = auto                   */
   int32_t validate;       /* 1 = reject rows that are not permutations (default)   */
-  int32_t reserved[9];
+  int32_t coherent_hint;  /* 1 = partner-word hints read past non-coherent caches
+                             (default); 0 = plain cached reads (may be stale, which
+                             costs failed atomics, never correctness)             */
+  int32_t reserved[8];
 } smp_opts;
 
 typedef struct smp_times {
@@ -120,6 +123,7 @@ typedef struct smp_times {
   int64_t proposals;       /* sum over men of (final list position + 1)           */
   int64_t atomics;         /* atomicMin attempts made by the proposal kernel      */
   int64_t host_proposals;  /* proposals made by the host tail                     */
+  int64_t steps;           /* 64-entry segment scans made by the proposal kernel  */
   int32_t is_perfect;      /* 1 if the first-choice fast path answered            */
   int32_t handoff_free;    /* free men handed to the host (0 = GPU finished)      */
   int32_t prefix_cols;     /* PR columns actually gathered                        */
@@ -127,6 +131,18 @@ typedef struct smp_times {
   int32_t host_row_fetches;/* PR rows the host tail fetched beyond the prefix     */
   int32_t reserved[7];
 } smp_times;
+
+/* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;
+ * SmpInitEngine3 include/smp/smp_init_3.h:68-129): build the women's rank matrix
+ * and, if init_pr != 0, the men-major PR matrix over the engine's prefix_cols
+ * (set prefix_cols >= n for the full matrix), then copy the result to the host as
+ * the reference does.  Times are wall-clock ms plus HIP-event kernel ms. */
+typedef struct smp_init_times {
+  double rank_ms, pr_ms, transfer_ms, total_ms;
+  double rank_kernel_ms, pr_kernel_ms;
+  int32_t pr_cols;
+  int32_t reserved[7];
+} smp_init_times;
 
 void smp_default_opts(smp_opts* opts);
 
@@ -144,6 +160,26 @@ int smp_result(smp_engine* e, int32_t* match_m2w);
 /* Device address of the last result (int32[n] in HBM, owned by the engine). */
 int smp_result_device(smp_engine* e, const int32_t** d_match);
 int smp_profile(const smp_engine* e, smp_times* t);
+int smp_init_exec(smp_engine* e, int32_t init_pr, smp_init_times* t);
+/* Measure the hand-off threshold and PR prefix width on this instance: every
+ * candidate (threshold in {0 (GPU finishes), 1, 2, 4, ..., 1024}, prefix in
+ * {64, 128, 256}) is solved `trials` times; the lowest median wall time is kept
+ * and used by later smp_exec calls in SMP_TAIL_AUTO mode. */
+typedef struct smp_tune_result {
+  int32_t prefix_cols, tail_threshold;
+  double best_ms;      /* median wall ms of the chosen pair              */
+  double gpu_only_ms;  /* best median with the GPU finishing every chain */
+  int32_t candidates;
+  int32_t reserved[7];
+} smp_tune_result;
+int smp_tune(smp_engine* e, int32_t trials, smp_tune_result* out);
+/* Copies of the engine's device matrices after smp_exec or smp_init_exec (the
+ * reference's RetrieveRankMatrixForTest / RetrievePRMatrixForTest hooks,
+ * include/smp/smp_engine_hybrid.cuh:26-32).  rank_out: int32[n*n].  pr_out:
+ * int32[n * prefix_cols * 2] as PRNode pairs (woman, rank), include/utils/types.h:58-61;
+ * *cols receives prefix_cols. */
+int smp_rank_matrix(smp_engine* e, int32_t* rank_out);
+int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols);
 void smp_destroy(smp_engine* e);
 
 /* ---- batched: many independent instances in one persistent launch -------- */

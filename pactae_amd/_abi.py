@@ -42,7 +42,8 @@ class SmpOpts(ctypes.Structure):
         ("precheck", i32),
         ("waves_per_cu", i32),
         ("validate", i32),
-        ("reserved", i32 * 9),
+        ("coherent_hint", i32),
+        ("reserved", i32 * 8),
     ]
 
 
@@ -61,11 +62,42 @@ class SmpTimes(ctypes.Structure):
         ("proposals", i64),
         ("atomics", i64),
         ("host_proposals", i64),
+        ("steps", i64),
         ("is_perfect", i32),
         ("handoff_free", i32),
         ("prefix_cols", i32),
         ("tail_threshold", i32),
         ("host_row_fetches", i32),
+        ("reserved", i32 * 7),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class SmpInitTimes(ctypes.Structure):
+    _fields_ = [
+        ("rank_ms", ctypes.c_double),
+        ("pr_ms", ctypes.c_double),
+        ("transfer_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("rank_kernel_ms", ctypes.c_double),
+        ("pr_kernel_ms", ctypes.c_double),
+        ("pr_cols", i32),
+        ("reserved", i32 * 7),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class SmpTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("prefix_cols", i32),
+        ("tail_threshold", i32),
+        ("best_ms", ctypes.c_double),
+        ("gpu_only_ms", ctypes.c_double),
+        ("candidates", i32),
         ("reserved", i32 * 7),
     ]
 
@@ -160,6 +192,10 @@ SIGNATURES = [
     ("smp_result", ctypes.c_int, [vp, i32p]),
     ("smp_result_device", ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     ("smp_profile", ctypes.c_int, [vp, ctypes.POINTER(SmpTimes)]),
+    ("smp_init_exec", ctypes.c_int, [vp, i32, ctypes.POINTER(SmpInitTimes)]),
+    ("smp_tune", ctypes.c_int, [vp, i32, ctypes.POINTER(SmpTuneResult)]),
+    ("smp_rank_matrix", ctypes.c_int, [vp, i32p]),
+    ("smp_pr_matrix", ctypes.c_int, [vp, i32p, i32p]),
     ("smp_destroy", None, [vp]),
     ("smp_batch_default_opts", None, [ctypes.POINTER(SmpBatchOpts)]),
     ("smp_batch_create", ctypes.c_int,

@@ -28,6 +28,7 @@ void smp_default_opts(smp_opts* o) {
   o->precheck = 1;
   o->waves_per_cu = 0;
   o->validate = 1;
+  o->coherent_hint = 1;
 }
 
 static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
@@ -141,6 +142,52 @@ int smp_profile(const smp_engine* e, smp_times* t) {
     return SMP_ERR_INVALID;
   }
   *t = e->times;
+  return SMP_OK;
+}
+
+int smp_init_exec(smp_engine* e, int32_t init_pr, smp_init_times* t) {
+  if (!e || !t) {
+    pactae::set_error("null argument");
+    return SMP_ERR_INVALID;
+  }
+  SMP_HIP_TRY(hipSetDevice(e->opts.device));
+  return e->init_only(init_pr != 0, t);
+}
+
+int smp_tune(smp_engine* e, int32_t trials, smp_tune_result* out) {
+  if (!e) {
+    pactae::set_error("null engine");
+    return SMP_ERR_INVALID;
+  }
+  SMP_HIP_TRY(hipSetDevice(e->opts.device));
+  return e->tune(trials, out);
+}
+
+int smp_rank_matrix(smp_engine* e, int32_t* rank_out) {
+  if (!e || !rank_out) {
+    pactae::set_error("null argument");
+    return SMP_ERR_INVALID;
+  }
+  SMP_HIP_TRY(hipStreamSynchronize(e->stream));
+  SMP_HIP_TRY(hipMemcpy(rank_out, e->d_rank_w, (size_t)e->n * e->n * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+  return SMP_OK;
+}
+
+int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols) {
+  if (!e || !pr_out || !cols) {
+    pactae::set_error("null argument");
+    return SMP_ERR_INVALID;
+  }
+  SMP_HIP_TRY(hipStreamSynchronize(e->stream));
+  const size_t cnt = (size_t)e->n * e->K;
+  std::vector<uint32_t> packed(cnt);
+  SMP_HIP_TRY(hipMemcpy(packed.data(), e->d_pr, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < cnt; ++i) {
+    pr_out[2 * i] = (int32_t)pactae::pr_woman(packed[i]);
+    pr_out[2 * i + 1] = (int32_t)pactae::pr_rank(packed[i]);
+  }
+  *cols = e->K;
   return SMP_OK;
 }
 

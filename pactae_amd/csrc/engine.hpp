@@ -111,6 +111,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   pactae::DeviceInfo dev;
   int K = 0;  // PR prefix columns
   int T = 0;  // hand-off threshold (0 = GPU finishes)
+  int K_alloc = 0;               // PR columns the buffers hold
+  int tuned_K = 0, tuned_T = -1;  // measured choice for AUTO mode (smp_tune)
   bool owns_lists = false;
   bool have_result = false;
 
@@ -124,6 +126,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   Scalars* d_sc = nullptr;
   int2* d_parked = nullptr;
   uint32_t* d_rowbuf = nullptr;
+  unsigned long long* d_wstats = nullptr;  // per proposal wave: atomics, steps
 
   uint32_t* h_pr = nullptr;  // pinned copy of the PR prefix for the host tail
   unsigned long long* h_partner = nullptr;
@@ -136,6 +139,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   hipEvent_t ev[8] = {};
 
   smp_times times{};
+  volatile uint64_t touch_sink = 0;
   static constexpr int kRowFetch = 2048;
 
   ~smp_engine() override { release(); }
@@ -156,6 +160,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipFree(d_sc);
     (void)hipFree(d_parked);
     (void)hipFree(d_rowbuf);
+    (void)hipFree(d_wstats);
+    d_wstats = nullptr;
     (void)hipHostFree(h_pr);
     (void)hipHostFree(h_partner);
     (void)hipHostFree(h_sc);
@@ -184,23 +190,25 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (int st = pactae::query_device(opts.device, &dev)) return st;
     const size_t nn = (size_t)n * n;
     n_alloc = n;
+    K_alloc = std::min(n, std::max(prefix_for(n), 256));
     configure(n);
     SMP_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     for (auto& e : ev) SMP_HIP_TRY(hipEventCreate(&e));
     if (hipMalloc(&d_rank_w, nn * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&d_pr, (size_t)n * K * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_pr, (size_t)n * K_alloc * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_partner, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&d_match, (size_t)n * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&d_seen, (size_t)n * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&d_parked, (size_t)n * sizeof(int2)) != hipSuccess ||
-        hipMalloc(&d_rowbuf, kRowFetch * sizeof(uint32_t)) != hipSuccess) {
+        hipMalloc(&d_rowbuf, kRowFetch * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_wstats, (size_t)dev.cus * 32 * 2 * sizeof(unsigned long long)) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("device allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
     }
-    if (hipHostMalloc(&h_pr, (size_t)n * K * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc(&h_pr, (size_t)n * K_alloc * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&h_partner, (size_t)n * sizeof(unsigned long long), hipHostMallocDefault) !=
             hipSuccess ||
         hipHostMalloc(&h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
@@ -230,6 +238,61 @@ struct smp_engine : public pactae::AbsSmpEngine {
       T = 0;
     else
       T = opts.tail_threshold > 0 ? std::min(opts.tail_threshold, ni) : std::min(ni, 32);
+    if (ni == n_alloc && opts.tail_mode == SMP_TAIL_AUTO && tuned_T >= 0) {
+      if (opts.prefix_cols <= 0) K = tuned_K;
+      if (opts.tail_threshold <= 0) T = tuned_T;
+    }
+  }
+
+  // Measured choice of the hand-off threshold T and the prefix width K: every
+  // candidate pair is run `trials` times on this instance and the lowest median
+  // wall time wins.  AUTO mode then uses it.  Results are identical for every
+  // pair (the matching is unique); only the time differs.
+  int tune(int trials, smp_tune_result* out) {
+    if (trials <= 0) trials = 3;
+    const int Ks[] = {64, 128, 256};
+    const int Ts[] = {0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 1024};
+    const smp_opts saved = opts;
+    double best = 1e300, gpu_only = 1e300;
+    int bK = prefix_for(n), bT = 0, cands = 0;
+    std::vector<double> w(trials);
+    for (int k : Ks) {
+      if (k > K_alloc) continue;
+      for (int t : Ts) {
+        if (t > n) continue;
+        opts.prefix_cols = k;
+        opts.tail_mode = t == 0 ? SMP_TAIL_GPU : SMP_TAIL_HOST;
+        opts.tail_threshold = t;
+        configure(n);
+        for (int i = 0; i < trials; ++i) {
+          auto t0 = Clock::now();
+          if (int st = exec()) {
+            opts = saved;
+            configure(n);
+            return st;
+          }
+          w[i] = ms_since(t0);
+        }
+        std::sort(w.begin(), w.end());
+        const double med = w[trials / 2];
+        ++cands;
+        if (t == 0) gpu_only = std::min(gpu_only, med);
+        if (med < best) best = med, bK = k, bT = t;
+      }
+    }
+    opts = saved;
+    tuned_K = bK;
+    tuned_T = bT;
+    configure(n);
+    if (out) {
+      *out = smp_tune_result{};
+      out->prefix_cols = bK;
+      out->tail_threshold = bT;
+      out->best_ms = best;
+      out->gpu_only_ms = gpu_only;
+      out->candidates = cands;
+    }
+    return SMP_OK;
   }
 
   // Solve another instance (lists already in HBM, size <= n_alloc) with this
@@ -247,6 +310,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
       SMP_HIP_TRY(hipStreamSynchronize(stream));
     }
     return SMP_OK;
+  }
+
+  // Workgroups (4 waves each) of the persistent proposal kernel.
+  int propose_grid() const {
+    const int wpc = std::min(32, opts.waves_per_cu > 0 ? opts.waves_per_cu : 16);
+    const long long waves = std::min<long long>((long long)dev.cus * wpc, (long long)n);
+    return (int)std::max<long long>(1, (waves + 3) / 4);
   }
 
   int read_scalars() {
@@ -270,7 +340,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int IsPerfect(bool* perfect) override {
     times = smp_times{};
     have_result = false;
-    SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, n, stream));
+    SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
     if (!opts.precheck) {
       *perfect = false;
       return SMP_OK;
@@ -313,46 +383,63 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.partner = d_partner;
     a.parked = d_parked;
     a.sc = d_sc;
+    a.wstats = d_wstats;
     a.n = n;
     a.K = K;
     a.T = T;
-    const int wpc = opts.waves_per_cu > 0 ? opts.waves_per_cu : 16;
-    long long waves = std::min<long long>((long long)dev.cus * wpc, (long long)n);
-    const int grid = (int)std::max<long long>(1, (waves + 3) / 4);
+    a.coherent_hint = opts.coherent_hint;
+    const int grid = propose_grid();
     SMP_HIP_TRY(hipEventRecord(ev[3], stream));
     SMP_HIP_TRY(pactae::launch_propose(a, grid, stream));
     SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+    if (T > 0) {
+      // While the kernel runs: wait for the PR prefix copy and pull it into the
+      // host caches (a DMA-written buffer starts out in DRAM only).
+      SMP_HIP_TRY(hipStreamSynchronize(copy_stream));
+      touch_host_prefix();
+    }
     if (int st = read_scalars()) return st;
     if (int st = check_device_error("proposal kernel")) return st;
-    times.atomics = (int64_t)h_sc->atomics;
-    // Men [0, waves) were taken by the waves' first iterations, the rest from the cursor.
-    const int started = (int)std::min<long long>(n, (long long)grid * 4 + h_sc->cursor);
-    const int handoff = h_sc->parked_count + (n - started);
+    const int handoff = h_sc->parked_count;
     times.handoff_free = handoff;
     if (handoff > 0) {
-      if (int st = host_tail(started)) return st;
+      if (int st = host_tail()) return st;
     }
     return SMP_OK;
   }
 
+  void touch_host_prefix() {
+    const uint64_t* p = reinterpret_cast<const uint64_t*>(h_pr);
+    const size_t words = (size_t)n * K / 2;
+    uint64_t acc = 0;
+    for (size_t i = 0; i < words; i += 8) acc += p[i];  // one load per 64-byte line
+    touch_sink = acc;
+  }
+
   // Host tail: McVitie-Wilson chains (smp_engine_mw_1_f9.cu:15-79, and the
-  // reference's host LAProcedure smp_engine_bamboo.cu:197-230) over the packed
-  // partner words, starting from the state the kernel parked.
-  int host_tail(int started) {
+  // reference's host LAProcedure smp_engine_bamboo.cu:197-230) over the partner
+  // state the kernel left, split into rank / man / position arrays that stay in
+  // the host caches.
+  int host_tail() {
     auto t0 = Clock::now();
     SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, stream));
     const int parked = h_sc->parked_count;
-    if (parked > 0)
-      SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)parked * sizeof(int2),
-                                 hipMemcpyDeviceToHost, stream));
+    SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)parked * sizeof(int2),
+                               hipMemcpyDeviceToHost, stream));
     SMP_HIP_TRY(hipStreamSynchronize(stream));
     SMP_HIP_TRY(hipStreamSynchronize(copy_stream));
     times.handoff_ms = ms_since(t0);
 
     t0 = Clock::now();
-    std::vector<int2> todo(h_parked, h_parked + parked);
-    for (int m = started; m < n; ++m) todo.push_back(make_int2(m, 0));
+    constexpr uint32_t kFree = 0xFFFFFFFFu;
+    std::vector<uint32_t> hr(n), hm(n), hp(n);
+    for (int w = 0; w < n; ++w) {
+      const unsigned long long v = h_partner[w];
+      hr[w] = v == pactae::kEmpty ? kFree : pactae::pw_rank(v);
+      hm[w] = pactae::pw_man(v);
+      hp[w] = pactae::pw_pos(v);
+    }
     struct Ext {
       int k0 = 0;
       std::vector<uint32_t> e;
@@ -360,8 +447,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
     std::unordered_map<int, Ext> ext;
     long long props = 0;
     int fetches = 0;
-    for (const int2& start : todo) {
-      int m = start.x, k = start.y;
+    const uint32_t* pr = h_pr;
+    for (int i = 0; i < parked; ++i) {
+      int m = h_parked[i].x, k = h_parked[i].y;
       for (;;) {
         if (k >= n) {
           pactae::set_error("host tail: a man exhausted his list (rows are not permutations)");
@@ -369,7 +457,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
         }
         uint32_t e;
         if (k < K) {
-          e = h_pr[(size_t)m * K + k];
+          e = pr[(size_t)m * K + k];
         } else {
           Ext& x = ext[m];
           if (x.e.empty() || k < x.k0 || k >= x.k0 + (int)x.e.size()) {
@@ -385,23 +473,26 @@ struct smp_engine : public pactae::AbsSmpEngine {
           e = x.e[k - x.k0];
         }
         ++props;
-        const uint32_t w = pactae::pr_woman(e);
+        const uint32_t w = pactae::pr_woman(e), r = pactae::pr_rank(e);
         if (w >= (uint32_t)n) {
           pactae::set_error("host tail: preference entry outside [0,n)");
           return SMP_ERR_INVALID;
         }
-        const unsigned long long mine = pactae::pw_pack(pactae::pr_rank(e), (uint32_t)m, (uint32_t)k);
-        const unsigned long long cur = h_partner[w];
-        if (mine < cur) {
-          h_partner[w] = mine;
-          if (cur == pactae::kEmpty) break;
-          m = (int)pactae::pw_man(cur);
-          k = (int)pactae::pw_pos(cur) + 1;
+        if (r < hr[w]) {
+          const uint32_t old_r = hr[w], old_m = hm[w], old_p = hp[w];
+          hr[w] = r;
+          hm[w] = (uint32_t)m;
+          hp[w] = (uint32_t)k;
+          if (old_r == kFree) break;
+          m = (int)old_m;
+          k = (int)old_p + 1;
         } else {
           ++k;
         }
       }
     }
+    for (int w = 0; w < n; ++w)
+      h_partner[w] = hr[w] == kFree ? pactae::kEmpty : pactae::pw_pack(hr[w], hm[w], hp[w]);
     times.host_proposals = props;
     times.host_row_fetches = fetches;
     SMP_HIP_TRY(hipMemcpyAsync(d_partner, h_partner, (size_t)n * sizeof(unsigned long long),
@@ -415,7 +506,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (perfect) {
       SMP_HIP_TRY(pactae::launch_first_match(d_pref_m, n, d_match, stream));
     } else {
-      SMP_HIP_TRY(pactae::launch_post(d_partner, n, d_match, d_sc, stream));
+      SMP_HIP_TRY(pactae::launch_post(d_partner, n, d_match, d_sc, d_wstats, propose_grid() * 4, stream));
     }
     SMP_HIP_TRY(hipMemcpyAsync(h_match.data(), d_match, (size_t)n * sizeof(int32_t),
                                hipMemcpyDeviceToHost, stream));
@@ -423,11 +514,52 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipStreamSynchronize(stream));
     if (int st = check_device_error("post")) return st;
     if (perfect) {
-      long long p = n;  // every man proposes once
-      times.proposals = p;
+      times.proposals = n;  // every man proposes once
     } else {
       times.proposals = (int64_t)h_sc->proposals;
+      times.atomics = (int64_t)h_sc->atomics;
+      times.steps = (int64_t)h_sc->steps;
     }
+    return SMP_OK;
+  }
+
+  // Init-only run: rank matrix, optionally the PR matrix, then the D2H copy of
+  // the last matrix built (AbsSmpInitEngine::Init, smp_init_abs.h:16-28).
+  int init_only(bool init_pr, smp_init_times* t) {
+    *t = smp_init_times{};
+    SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
+    const int chunk = std::min(n, std::max(1024, (dev.max_lds / 4) & ~3));
+    auto t0 = Clock::now();
+    SMP_HIP_TRY(hipEventRecord(ev[0], stream));
+    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, 1024, d_sc,
+                                           stream));
+    SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+    SMP_HIP_TRY(hipStreamSynchronize(stream));
+    t->rank_ms = ms_since(t0);
+    if (init_pr) {
+      t0 = Clock::now();
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[2], stream));
+      SMP_HIP_TRY(hipStreamSynchronize(stream));
+      t->pr_ms = ms_since(t0);
+    }
+    if (int st = read_scalars()) return st;
+    if (int st = check_device_error("init")) return st;
+    t0 = Clock::now();
+    std::vector<uint32_t> host;
+    if (init_pr) {
+      host.resize((size_t)n * K);
+      SMP_HIP_TRY(hipMemcpy(host.data(), d_pr, host.size() * 4, hipMemcpyDeviceToHost));
+    } else {
+      host.resize((size_t)n * n);
+      SMP_HIP_TRY(hipMemcpy(host.data(), d_rank_w, host.size() * 4, hipMemcpyDeviceToHost));
+    }
+    t->transfer_ms = ms_since(t0);
+    t->total_ms = t->rank_ms + t->pr_ms + t->transfer_ms;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) t->rank_kernel_ms = ms;
+    if (init_pr && hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) t->pr_kernel_ms = ms;
+    t->pr_cols = K;
     return SMP_OK;
   }
 

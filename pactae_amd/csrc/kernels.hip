@@ -36,10 +36,9 @@ __global__ void k_fill_i32(int* p, size_t count, int v) {
     p[i] = v;
 }
 
-__global__ void k_reset_scalars(Scalars* sc, int n) {
+__global__ void k_reset_scalars(Scalars* sc, int active) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    sc->cursor = 0;
-    sc->free_count = n;
+    sc->active = active;
     sc->parked_count = 0;
     sc->error = 0;
     sc->dup_first = 0;
@@ -186,48 +185,50 @@ __global__ void k_gather_row(const int32_t* __restrict__ pref_m, const int32_t* 
 
 // ---------------------------------------------------------------------------
 // Proposal kernel.  Persistent: each wave owns one free man ("chain") at a time.
-// A step loads the next 64 entries of the man's list (one coalesced PR load, or
-// pref_m + rank_w beyond the gathered prefix), reads the 64 women's partner words
-// (plain, coherent loads: words only ever decrease, so a stale read only makes
-// the wave try an atomic that then fails), and ballots the lanes where the man
-// beats the current partner.  Candidates are tried in list order with a 64-bit
-// atomicMin; the first success engages the man.  If the woman was taken, the wave
-// continues as the displaced man at his stored position + 1; if she was free the
-// chain ends and the wave takes a never-started man from the cursor.
+// Wave g starts the men g, g + W, g + 2W, ... (W waves in the grid) in turn, so
+// starting a man needs no shared counter.  A step loads the next 64 entries of
+// the man's list (one coalesced PR load, or pref_m + rank_w beyond the gathered
+// prefix), reads the 64 women's partner words and ballots the lanes where the
+// man beats the current partner.  A partner word only ever decreases, so a stale
+// read can only make the wave try an atomic that then fails; coherent_hint picks
+// whether those reads bypass the non-coherent caches.  Candidates are tried in
+// list order with a 64-bit atomicMin; the first success engages the man.  If the
+// woman was taken the wave continues as the displaced man at his stored position
+// + 1; if she was free the chain ends.
 //
-// Hand-off: with T > 0 a wave that sees free_count <= T after a step parks its
-// chain (man, next position) for the host and exits; waves do not start new men.
-
-
+// Hand-off: `active` counts waves that still hold work; a wave decrements it once,
+// when its chain ends and it has no unstarted man left.  With T > 0, a wave that
+// sees active <= T after a step parks its chain (man, next position) and its
+// unstarted men for the host, and exits.  Near the end every unstarted man has
+// been started, so `active` equals the number of free men.
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   const int lane = threadIdx.x & 63;
   const int n = a.n, K = a.K, T = a.T;
-  // Wave g starts with man g; later men come from the cursor, offset by the
-  // number of waves, so the first round needs no atomic.
   const int waves = (int)(gridDim.x * (blockDim.x >> 6));
   const int gwave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  int next_init = gwave;
   int m = -1, k = 0;
-  bool first = true;
   unsigned long long n_atomics = 0, n_steps = 0;
+  if (__hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    next_init = n;  // init found malformed lists: do nothing
+  }
   for (;;) {
     if (m < 0) {
-      int got = gwave;
-      if (!first) {
-        if (lane == 0) got = waves + atomicAdd(&a.sc->cursor, 1);
-        got = __shfl(got, 0);
+      if (next_init >= n) {
+        if (lane == 0 && gwave < n) atomicSub(&a.sc->active, 1);
+        break;
       }
-      first = false;
-      if (got >= n) break;
-      m = got;
+      m = next_init;
+      next_init += waves;
       k = 0;
     }
     if (k >= n) {  // only possible when rows are not permutations
       if (lane == 0) atomicOr(&a.sc->error, kErrListEnd);
       break;
     }
-    int fc = 0;
+    int act = 0;
     if (T > 0 && lane == 0)
-      fc = __hip_atomic_load(&a.sc->free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      act = __hip_atomic_load(&a.sc->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ++n_steps;
     const int kk = k + lane;
     bool valid = kk < n;
@@ -251,7 +252,11 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     }
     const unsigned long long mine = pw_pack(r, (uint32_t)m, (uint32_t)kk);
     unsigned long long hint = 0;
-    if (valid) hint = __hip_atomic_load(&a.partner[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (valid) {
+      hint = a.coherent_hint
+                 ? __hip_atomic_load(&a.partner[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                 : a.partner[w];
+    }
     unsigned long long cand = __ballot(valid && mine < hint);
     bool moved = false;
     while (cand) {
@@ -263,7 +268,6 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       ++n_atomics;
       if (old > mj) {
         if (old == kEmpty) {
-          if (lane == 0) atomicSub(&a.sc->free_count, 1);
           m = -1;
         } else {
           m = (int)pw_man(old);
@@ -276,19 +280,20 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     }
     if (!moved) k += 64;
     if (T > 0) {
-      fc = __shfl(fc, 0);
-      if (fc <= T) {
-        if (m >= 0 && lane == 0) {
-          const int slot = atomicAdd(&a.sc->parked_count, 1);
-          a.parked[slot] = make_int2(m, k);
+      act = __shfl(act, 0);
+      if (act <= T) {
+        if (lane == 0) {
+          if (m >= 0) a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(m, k);
+          for (; next_init < n; next_init += waves)
+            a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(next_init, 0);
         }
         break;
       }
     }
   }
   if (lane == 0) {
-    atomicAdd(&a.sc->atomics, n_atomics);
-    atomicAdd(&a.sc->steps, n_steps);
+    a.wstats[2 * gwave] = n_atomics;
+    a.wstats[2 * gwave + 1] = n_steps;
   }
 }
 
@@ -297,8 +302,28 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
 // over all women, which equals the total number of proposals of any
 // Gale-Shapley run (each man proposes down to his final partner exactly once).
 __global__ __launch_bounds__(256) void k_post(const unsigned long long* __restrict__ partner, int n,
-                                              int32_t* __restrict__ match, Scalars* sc) {
+                                              int32_t* __restrict__ match, Scalars* sc,
+                                              const unsigned long long* __restrict__ wstats,
+                                              int waves) {
   __shared__ unsigned long long red[4];
+  if (blockIdx.x == 0 && wstats) {  // fold the proposal kernel's per-wave counters
+    __shared__ unsigned long long red2[2][4];
+    unsigned long long at = 0, st = 0;
+    for (int g = threadIdx.x; g < waves; g += blockDim.x) {
+      at += wstats[2 * g];
+      st += wstats[2 * g + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      at += __shfl_down(at, off);
+      st += __shfl_down(st, off);
+    }
+    if ((threadIdx.x & 63) == 0) red2[0][threadIdx.x >> 6] = at, red2[1][threadIdx.x >> 6] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sc->atomics = red2[0][0] + red2[0][1] + red2[0][2] + red2[0][3];
+      sc->steps = red2[1][0] + red2[1][1] + red2[1][2] + red2[1][3];
+    }
+  }
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long props = 0;
   if (w < n) {
@@ -350,8 +375,8 @@ hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_reset_scalars(Scalars* sc, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, n);
+hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, active);
   return hipGetLastError();
 }
 
@@ -412,8 +437,9 @@ hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s) {
 }
 
 hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, partner, n, match, sc);
+                       const unsigned long long* wstats, int waves, hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, partner, n, match, sc, wstats,
+                     waves);
   return hipGetLastError();
 }
 

@@ -14,13 +14,15 @@ struct ProposeArgs {
   unsigned long long* partner;
   int2* parked;
   Scalars* sc;
+  unsigned long long* wstats;  // per wave: [2*g] atomics, [2*g+1] segment steps
   int n, K, T;
+  int coherent_hint;           // 1: partner hints bypass non-coherent caches
 };
 
 hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s);
-hipError_t launch_reset_scalars(Scalars* sc, int n, hipStream_t s);
+hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s);
 hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars* sc, hipStream_t s);
 hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s);
 hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
@@ -32,6 +34,6 @@ hipError_t launch_gather_row(const int32_t* pref_m, const int32_t* rank_w, int n
                              int count, uint32_t* out, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
 hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
-                       hipStream_t s);
+                       const unsigned long long* wstats, int waves, hipStream_t s);
 
 }  // namespace pactae

@@ -49,8 +49,8 @@ enum : int {
 
 // Small device-resident control block of one solve.
 struct Scalars {
-  int cursor;        // next never-started man (work queue over 0..n-1)
-  int free_count;    // men not held by any woman (started or not)
+  int active;        // proposal waves that still hold work (a chain or unstarted men)
+  int pad0;
   int parked_count;  // chains parked for the host tail
   int error;         // kErr* bits
   int dup_first;     // precheck: 1 if two men share a first choice

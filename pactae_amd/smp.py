@@ -247,6 +247,31 @@ class SmpEngineBamboo:
         print(f"{self.name} Total Time: {p['precheck_ms'] + p['init_ms'] + p['core_ms']} ms")
         print("=" * 66)
 
+    def init_only(self, init_pr: bool = True) -> dict:
+        """SmpInitEngine::Init (smp_init_abs.h:16-28): rank matrix [+ PR matrix] + D2H."""
+        t = _abi.SmpInitTimes()
+        _check(_lib.smp_init_exec(self._h, int(init_pr), ctypes.byref(t)), "smp_init_exec")
+        return t.as_dict()
+
+    def tune(self, trials: int = 3) -> dict:
+        """Measure the hand-off threshold and prefix width (smp_tune); AUTO mode uses them."""
+        r = _abi.SmpTuneResult()
+        _check(_lib.smp_tune(self._h, trials, ctypes.byref(r)), "smp_tune")
+        return r.as_dict()
+
+    def rank_matrix(self) -> np.ndarray:
+        out = np.empty(self.n * self.n, dtype=np.int32)
+        _check(_lib.smp_rank_matrix(self._h, _i32p(out)), "smp_rank_matrix")
+        return out.reshape(self.n, self.n)
+
+    def pr_matrix(self) -> np.ndarray:
+        """PR matrix as PRNode pairs: array [n, cols, 2] of (woman, rank)."""
+        cols = self.profile()["prefix_cols"] or min(self.n, 128)
+        out = np.empty(self.n * max(cols, self.n) * 2, dtype=np.int32)
+        c = ctypes.c_int32(0)
+        _check(_lib.smp_pr_matrix(self._h, _i32p(out), ctypes.byref(c)), "smp_pr_matrix")
+        return out[: self.n * c.value * 2].reshape(self.n, c.value, 2)
+
     def close(self) -> None:
         if self._h:
             _lib.smp_destroy(self._h)

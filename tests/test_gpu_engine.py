@@ -116,3 +116,23 @@ def test_invalid_lists_are_rejected(gpu):
     with pytest.raises(smp.SmpError) as ei:
         solve(smp, n, bad_m, pw, precheck=0)
     assert ei.value.status == 1
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 333, 1024, 4001])
+def test_init_only_rank_and_pr_matrices(gpu, n):
+    """Kernel-level parity: k_invert_rows against numpy argsort; k_gather_pr against a
+    numpy gather, for prefix and full widths (SmpInitEngine / figure 5 path)."""
+    smp = gpu
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=n)
+    M, W = pm.reshape(n, n), pw.reshape(n, n)
+    rank_w = np.argsort(W, axis=1)
+    for cols in (1, 64, n):
+        with smp.SmpEngineBamboo(smp.SmpObj.create_from_flat_lists(pm, pw, n), n, prefix_cols=cols) as e:
+            t = e.init_only(init_pr=True)
+            assert t["pr_cols"] == min(cols, n)
+            assert np.array_equal(e.rank_matrix(), rank_w)
+            pr = e.pr_matrix()
+            K = min(cols, n)
+            assert pr.shape == (n, K, 2)
+            assert np.array_equal(pr[:, :, 0], M[:, :K])
+            assert np.array_equal(pr[:, :, 1], rank_w[M[:, :K], np.arange(n)[:, None]])
