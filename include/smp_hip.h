@@ -108,7 +108,9 @@ typedef struct smp_opts {
   int32_t coherent_hint;  /* 1 = partner-word hints read past non-coherent caches
                              (default); 0 = plain cached reads (may be stale, which
                              costs failed atomics, never correctness)             */
-  int32_t reserved[8];
+  int32_t poll_interval;  /* proposal-kernel steps between reads of the shared
+                             active-wave counter that triggers the hand-off; 0 = 4 */
+  int32_t reserved[7];
 } smp_opts;
 
 typedef struct smp_times {

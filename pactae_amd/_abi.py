@@ -43,7 +43,8 @@ class SmpOpts(ctypes.Structure):
         ("waves_per_cu", i32),
         ("validate", i32),
         ("coherent_hint", i32),
-        ("reserved", i32 * 8),
+        ("poll_interval", i32),
+        ("reserved", i32 * 7),
     ]
 
 

@@ -28,7 +28,8 @@ void smp_default_opts(smp_opts* o) {
   o->precheck = 1;
   o->waves_per_cu = 0;
   o->validate = 1;
-  o->coherent_hint = 1;
+  o->coherent_hint = 0;
+  o->poll_interval = 4;
 }
 
 static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {

@@ -218,8 +218,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error("pinned host allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
     }
-    const int chunk_cols = std::max(1024, (dev.max_lds / 4) & ~3);
-    if (hipError_t e = pactae::set_invert_lds_limit(std::min(n, chunk_cols) * 4); e != hipSuccess) {
+    if (hipError_t e = pactae::set_invert_lds_limit(((invert_chunk() + 3) & ~3) * 2); e != hipSuccess) {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
@@ -312,6 +311,14 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return SMP_OK;
   }
 
+  // Columns per LDS pass of the rank inversion: the whole row when its 16-bit
+  // staging fits in a workgroup's LDS (n up to 81,920 on gfx950), else a multiple
+  // of 4 that does.
+  int invert_chunk() const {
+    const int cap = (dev.max_lds / 2) & ~3;
+    return n <= cap ? n : cap;
+  }
+
   // Workgroups (4 waves each) of the persistent proposal kernel.
   int propose_grid() const {
     const int wpc = std::min(32, opts.waves_per_cu > 0 ? opts.waves_per_cu : 16);
@@ -356,11 +363,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   // -- init: rank matrix + PR prefix gather (smp_engine_bamboo.cu:33-106) -----
   int InitProc() override {
-    const int chunk = std::min(n, std::max(1024, (dev.max_lds / 4) & ~3));
+    const int chunk = invert_chunk();
     SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
     SMP_HIP_TRY(hipEventRecord(ev[0], stream));
-    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, 1024, d_sc,
-                                           stream));
+    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream));
     SMP_HIP_TRY(hipEventRecord(ev[1], stream));
     SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
     SMP_HIP_TRY(hipEventRecord(ev[2], stream));
@@ -388,6 +394,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.K = K;
     a.T = T;
     a.coherent_hint = opts.coherent_hint;
+    a.poll = opts.poll_interval > 0 ? opts.poll_interval : 4;
     const int grid = propose_grid();
     SMP_HIP_TRY(hipEventRecord(ev[3], stream));
     SMP_HIP_TRY(pactae::launch_propose(a, grid, stream));
@@ -528,11 +535,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int init_only(bool init_pr, smp_init_times* t) {
     *t = smp_init_times{};
     SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
-    const int chunk = std::min(n, std::max(1024, (dev.max_lds / 4) & ~3));
+    const int chunk = invert_chunk();
     auto t0 = Clock::now();
     SMP_HIP_TRY(hipEventRecord(ev[0], stream));
-    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, 1024, d_sc,
-                                           stream));
+    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream));
     SMP_HIP_TRY(hipEventRecord(ev[1], stream));
     SMP_HIP_TRY(hipStreamSynchronize(stream));
     t->rank_ms = ms_since(t0);

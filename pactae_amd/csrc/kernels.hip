@@ -70,16 +70,24 @@ __global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t
 
 // ---------------------------------------------------------------------------
 // Rank inversion, one workgroup per row: rank[w][pref[w][r]] = r.
-// The row's inverse is assembled in LDS (`chunk` columns at a time; one pass when
-// the row fits) so that both the HBM read of pref and the HBM write of rank are
-// fully coalesced; only the LDS scatter is random.  With validate != 0 every LDS
-// slot starts at -1 and a slot left at -1 marks a row that is not a permutation.
+// The row's inverse is assembled in LDS as 16-bit positions (n <= 65535, so a
+// position fits and 0xFFFF is free to mark an empty slot), `chunk` columns per
+// pass (one pass when the row fits: 40 KB at n = 20,000, so four 512-thread
+// workgroups share a CU).  Each thread first issues up to kBlk 16-byte loads of
+// the row and only then scatters them, so many loads are in flight per thread;
+// the read of pref and the write of rank are both coalesced 16-byte accesses and
+// only the LDS scatter is random.  With validate != 0 a slot left at 0xFFFF marks
+// a row that is not a permutation.
+constexpr int kInvBlock = 512;
+constexpr int kBlk = 8;
+
 template <int VEC>
-__global__ __launch_bounds__(1024) void k_invert_rows(const int32_t* __restrict__ pref,
-                                                      int32_t* __restrict__ rank, int n, int chunk,
-                                                      int validate, Scalars* sc) {
-  extern __shared__ int32_t lds[];
-  const int tid = threadIdx.x, bs = blockDim.x;
+__global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __restrict__ pref,
+                                                           int32_t* __restrict__ rank, int n,
+                                                           int chunk, int validate, Scalars* sc) {
+  extern __shared__ uint16_t lds16[];
+  uint32_t* lds32 = reinterpret_cast<uint32_t*>(lds16);
+  const int tid = threadIdx.x;
   int bad = 0;
   for (int row = blockIdx.x; row < n; row += gridDim.x) {
     const int32_t* src = pref + (size_t)row * n;
@@ -87,41 +95,68 @@ __global__ __launch_bounds__(1024) void k_invert_rows(const int32_t* __restrict_
     for (int c0 = 0; c0 < n; c0 += chunk) {
       const int clen = min(chunk, n - c0);
       if (validate)
-        for (int c = tid; c < clen; c += bs) lds[c] = -1;
+        for (int c = tid; c < (clen + 1) / 2; c += kInvBlock) lds32[c] = 0xFFFFFFFFu;
       __syncthreads();
       if constexpr (VEC == 4) {
         const int4* src4 = reinterpret_cast<const int4*>(src);
-        for (int q = tid; q < (n >> 2); q += bs) {
-          int4 v = src4[q];
-          const int r = q << 2;
-          unsigned u;
-          u = (unsigned)(v.x - c0); if (u < (unsigned)clen) lds[u] = r;     else bad |= (unsigned)v.x >= (unsigned)n;
-          u = (unsigned)(v.y - c0); if (u < (unsigned)clen) lds[u] = r + 1; else bad |= (unsigned)v.y >= (unsigned)n;
-          u = (unsigned)(v.z - c0); if (u < (unsigned)clen) lds[u] = r + 2; else bad |= (unsigned)v.z >= (unsigned)n;
-          u = (unsigned)(v.w - c0); if (u < (unsigned)clen) lds[u] = r + 3; else bad |= (unsigned)v.w >= (unsigned)n;
+        const int nq = n >> 2;
+        for (int q0 = tid; q0 < nq; q0 += kInvBlock * kBlk) {
+          int4 v[kBlk];
+#pragma unroll
+          for (int j = 0; j < kBlk; ++j) {
+            const int q = q0 + j * kInvBlock;
+            if (q < nq) v[j] = src4[q];
+          }
+#pragma unroll
+          for (int j = 0; j < kBlk; ++j) {
+            const int q = q0 + j * kInvBlock;
+            if (q >= nq) break;
+            const int r = q << 2;
+            const int e[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const unsigned u = (unsigned)(e[t] - c0);
+              if (u < (unsigned)clen) lds16[u] = (uint16_t)(r + t);
+              else bad |= (unsigned)e[t] >= (unsigned)n;
+            }
+          }
         }
       } else {
-        for (int r = tid; r < n; r += bs) {
-          int v = src[r];
-          unsigned u = (unsigned)(v - c0);
-          if (u < (unsigned)clen) lds[u] = r;
-          else bad |= (unsigned)v >= (unsigned)n;
+        for (int r0 = tid; r0 < n; r0 += kInvBlock * kBlk) {
+          int v[kBlk];
+#pragma unroll
+          for (int j = 0; j < kBlk; ++j) {
+            const int r = r0 + j * kInvBlock;
+            if (r < n) v[j] = src[r];
+          }
+#pragma unroll
+          for (int j = 0; j < kBlk; ++j) {
+            const int r = r0 + j * kInvBlock;
+            if (r >= n) break;
+            const unsigned u = (unsigned)(v[j] - c0);
+            if (u < (unsigned)clen) lds16[u] = (uint16_t)r;
+            else bad |= (unsigned)v[j] >= (unsigned)n;
+          }
         }
       }
       __syncthreads();
       if constexpr (VEC == 4) {
+        const uint2* l2 = reinterpret_cast<const uint2*>(lds16);
         int4* dst4 = reinterpret_cast<int4*>(dst + c0);
-        const int4* lds4 = reinterpret_cast<const int4*>(lds);
-        for (int q = tid; q < (clen >> 2); q += bs) {
-          int4 v = lds4[q];
-          if (validate && (v.x < 0 || v.y < 0 || v.z < 0 || v.w < 0)) bad |= 2;
-          dst4[q] = v;
+        for (int q = tid; q < (clen >> 2); q += kInvBlock) {
+          const uint2 p = l2[q];
+          const int4 o = make_int4((int)(p.x & 0xFFFFu), (int)(p.x >> 16), (int)(p.y & 0xFFFFu),
+                                   (int)(p.y >> 16));
+          if (validate && ((p.x & 0xFFFFu) == 0xFFFFu || (p.x >> 16) == 0xFFFFu ||
+                           (p.y & 0xFFFFu) == 0xFFFFu || (p.y >> 16) == 0xFFFFu))
+            bad |= 2;
+          dst4[q] = o;
         }
       } else {
-        for (int c = tid; c < clen; c += bs) {
-          int v = lds[c];
-          if (validate && v < 0) bad |= 2;
-          dst[c0 + c] = v;
+        for (int c = tid; c < clen; c += kInvBlock) {
+          const uint32_t v = lds16[c];
+          if (validate && v == 0xFFFFu) bad |= 2;
+          dst[c0 + c] = (int32_t)v;
         }
       }
       __syncthreads();
@@ -208,7 +243,12 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   const int gwave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   int next_init = gwave;
   int m = -1, k = 0;
+  // Speculation: while a candidate's atomicMin is in flight, the segment of the
+  // man it would displace is loaded too; (spec_m, spec_k) says whose it is.
+  int spec_m = -1, spec_k = -1;
+  uint32_t spec_e = 0;
   unsigned long long n_atomics = 0, n_steps = 0;
+  int since_poll = 0;
   if (__hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
     next_init = n;  // init found malformed lists: do nothing
   }
@@ -227,7 +267,8 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       break;
     }
     int act = 0;
-    if (T > 0 && lane == 0)
+    const bool poll = T > 0 && ++since_poll >= a.poll;
+    if (poll && lane == 0)
       act = __hip_atomic_load(&a.sc->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ++n_steps;
     const int kk = k + lane;
@@ -235,7 +276,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     uint32_t w = 0, r = 0;
     if (valid) {
       if (kk < K) {
-        const uint32_t e = a.pr[(size_t)m * K + kk];
+        const uint32_t e = (spec_m == m && spec_k == k) ? spec_e : a.pr[(size_t)m * K + kk];
         w = pr_woman(e);
         r = pr_rank(e);
       } else {
@@ -250,6 +291,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       }
       valid = valid && w < (uint32_t)n;
     }
+    spec_m = -1;
     const unsigned long long mine = pw_pack(r, (uint32_t)m, (uint32_t)kk);
     unsigned long long hint = 0;
     if (valid) {
@@ -263,6 +305,15 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       const int j = __ffsll((long long)cand) - 1;
       unsigned long long old = 0;
       if (lane == j) old = atomicMin(&a.partner[w], mine);
+      // The man this attempt would displace, by the hint; his next segment is
+      // loaded while the atomic is in flight.
+      const unsigned long long hj = __shfl(hint, j);
+      int sm = -1, sk = -1;
+      if (hj != kEmpty) {
+        sm = (int)pw_man(hj);
+        sk = (int)pw_pos(hj) + 1;
+        if (sk + lane < K && sk + lane < n) spec_e = a.pr[(size_t)sm * K + sk + lane];
+      }
       old = __shfl(old, j);
       const unsigned long long mj = __shfl(mine, j);
       ++n_atomics;
@@ -272,6 +323,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         } else {
           m = (int)pw_man(old);
           k = (int)pw_pos(old) + 1;
+          if (m == sm && k == sk) spec_m = sm, spec_k = sk;
         }
         moved = true;
         break;
@@ -279,7 +331,8 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       cand &= cand - 1;
     }
     if (!moved) k += 64;
-    if (T > 0) {
+    if (poll) {
+      since_poll = 0;
       act = __shfl(act, 0);
       if (act <= T) {
         if (lane == 0) {
@@ -391,15 +444,16 @@ hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipS
 }
 
 // chunk: columns of one LDS pass (<= max LDS bytes / 4).
+// chunk: columns of one LDS pass (a multiple of 4, <= max LDS bytes / 2).
 hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
-                              int threads, Scalars* sc, hipStream_t s) {
-  const size_t lds = (size_t)chunk * sizeof(int32_t);
+                              Scalars* sc, hipStream_t s) {
+  const size_t lds = (size_t)((chunk + 3) & ~3) * sizeof(uint16_t);
   const bool vec = (n % 4 == 0) && (chunk % 4 == 0);
   if (vec)
-    hipLaunchKernelGGL(k_invert_rows<4>, dim3(n), dim3(threads), lds, s, pref, rank, n, chunk,
+    hipLaunchKernelGGL(k_invert_rows<4>, dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
                        validate, sc);
   else
-    hipLaunchKernelGGL(k_invert_rows<1>, dim3(n), dim3(threads), lds, s, pref, rank, n, chunk,
+    hipLaunchKernelGGL(k_invert_rows<1>, dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
                        validate, sc);
   return hipGetLastError();
 }
