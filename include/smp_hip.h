@@ -109,7 +109,7 @@ typedef struct smp_opts {
                              (default); 0 = plain cached reads (may be stale, which
                              costs failed atomics, never correctness)             */
   int32_t poll_interval;  /* proposal-kernel steps between reads of the shared
-                             active-wave counter that triggers the hand-off; 0 = 4 */
+                             active-wave counter that triggers the hand-off; 0 = 16 */
   int32_t reserved[7];
 } smp_opts;
 
@@ -122,15 +122,19 @@ typedef struct smp_times {
   double rank_kernel_ms, gather_kernel_ms, propose_kernel_ms;
   /* Host time of the hand-off (copies) and of the host tail loop. */
   double handoff_ms, host_tail_ms;
+  double copy_wait_ms;     /* host wait for the PR-prefix copy after the kernel launch */
+  double warm_ms;          /* host cache warm-up of that copy                          */
+  double resume_ms;        /* proposal launch for chains the host handed back          */
   int64_t proposals;       /* sum over men of (final list position + 1)           */
-  int64_t atomics;         /* atomicMin attempts made by the proposal kernel      */
+  int64_t atomics;         /* atomicMin attempts made by the proposal kernel(s)   */
   int64_t host_proposals;  /* proposals made by the host tail                     */
   int64_t steps;           /* 64-entry segment scans made by the proposal kernel  */
   int32_t is_perfect;      /* 1 if the first-choice fast path answered            */
   int32_t handoff_free;    /* free men handed to the host (0 = GPU finished)      */
   int32_t prefix_cols;     /* PR columns actually gathered                        */
   int32_t tail_threshold;  /* threshold used                                      */
-  int32_t host_row_fetches;/* PR rows the host tail fetched beyond the prefix     */
+  int32_t bailed;          /* chains the host handed back to the GPU because they
+                              walked past the PR prefix                           */
   int32_t reserved[7];
 } smp_times;
 

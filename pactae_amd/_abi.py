@@ -60,6 +60,9 @@ class SmpTimes(ctypes.Structure):
         ("propose_kernel_ms", ctypes.c_double),
         ("handoff_ms", ctypes.c_double),
         ("host_tail_ms", ctypes.c_double),
+        ("copy_wait_ms", ctypes.c_double),
+        ("warm_ms", ctypes.c_double),
+        ("resume_ms", ctypes.c_double),
         ("proposals", i64),
         ("atomics", i64),
         ("host_proposals", i64),
@@ -68,7 +71,7 @@ class SmpTimes(ctypes.Structure):
         ("handoff_free", i32),
         ("prefix_cols", i32),
         ("tail_threshold", i32),
-        ("host_row_fetches", i32),
+        ("bailed", i32),
         ("reserved", i32 * 7),
     ]
 

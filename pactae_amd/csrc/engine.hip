@@ -29,7 +29,7 @@ void smp_default_opts(smp_opts* o) {
   o->waves_per_cu = 0;
   o->validate = 1;
   o->coherent_hint = 0;
-  o->poll_interval = 4;
+  o->poll_interval = 16;
 }
 
 static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
@@ -120,7 +120,7 @@ int smp_result(smp_engine* e, int32_t* match) {
     pactae::set_error("smp_result before a successful smp_exec");
     return SMP_ERR_STATE;
   }
-  std::memcpy(match, e->h_match.data(), sizeof(int32_t) * e->n);
+  std::memcpy(match, e->h_match, sizeof(int32_t) * e->n);
   return SMP_OK;
 }
 

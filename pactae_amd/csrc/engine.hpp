@@ -133,13 +133,17 @@ struct smp_engine : public pactae::AbsSmpEngine {
   Scalars* h_sc = nullptr;
   int2* h_parked = nullptr;
   uint32_t* h_rowbuf = nullptr;
-  std::vector<int32_t> h_match;
+  int32_t* h_match = nullptr;  // pinned result (man -> woman)
+  int2* d_starts = nullptr;     // chains handed back by the host tail
+  bool host_finished = false;   // the host tail completed every chain
 
   hipStream_t stream = nullptr, copy_stream = nullptr;
   hipEvent_t ev[8] = {};
 
   smp_times times{};
   volatile uint64_t touch_sink = 0;
+  long long host_proposals_total = 0;
+  std::vector<uint32_t> hr, hm, hp;  // host tail: partner rank / man / position per woman
   static constexpr int kRowFetch = 2048;
 
   ~smp_engine() override { release(); }
@@ -162,6 +166,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipFree(d_rowbuf);
     (void)hipFree(d_wstats);
     d_wstats = nullptr;
+    (void)hipFree(d_starts);
+    d_starts = nullptr;
+    (void)hipHostFree(h_match);
+    h_match = nullptr;
     (void)hipHostFree(h_pr);
     (void)hipHostFree(h_partner);
     (void)hipHostFree(h_sc);
@@ -222,7 +230,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
-    h_match.assign(n, -1);
+    if (hipHostMalloc(&h_match, (size_t)n * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&d_starts, (size_t)n * sizeof(int2)) != hipSuccess) {
+      (void)hipGetLastError();
+      pactae::set_error("allocation of the result buffers failed");
+      return SMP_ERR_NOMEM;
+    }
     return SMP_OK;
   }
 
@@ -381,7 +394,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   }
 
   // -- core: persistent proposal kernel + host tail --------------------------
-  int CoreProc() override {
+  pactae::ProposeArgs propose_args(int T_, const int2* starts, int nstarts) const {
     pactae::ProposeArgs a;
     a.pr = d_pr;
     a.pref_m = d_pref_m;
@@ -392,18 +405,30 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.wstats = d_wstats;
     a.n = n;
     a.K = K;
-    a.T = T;
+    a.T = T_;
     a.coherent_hint = opts.coherent_hint;
-    a.poll = opts.poll_interval > 0 ? opts.poll_interval : 4;
+    a.poll = opts.poll_interval > 0 ? opts.poll_interval : 16;
+    a.starts = starts;
+    a.nstarts = nstarts;
+    return a;
+  }
+
+  int CoreProc() override {
+    host_finished = false;
     const int grid = propose_grid();
     SMP_HIP_TRY(hipEventRecord(ev[3], stream));
-    SMP_HIP_TRY(pactae::launch_propose(a, grid, stream));
+    SMP_HIP_TRY(pactae::launch_propose(propose_args(T, nullptr, 0), grid, stream));
     SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+    SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
     if (T > 0) {
       // While the kernel runs: wait for the PR prefix copy and pull it into the
       // host caches (a DMA-written buffer starts out in DRAM only).
+      auto t0 = Clock::now();
       SMP_HIP_TRY(hipStreamSynchronize(copy_stream));
+      times.copy_wait_ms = ms_since(t0);
+      t0 = Clock::now();
       touch_host_prefix();
+      times.warm_ms = ms_since(t0);
     }
     if (int st = read_scalars()) return st;
     if (int st = check_device_error("proposal kernel")) return st;
@@ -426,7 +451,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // Host tail: McVitie-Wilson chains (smp_engine_mw_1_f9.cu:15-79, and the
   // reference's host LAProcedure smp_engine_bamboo.cu:197-230) over the partner
   // state the kernel left, split into rank / man / position arrays that stay in
-  // the host caches.
+  // the host caches.  A chain that walks past the PR prefix is handed back: it
+  // is collected, and once every other chain is done the GPU resumes all of them
+  // from the shared partner words (the host never stalls on single-row fetches).
   int host_tail() {
     auto t0 = Clock::now();
     SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
@@ -435,50 +462,35 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)parked * sizeof(int2),
                                hipMemcpyDeviceToHost, stream));
     SMP_HIP_TRY(hipStreamSynchronize(stream));
-    SMP_HIP_TRY(hipStreamSynchronize(copy_stream));
     times.handoff_ms = ms_since(t0);
 
     t0 = Clock::now();
     constexpr uint32_t kFree = 0xFFFFFFFFu;
-    std::vector<uint32_t> hr(n), hm(n), hp(n);
+    hr.resize(n);
+    hm.resize(n);
+    hp.resize(n);
     for (int w = 0; w < n; ++w) {
       const unsigned long long v = h_partner[w];
       hr[w] = v == pactae::kEmpty ? kFree : pactae::pw_rank(v);
       hm[w] = pactae::pw_man(v);
       hp[w] = pactae::pw_pos(v);
     }
-    struct Ext {
-      int k0 = 0;
-      std::vector<uint32_t> e;
-    };
-    std::unordered_map<int, Ext> ext;
     long long props = 0;
-    int fetches = 0;
+    int bailed = 0;
     const uint32_t* pr = h_pr;
+    int2* bail = h_parked;  // handed-back chains overwrite the consumed parked list
     for (int i = 0; i < parked; ++i) {
       int m = h_parked[i].x, k = h_parked[i].y;
+      if ((unsigned)m >= (unsigned)n) {
+        pactae::set_error("host tail: parked man outside [0,n)");
+        return SMP_ERR_INVALID;
+      }
       for (;;) {
-        if (k >= n) {
-          pactae::set_error("host tail: a man exhausted his list (rows are not permutations)");
-          return SMP_ERR_INVALID;
+        if (k >= K || k >= n) {
+          bail[bailed++] = make_int2(m, k);
+          break;
         }
-        uint32_t e;
-        if (k < K) {
-          e = pr[(size_t)m * K + k];
-        } else {
-          Ext& x = ext[m];
-          if (x.e.empty() || k < x.k0 || k >= x.k0 + (int)x.e.size()) {
-            const int cnt = std::min(kRowFetch, n - k);
-            SMP_HIP_TRY(pactae::launch_gather_row(d_pref_m, d_rank_w, n, m, k, cnt, d_rowbuf, stream));
-            SMP_HIP_TRY(hipMemcpyAsync(h_rowbuf, d_rowbuf, cnt * sizeof(uint32_t),
-                                       hipMemcpyDeviceToHost, stream));
-            SMP_HIP_TRY(hipStreamSynchronize(stream));
-            x.k0 = k;
-            x.e.assign(h_rowbuf, h_rowbuf + cnt);
-            ++fetches;
-          }
-          e = x.e[k - x.k0];
-        }
+        const uint32_t e = pr[(size_t)m * K + k];
         ++props;
         const uint32_t w = pactae::pr_woman(e), r = pactae::pr_rank(e);
         if (w >= (uint32_t)n) {
@@ -498,25 +510,62 @@ struct smp_engine : public pactae::AbsSmpEngine {
         }
       }
     }
+    times.host_proposals = props;
+    times.bailed = bailed;
+    if (bailed == 0) {
+      // Every chain finished on the host: assemble the matching here.
+      long long total = 0;
+      for (int w = 0; w < n; ++w) {
+        if (hr[w] == kFree) {
+          pactae::set_error("host tail: a woman was left unmatched");
+          return SMP_ERR_INVALID;
+        }
+        h_match[hm[w]] = w;
+        total += hp[w] + 1ll;
+      }
+      host_proposals_total = total;
+      host_finished = true;
+      times.host_tail_ms = ms_since(t0);
+      return SMP_OK;
+    }
     for (int w = 0; w < n; ++w)
       h_partner[w] = hr[w] == kFree ? pactae::kEmpty : pactae::pw_pack(hr[w], hm[w], hp[w]);
-    times.host_proposals = props;
-    times.host_row_fetches = fetches;
+    times.host_tail_ms = ms_since(t0);
+
+    // Resume the handed-back chains on the GPU (no hand-off this time).
+    t0 = Clock::now();
     SMP_HIP_TRY(hipMemcpyAsync(d_partner, h_partner, (size_t)n * sizeof(unsigned long long),
                                hipMemcpyHostToDevice, stream));
-    times.host_tail_ms = ms_since(t0);
+    SMP_HIP_TRY(hipMemcpyAsync(d_starts, bail, (size_t)bailed * sizeof(int2), hipMemcpyHostToDevice,
+                               stream));
+    const int grid = std::max(1, std::min(propose_grid(), (bailed + 3) / 4));
+    SMP_HIP_TRY(pactae::launch_fill_i32(&d_sc->active, 1, std::min(bailed, grid * 4), stream));
+    SMP_HIP_TRY(pactae::launch_propose(propose_args(0, d_starts, bailed), grid, stream));
+    SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
+    SMP_HIP_TRY(hipStreamSynchronize(stream));
+    times.resume_ms = ms_since(t0);
     return SMP_OK;
   }
 
   // -- post (smp_engine_bamboo.cu:289-306) ----------------------------------
   int PostProc(bool perfect) override {
+    if (host_finished) {
+      // The host holds the matching; mirror it to the device result buffer.
+      SMP_HIP_TRY(hipMemcpyAsync(d_match, h_match, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 stream));
+      SMP_HIP_TRY(hipStreamSynchronize(stream));
+      times.proposals = host_proposals_total;
+      times.atomics = (int64_t)h_sc->atomics;
+      times.steps = (int64_t)h_sc->steps;
+      return SMP_OK;
+    }
     if (perfect) {
       SMP_HIP_TRY(pactae::launch_first_match(d_pref_m, n, d_match, stream));
     } else {
-      SMP_HIP_TRY(pactae::launch_post(d_partner, n, d_match, d_sc, d_wstats, propose_grid() * 4, stream));
+      SMP_HIP_TRY(pactae::launch_post(d_partner, n, d_match, d_sc, stream));
     }
-    SMP_HIP_TRY(hipMemcpyAsync(h_match.data(), d_match, (size_t)n * sizeof(int32_t),
-                               hipMemcpyDeviceToHost, stream));
+    SMP_HIP_TRY(hipMemcpyAsync(h_match, d_match, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               stream));
     SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
     SMP_HIP_TRY(hipStreamSynchronize(stream));
     if (int st = check_device_error("post")) return st;

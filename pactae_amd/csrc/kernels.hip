@@ -241,6 +241,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   const int n = a.n, K = a.K, T = a.T;
   const int waves = (int)(gridDim.x * (blockDim.x >> 6));
   const int gwave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int total = a.starts ? a.nstarts : n;  // work items: men, or resumed chains
   int next_init = gwave;
   int m = -1, k = 0;
   // Speculation: while a candidate's atomicMin is in flight, the segment of the
@@ -250,17 +251,28 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   unsigned long long n_atomics = 0, n_steps = 0;
   int since_poll = 0;
   if (__hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-    next_init = n;  // init found malformed lists: do nothing
+    next_init = total;  // init found malformed lists: do nothing
   }
   for (;;) {
     if (m < 0) {
-      if (next_init >= n) {
-        if (lane == 0 && gwave < n) atomicSub(&a.sc->active, 1);
+      if (next_init >= total) {
+        if (lane == 0 && gwave < total) atomicSub(&a.sc->active, 1);
         break;
       }
-      m = next_init;
+      if (a.starts) {
+        const int2 st = a.starts[next_init];
+        m = st.x;
+        k = st.y;
+      } else {
+        m = next_init;
+        k = 0;
+      }
       next_init += waves;
-      k = 0;
+      if ((unsigned)m >= (unsigned)n) {  // a resume list holds only men of this instance
+        if (lane == 0) atomicOr(&a.sc->error, kErrEntryRange);
+        m = -1;
+        continue;
+      }
     }
     if (k >= n) {  // only possible when rows are not permutations
       if (lane == 0) atomicOr(&a.sc->error, kErrListEnd);
@@ -337,8 +349,9 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       if (act <= T) {
         if (lane == 0) {
           if (m >= 0) a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(m, k);
-          for (; next_init < n; next_init += waves)
-            a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(next_init, 0);
+          for (; next_init < total; next_init += waves)
+            a.parked[atomicAdd(&a.sc->parked_count, 1)] =
+                a.starts ? a.starts[next_init] : make_int2(next_init, 0);
         }
         break;
       }
@@ -350,33 +363,34 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   }
 }
 
+// Adds the proposal kernel's per-wave counters into the control block.
+__global__ __launch_bounds__(256) void k_fold_wstats(const unsigned long long* __restrict__ wstats,
+                                                     int waves, Scalars* sc) {
+  __shared__ unsigned long long red[2][4];
+  unsigned long long at = 0, st = 0;
+  for (int g = threadIdx.x; g < waves; g += blockDim.x) {
+    at += wstats[2 * g];
+    st += wstats[2 * g + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    at += __shfl_down(at, off);
+    st += __shfl_down(st, off);
+  }
+  if ((threadIdx.x & 63) == 0) red[0][threadIdx.x >> 6] = at, red[1][threadIdx.x >> 6] = st;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sc->atomics += red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    sc->steps += red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Post: match[man] = woman from the partner words; also sums (position + 1)
 // over all women, which equals the total number of proposals of any
 // Gale-Shapley run (each man proposes down to his final partner exactly once).
 __global__ __launch_bounds__(256) void k_post(const unsigned long long* __restrict__ partner, int n,
-                                              int32_t* __restrict__ match, Scalars* sc,
-                                              const unsigned long long* __restrict__ wstats,
-                                              int waves) {
+                                              int32_t* __restrict__ match, Scalars* sc) {
   __shared__ unsigned long long red[4];
-  if (blockIdx.x == 0 && wstats) {  // fold the proposal kernel's per-wave counters
-    __shared__ unsigned long long red2[2][4];
-    unsigned long long at = 0, st = 0;
-    for (int g = threadIdx.x; g < waves; g += blockDim.x) {
-      at += wstats[2 * g];
-      st += wstats[2 * g + 1];
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      at += __shfl_down(at, off);
-      st += __shfl_down(st, off);
-    }
-    if ((threadIdx.x & 63) == 0) red2[0][threadIdx.x >> 6] = at, red2[1][threadIdx.x >> 6] = st;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      sc->atomics = red2[0][0] + red2[0][1] + red2[0][2] + red2[0][3];
-      sc->steps = red2[1][0] + red2[1][1] + red2[1][2] + red2[1][3];
-    }
-  }
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long props = 0;
   if (w < n) {
@@ -485,15 +499,19 @@ hipError_t launch_gather_row(const int32_t* pref_m, const int32_t* rank_w, int n
   return hipGetLastError();
 }
 
+hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_fold_wstats, dim3(1), dim3(256), 0, s, wstats, waves, sc);
+  return hipGetLastError();
+}
+
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
-                       const unsigned long long* wstats, int waves, hipStream_t s) {
-  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, partner, n, match, sc, wstats,
-                     waves);
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, partner, n, match, sc);
   return hipGetLastError();
 }
 

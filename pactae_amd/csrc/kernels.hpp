@@ -18,6 +18,8 @@ struct ProposeArgs {
   int n, K, T;
   int coherent_hint;           // 1: partner hints bypass non-coherent caches
   int poll;                    // steps between reads of the shared `active` counter
+  const int2* starts;          // resume mode: (man, position) pairs to start from;
+  int nstarts;                 //   null = every man from position 0
 };
 
 hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s);
@@ -34,7 +36,8 @@ hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32
 hipError_t launch_gather_row(const int32_t* pref_m, const int32_t* rank_w, int n, int m, int k0,
                              int count, uint32_t* out, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
+hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);
 hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
-                       const unsigned long long* wstats, int waves, hipStream_t s);
+                       hipStream_t s);
 
 }  // namespace pactae

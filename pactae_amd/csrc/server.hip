@@ -420,7 +420,7 @@ struct smp_server {
           int64_t props = 0;
           std::string err;
           if (st == SMP_OK) {
-            res.assign(plain->h_match.begin(), plain->h_match.begin() + r->n);
+            res.assign(plain->h_match, plain->h_match + r->n);
             props = plain->times.proposals;
           } else {
             err = pactae::last_error();

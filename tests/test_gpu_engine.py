@@ -18,6 +18,8 @@ TAIL_CONFIGS = [
     dict(tail_mode=1, prefix_cols=1),           # almost everything via the direct path
     dict(tail_mode=2, tail_threshold=8, prefix_cols=64),
     dict(tail_mode=1, prefix_cols=100000),      # full n x n PR matrix (reference layout)
+    dict(tail_mode=2, tail_threshold=37, prefix_cols=1),  # every host chain is handed back
+    dict(tail_mode=2, tail_threshold=5, poll_interval=1, coherent_hint=1),
 ]
 
 
