@@ -181,13 +181,16 @@ int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols) {
     return SMP_ERR_INVALID;
   }
   SMP_HIP_TRY(hipStreamSynchronize(e->stream));
-  const size_t cnt = (size_t)e->n * e->K;
-  std::vector<uint32_t> packed(cnt);
-  SMP_HIP_TRY(hipMemcpy(packed.data(), e->d_pr, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < cnt; ++i) {
-    pr_out[2 * i] = (int32_t)pactae::pr_woman(packed[i]);
-    pr_out[2 * i + 1] = (int32_t)pactae::pr_rank(packed[i]);
-  }
+  const int n = e->n, K = e->K;
+  std::vector<uint32_t> packed(pactae::pr_entries(n, K));
+  SMP_HIP_TRY(hipMemcpy(packed.data(), e->d_pr, packed.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (int m = 0; m < n; ++m)
+    for (int k = 0; k < K; ++k) {
+      const uint32_t v = packed[pactae::pr_index(m, k, n)];
+      const size_t i = (size_t)m * K + k;
+      pr_out[2 * i] = (int32_t)pactae::pr_woman(v);
+      pr_out[2 * i + 1] = (int32_t)pactae::pr_rank(v);
+    }
   *cols = e->K;
   return SMP_OK;
 }

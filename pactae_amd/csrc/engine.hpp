@@ -137,7 +137,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int2* d_starts = nullptr;     // chains handed back by the host tail
   bool host_finished = false;   // the host tail completed every chain
 
-  hipStream_t stream = nullptr, copy_stream = nullptr;
+  hipStream_t stream = nullptr, copy_stream = nullptr, copy_stream2 = nullptr;
+  bool rest_synced = false;  // the PR blocks past block 0 have reached the host
   hipEvent_t ev[8] = {};
 
   smp_times times{};
@@ -151,6 +152,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   void release() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    if (copy_stream2) (void)hipStreamSynchronize(copy_stream2);
     if (owns_lists) {
       (void)hipFree(const_cast<int32_t*>(d_pref_m));
       (void)hipFree(const_cast<int32_t*>(d_pref_w));
@@ -192,6 +194,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (stream) (void)hipStreamDestroy(stream), stream = nullptr;
     if (copy_stream) (void)hipStreamDestroy(copy_stream), copy_stream = nullptr;
+    if (copy_stream2) (void)hipStreamDestroy(copy_stream2), copy_stream2 = nullptr;
   }
 
   int setup() {
@@ -202,9 +205,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     configure(n);
     SMP_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+    SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream2, hipStreamNonBlocking));
     for (auto& e : ev) SMP_HIP_TRY(hipEventCreate(&e));
     if (hipMalloc(&d_rank_w, nn * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&d_pr, (size_t)n * K_alloc * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_partner, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&d_match, (size_t)n * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&d_seen, (size_t)n * sizeof(int)) != hipSuccess ||
@@ -216,7 +220,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error("device allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
     }
-    if (hipHostMalloc(&h_pr, (size_t)n * K_alloc * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc(&h_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t), hipHostMallocDefault) !=
+            hipSuccess ||
         hipHostMalloc(&h_partner, (size_t)n * sizeof(unsigned long long), hipHostMallocDefault) !=
             hipSuccess ||
         hipHostMalloc(&h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
@@ -379,18 +384,32 @@ struct smp_engine : public pactae::AbsSmpEngine {
     const int chunk = invert_chunk();
     SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
     SMP_HIP_TRY(hipEventRecord(ev[0], stream));
-    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream));
+    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream,
+                                           /*nontemporal=*/1));
     SMP_HIP_TRY(hipEventRecord(ev[1], stream));
     SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
     SMP_HIP_TRY(hipEventRecord(ev[2], stream));
+    rest_synced = false;
     if (T > 0) {
-      // PR prefix to the host, overlapped with the proposal kernel.
+      // PR prefix to the host, overlapped with the proposal kernel: block 0 (the
+      // positions the host tail nearly always reads) on one stream, the other
+      // blocks on a second.
+      const size_t b0 = (size_t)n * pactae::kPrBlock;
+      const size_t all = pactae::pr_entries(n, K);
       SMP_HIP_TRY(hipStreamWaitEvent(copy_stream, ev[2], 0));
-      SMP_HIP_TRY(hipMemcpyAsync(h_pr, d_pr, (size_t)n * K * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 copy_stream));
+      SMP_HIP_TRY(hipMemcpyAsync(h_pr, d_pr, std::min(b0, all) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, copy_stream));
+      if (all > b0) {
+        SMP_HIP_TRY(hipStreamWaitEvent(copy_stream2, ev[2], 0));
+        SMP_HIP_TRY(hipMemcpyAsync(h_pr + b0, d_pr + b0, (all - b0) * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, copy_stream2));
+      } else {
+        rest_synced = true;
+      }
     }
-    if (int st = read_scalars()) return st;
-    return check_device_error("init");
+    // No wait here: the proposal kernel does nothing when init flagged malformed
+    // lists, and the error is reported after it.
+    return SMP_OK;
   }
 
   // -- core: persistent proposal kernel + host tail --------------------------
@@ -440,9 +459,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return SMP_OK;
   }
 
-  void touch_host_prefix() {
+  void touch_host_prefix() {  // block 0 only
     const uint64_t* p = reinterpret_cast<const uint64_t*>(h_pr);
-    const size_t words = (size_t)n * K / 2;
+    const size_t words = std::min((size_t)n * pactae::kPrBlock, pactae::pr_entries(n, K)) / 2;
     uint64_t acc = 0;
     for (size_t i = 0; i < words; i += 8) acc += p[i];  // one load per 64-byte line
     touch_sink = acc;
@@ -490,7 +509,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
           bail[bailed++] = make_int2(m, k);
           break;
         }
-        const uint32_t e = pr[(size_t)m * K + k];
+        if (k >= pactae::kPrBlock && !rest_synced) {
+          SMP_HIP_TRY(hipStreamSynchronize(copy_stream2));
+          rest_synced = true;
+        }
+        const uint32_t e = pr[pactae::pr_index(m, k, n)];
         ++props;
         const uint32_t w = pactae::pr_woman(e), r = pactae::pr_rank(e);
         if (w >= (uint32_t)n) {
@@ -587,7 +610,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     const int chunk = invert_chunk();
     auto t0 = Clock::now();
     SMP_HIP_TRY(hipEventRecord(ev[0], stream));
-    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream));
+    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream,
+                                           /*nontemporal=*/1));
     SMP_HIP_TRY(hipEventRecord(ev[1], stream));
     SMP_HIP_TRY(hipStreamSynchronize(stream));
     t->rank_ms = ms_since(t0);
@@ -603,7 +627,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     t0 = Clock::now();
     std::vector<uint32_t> host;
     if (init_pr) {
-      host.resize((size_t)n * K);
+      host.resize(pactae::pr_entries(n, K));
       SMP_HIP_TRY(hipMemcpy(host.data(), d_pr, host.size() * 4, hipMemcpyDeviceToHost));
     } else {
       host.resize((size_t)n * n);

@@ -81,7 +81,7 @@ __global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t
 constexpr int kInvBlock = 512;
 constexpr int kBlk = 8;
 
-template <int VEC>
+template <int VEC, int NT>
 __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __restrict__ pref,
                                                            int32_t* __restrict__ rank, int n,
                                                            int chunk, int validate, Scalars* sc) {
@@ -105,7 +105,16 @@ __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __rest
 #pragma unroll
           for (int j = 0; j < kBlk; ++j) {
             const int q = q0 + j * kInvBlock;
-            if (q < nq) v[j] = src4[q];
+            if (q < nq) {
+              if constexpr (NT) {
+                v[j].x = __builtin_nontemporal_load(&src4[q].x);
+                v[j].y = __builtin_nontemporal_load(&src4[q].y);
+                v[j].z = __builtin_nontemporal_load(&src4[q].z);
+                v[j].w = __builtin_nontemporal_load(&src4[q].w);
+              } else {
+                v[j] = src4[q];
+              }
+            }
           }
 #pragma unroll
           for (int j = 0; j < kBlk; ++j) {
@@ -150,7 +159,14 @@ __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __rest
           if (validate && ((p.x & 0xFFFFu) == 0xFFFFu || (p.x >> 16) == 0xFFFFu ||
                            (p.y & 0xFFFFu) == 0xFFFFu || (p.y >> 16) == 0xFFFFu))
             bad |= 2;
-          dst4[q] = o;
+          if constexpr (NT) {
+            __builtin_nontemporal_store(o.x, &dst4[q].x);
+            __builtin_nontemporal_store(o.y, &dst4[q].y);
+            __builtin_nontemporal_store(o.z, &dst4[q].z);
+            __builtin_nontemporal_store(o.w, &dst4[q].w);
+          } else {
+            dst4[q] = o;
+          }
         }
       } else {
         for (int c = tid; c < clen; c += kInvBlock) {
@@ -167,7 +183,8 @@ __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __rest
 }
 
 // ---------------------------------------------------------------------------
-// PR gather: pr[m*K + k] = (w = pref_m[m][k], rank_w[w][m]) for k < K.
+// PR gather: pr[m, k] = (w = pref_m[m][k], rank_w[w][m]) for k < K, stored
+// column-blocked (pr_index).
 // Tiles are 32 men x 256 columns.  The rank reads rank_w[w][m0..m0+32) of one
 // men block share 128-byte lines, so tiles of the same men block are kept on one
 // XCD (workgroup b runs on XCD b % 8) and consecutive in its dispatch order,
@@ -199,7 +216,7 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
       bad = 1;
       e = pr_pack(0, 0xFFFFu);
     }
-    pr[(size_t)m * K + k] = e;
+    pr[pr_index(m, k, n)] = e;
   }
   if (bad) atomicOr(&sc->error, kErrEntryRange);
 }
@@ -288,7 +305,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     uint32_t w = 0, r = 0;
     if (valid) {
       if (kk < K) {
-        const uint32_t e = (spec_m == m && spec_k == k) ? spec_e : a.pr[(size_t)m * K + kk];
+        const uint32_t e = (spec_m == m && spec_k == k) ? spec_e : a.pr[pr_index(m, kk, n)];
         w = pr_woman(e);
         r = pr_rank(e);
       } else {
@@ -324,7 +341,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       if (hj != kEmpty) {
         sm = (int)pw_man(hj);
         sk = (int)pw_pos(hj) + 1;
-        if (sk + lane < K && sk + lane < n) spec_e = a.pr[(size_t)sm * K + sk + lane];
+        if (sk + lane < K && sk + lane < n) spec_e = a.pr[pr_index(sm, sk + lane, n)];
       }
       old = __shfl(old, j);
       const unsigned long long mj = __shfl(mine, j);
@@ -460,24 +477,30 @@ hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipS
 // chunk: columns of one LDS pass (<= max LDS bytes / 4).
 // chunk: columns of one LDS pass (a multiple of 4, <= max LDS bytes / 2).
 hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
-                              Scalars* sc, hipStream_t s) {
+                              Scalars* sc, hipStream_t s, int nontemporal) {
   const size_t lds = (size_t)((chunk + 3) & ~3) * sizeof(uint16_t);
   const bool vec = (n % 4 == 0) && (chunk % 4 == 0);
-  if (vec)
-    hipLaunchKernelGGL(k_invert_rows<4>, dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
+  if (vec && nontemporal)
+    hipLaunchKernelGGL((k_invert_rows<4, 1>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
+                       validate, sc);  // streaming: rank rows are written once, read much later
+  else if (vec)
+    hipLaunchKernelGGL((k_invert_rows<4, 0>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
                        validate, sc);
   else
-    hipLaunchKernelGGL(k_invert_rows<1>, dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
+    hipLaunchKernelGGL((k_invert_rows<1, 0>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
                        validate, sc);
   return hipGetLastError();
 }
 
 hipError_t set_invert_lds_limit(int bytes) {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_invert_rows<4>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_invert_rows<1>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  const void* fns[] = {reinterpret_cast<const void*>(&k_invert_rows<4, 1>),
+                       reinterpret_cast<const void*>(&k_invert_rows<4, 0>),
+                       reinterpret_cast<const void*>(&k_invert_rows<1, 0>)};
+  for (const void* f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,

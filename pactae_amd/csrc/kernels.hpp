@@ -29,7 +29,7 @@ hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s);
 hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars* sc, hipStream_t s);
 hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s);
 hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
-                              Scalars* sc, hipStream_t s);
+                              Scalars* sc, hipStream_t s, int nontemporal);
 hipError_t set_invert_lds_limit(int bytes);
 hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
                             Scalars* sc, hipStream_t s);

@@ -23,6 +23,20 @@ SMP_HD uint32_t pr_pack(uint32_t w, uint32_t r) { return (w << 16) | (r & 0xFFFF
 SMP_HD uint32_t pr_woman(uint32_t e) { return e >> 16; }
 SMP_HD uint32_t pr_rank(uint32_t e) { return e & 0xFFFFu; }
 
+// The PR prefix is stored column-blocked: blocks of kPrBlock list positions,
+// each block holding every man's kPrBlock entries contiguously
+// (index ((k / kPrBlock) * n + m) * kPrBlock + k % kPrBlock).  A 64-position
+// segment of one man is still two or three whole 128-byte lines for the GPU, and
+// the first block (positions 0..31 of every man, where nearly all late proposals
+// fall) is one contiguous piece that the host can copy and cache first.
+constexpr int kPrBlock = 32;
+SMP_HD size_t pr_index(int m, int k, int n) {
+  return ((size_t)(k >> 5) * (size_t)n + (size_t)m) * kPrBlock + (size_t)(k & (kPrBlock - 1));
+}
+SMP_HD size_t pr_entries(int n, int K) {  // allocation for positions [0, K)
+  return (size_t)((K + kPrBlock - 1) / kPrBlock) * (size_t)n * kPrBlock;
+}
+
 // Woman's partner word, claimed with a 64-bit atomicMin:
 //   bits 42..62 rank of the man in her list (decides the min: ranks are distinct)
 //   bits 21..41 the man
