@@ -138,7 +138,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
   bool host_finished = false;   // the host tail completed every chain
 
   hipStream_t stream = nullptr, copy_stream = nullptr, copy_stream2 = nullptr;
-  bool rest_synced = false;  // the PR blocks past block 0 have reached the host
+  static constexpr int kMaxBlocks = 8;  // PR prefix blocks copied to the host (K <= 256)
+  hipEvent_t ev_blk[kMaxBlocks] = {};
+  int blocks_copied = 0, blocks_ready = 0;
   hipEvent_t ev[8] = {};
 
   smp_times times{};
@@ -192,6 +194,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     h_rowbuf = nullptr;
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
+    for (auto& e : ev_blk)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
     if (stream) (void)hipStreamDestroy(stream), stream = nullptr;
     if (copy_stream) (void)hipStreamDestroy(copy_stream), copy_stream = nullptr;
     if (copy_stream2) (void)hipStreamDestroy(copy_stream2), copy_stream2 = nullptr;
@@ -207,6 +211,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream2, hipStreamNonBlocking));
     for (auto& e : ev) SMP_HIP_TRY(hipEventCreate(&e));
+    for (auto& e : ev_blk) SMP_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (hipMalloc(&d_rank_w, nn * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&d_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_partner, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
@@ -389,22 +394,23 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipEventRecord(ev[1], stream));
     SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
     SMP_HIP_TRY(hipEventRecord(ev[2], stream));
-    rest_synced = false;
+    blocks_copied = 0;
+    blocks_ready = 0;
     if (T > 0) {
-      // PR prefix to the host, overlapped with the proposal kernel: block 0 (the
-      // positions the host tail nearly always reads) on one stream, the other
-      // blocks on a second.
-      const size_t b0 = (size_t)n * pactae::kPrBlock;
-      const size_t all = pactae::pr_entries(n, K);
+      // PR prefix to the host, overlapped with the proposal kernel, one
+      // 32-position block per copy (block 0, where nearly all late proposals fall,
+      // first; the others alternate between two streams), each with an event so
+      // the host can cache a block as soon as it lands.
+      const size_t blk = (size_t)n * pactae::kPrBlock;
+      const int nb = (K + pactae::kPrBlock - 1) / pactae::kPrBlock;
       SMP_HIP_TRY(hipStreamWaitEvent(copy_stream, ev[2], 0));
-      SMP_HIP_TRY(hipMemcpyAsync(h_pr, d_pr, std::min(b0, all) * sizeof(uint32_t),
-                                 hipMemcpyDeviceToHost, copy_stream));
-      if (all > b0) {
-        SMP_HIP_TRY(hipStreamWaitEvent(copy_stream2, ev[2], 0));
-        SMP_HIP_TRY(hipMemcpyAsync(h_pr + b0, d_pr + b0, (all - b0) * sizeof(uint32_t),
-                                   hipMemcpyDeviceToHost, copy_stream2));
-      } else {
-        rest_synced = true;
+      SMP_HIP_TRY(hipStreamWaitEvent(copy_stream2, ev[2], 0));
+      for (int b = 0; b < nb && b < kMaxBlocks; ++b) {
+        hipStream_t cs = (b & 1) ? copy_stream2 : copy_stream;
+        SMP_HIP_TRY(hipMemcpyAsync(h_pr + b * blk, d_pr + b * blk, blk * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, cs));
+        SMP_HIP_TRY(hipEventRecord(ev_blk[b], cs));
+        blocks_copied = b + 1;
       }
     }
     // No wait here: the proposal kernel does nothing when init flagged malformed
@@ -440,14 +446,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipEventRecord(ev[4], stream));
     SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
     if (T > 0) {
-      // While the kernel runs: wait for the PR prefix copy and pull it into the
-      // host caches (a DMA-written buffer starts out in DRAM only).
-      auto t0 = Clock::now();
-      SMP_HIP_TRY(hipStreamSynchronize(copy_stream));
-      times.copy_wait_ms = ms_since(t0);
-      t0 = Clock::now();
-      touch_host_prefix();
-      times.warm_ms = ms_since(t0);
+      // While the kernel runs: pull the PR prefix blocks into the host caches as
+      // they land (a DMA-written buffer starts out in DRAM only).  Block 0 always;
+      // later blocks only while the kernel is still running.
+      for (int b = 0; b < blocks_copied; ++b) {
+        if (b > 0 && hipEventQuery(ev[4]) == hipSuccess) break;
+        auto t0 = Clock::now();
+        SMP_HIP_TRY(hipEventSynchronize(ev_blk[b]));
+        times.copy_wait_ms += ms_since(t0);
+        blocks_ready = b + 1;
+        t0 = Clock::now();
+        touch_block(b);
+        times.warm_ms += ms_since(t0);
+      }
     }
     if (int st = read_scalars()) return st;
     if (int st = check_device_error("proposal kernel")) return st;
@@ -459,11 +470,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return SMP_OK;
   }
 
-  void touch_host_prefix() {  // block 0 only
-    const uint64_t* p = reinterpret_cast<const uint64_t*>(h_pr);
-    const size_t words = std::min((size_t)n * pactae::kPrBlock, pactae::pr_entries(n, K)) / 2;
+  void touch_block(int b) {
+    const size_t blk = (size_t)n * pactae::kPrBlock;
+    const uint64_t* p = reinterpret_cast<const uint64_t*>(h_pr + b * blk);
     uint64_t acc = 0;
-    for (size_t i = 0; i < words; i += 8) acc += p[i];  // one load per 64-byte line
+    for (size_t i = 0; i < blk / 2; i += 8) acc += p[i];  // one load per 64-byte line
     touch_sink = acc;
   }
 
@@ -497,6 +508,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     long long props = 0;
     int bailed = 0;
     const uint32_t* pr = h_pr;
+    const int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
     int2* bail = h_parked;  // handed-back chains overwrite the consumed parked list
     for (int i = 0; i < parked; ++i) {
       int m = h_parked[i].x, k = h_parked[i].y;
@@ -505,13 +517,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
         return SMP_ERR_INVALID;
       }
       for (;;) {
-        if (k >= K || k >= n) {
+        if (k >= kcap) {
           bail[bailed++] = make_int2(m, k);
           break;
         }
-        if (k >= pactae::kPrBlock && !rest_synced) {
-          SMP_HIP_TRY(hipStreamSynchronize(copy_stream2));
-          rest_synced = true;
+        if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
+          while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(hipEventSynchronize(ev_blk[blocks_ready++]));
         }
         const uint32_t e = pr[pactae::pr_index(m, k, n)];
         ++props;
