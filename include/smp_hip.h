@@ -110,7 +110,9 @@ typedef struct smp_opts {
                              costs failed atomics, never correctness)             */
   int32_t poll_interval;  /* proposal-kernel steps between reads of the shared
                              active-wave counter that triggers the hand-off; 0 = 16 */
-  int32_t reserved[7];
+  int32_t host_spin;      /* 1 = the host polls for device completion (default),
+                             0 = it blocks in the runtime                         */
+  int32_t reserved[6];
 } smp_opts;
 
 typedef struct smp_times {

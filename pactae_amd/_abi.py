@@ -44,7 +44,8 @@ class SmpOpts(ctypes.Structure):
         ("validate", i32),
         ("coherent_hint", i32),
         ("poll_interval", i32),
-        ("reserved", i32 * 7),
+        ("host_spin", i32),
+        ("reserved", i32 * 6),
     ]
 
 

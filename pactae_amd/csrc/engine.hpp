@@ -249,6 +249,24 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return SMP_OK;
   }
 
+  // Host waits.  With opts.host_spin the host polls the stream / event instead
+  // of blocking in the runtime, which takes the wake-up latency out of every
+  // hand-off on the critical path (the waits here are short and few).
+  hipError_t wait_stream(hipStream_t s) {
+    if (!opts.host_spin) return hipStreamSynchronize(s);
+    for (;;) {
+      hipError_t e = hipStreamQuery(s);
+      if (e != hipErrorNotReady) return e;
+    }
+  }
+  hipError_t wait_event(hipEvent_t e) {
+    if (!opts.host_spin) return hipEventSynchronize(e);
+    for (;;) {
+      hipError_t q = hipEventQuery(e);
+      if (q != hipErrorNotReady) return q;
+    }
+  }
+
   // Prefix width and hand-off threshold for an instance of size ni <= n_alloc.
   int prefix_for(int ni) const {
     return opts.prefix_cols <= 0 ? std::min(ni, 128) : std::min(ni, opts.prefix_cols);
@@ -329,7 +347,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (d_out) {
       SMP_HIP_TRY(hipMemcpyAsync(d_out, d_match, (size_t)ni * sizeof(int32_t),
                                  hipMemcpyDeviceToDevice, stream));
-      SMP_HIP_TRY(hipStreamSynchronize(stream));
+      SMP_HIP_TRY(wait_stream(stream));
     }
     return SMP_OK;
   }
@@ -351,7 +369,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   int read_scalars() {
     SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
-    SMP_HIP_TRY(hipStreamSynchronize(stream));
+    SMP_HIP_TRY(wait_stream(stream));
     return SMP_OK;
   }
 
@@ -452,7 +470,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       for (int b = 0; b < blocks_copied; ++b) {
         if (b > 0 && hipEventQuery(ev[4]) == hipSuccess) break;
         auto t0 = Clock::now();
-        SMP_HIP_TRY(hipEventSynchronize(ev_blk[b]));
+        SMP_HIP_TRY(wait_event(ev_blk[b]));
         times.copy_wait_ms += ms_since(t0);
         blocks_ready = b + 1;
         t0 = Clock::now();
@@ -491,7 +509,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     const int parked = h_sc->parked_count;
     SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)parked * sizeof(int2),
                                hipMemcpyDeviceToHost, stream));
-    SMP_HIP_TRY(hipStreamSynchronize(stream));
+    SMP_HIP_TRY(wait_stream(stream));
     times.handoff_ms = ms_since(t0);
 
     t0 = Clock::now();
@@ -522,7 +540,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
           break;
         }
         if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
-          while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(hipEventSynchronize(ev_blk[blocks_ready++]));
+          while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(wait_event(ev_blk[blocks_ready++]));
         }
         const uint32_t e = pr[pactae::pr_index(m, k, n)];
         ++props;
@@ -576,7 +594,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(pactae::launch_fill_i32(&d_sc->active, 1, std::min(bailed, grid * 4), stream));
     SMP_HIP_TRY(pactae::launch_propose(propose_args(0, d_starts, bailed), grid, stream));
     SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
-    SMP_HIP_TRY(hipStreamSynchronize(stream));
+    SMP_HIP_TRY(wait_stream(stream));
     times.resume_ms = ms_since(t0);
     return SMP_OK;
   }
@@ -587,7 +605,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       // The host holds the matching; mirror it to the device result buffer.
       SMP_HIP_TRY(hipMemcpyAsync(d_match, h_match, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
                                  stream));
-      SMP_HIP_TRY(hipStreamSynchronize(stream));
+      SMP_HIP_TRY(wait_stream(stream));
       times.proposals = host_proposals_total;
       times.atomics = (int64_t)h_sc->atomics;
       times.steps = (int64_t)h_sc->steps;
@@ -601,7 +619,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipMemcpyAsync(h_match, d_match, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost,
                                stream));
     SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
-    SMP_HIP_TRY(hipStreamSynchronize(stream));
+    SMP_HIP_TRY(wait_stream(stream));
     if (int st = check_device_error("post")) return st;
     if (perfect) {
       times.proposals = n;  // every man proposes once
@@ -624,13 +642,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream,
                                            /*nontemporal=*/1));
     SMP_HIP_TRY(hipEventRecord(ev[1], stream));
-    SMP_HIP_TRY(hipStreamSynchronize(stream));
+    SMP_HIP_TRY(wait_stream(stream));
     t->rank_ms = ms_since(t0);
     if (init_pr) {
       t0 = Clock::now();
       SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
-      SMP_HIP_TRY(hipStreamSynchronize(stream));
+      SMP_HIP_TRY(wait_stream(stream));
       t->pr_ms = ms_since(t0);
     }
     if (int st = read_scalars()) return st;
