@@ -46,6 +46,19 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def measured_traffic(n):
+    """HBM bytes per k_invert_rows launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration), if
+    they were taken at this n; else None."""
+    path = os.path.join(ROOT, "profiles", f"counters_k_invert_rows_n{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    return int(c["fetch_size_kib_per_dispatch"] * 1024 * c["fetch_factor"]
+               + c["write_size_kib_per_dispatch"] * 1024 * c["write_factor"])
+
+
 def single_instance(smp, args):
     n = args.n
     t0 = time.perf_counter()
@@ -62,6 +75,19 @@ def single_instance(smp, args):
     eng = smp.SmpEngineBamboo.from_device(dm, dw, n, **opts)
     for _ in range(args.warmup):
         eng.exec()
+    # Untuned defaults, for reference.
+    untuned = []
+    for _ in range(max(3, args.steps // 4)):
+        t = time.perf_counter()
+        eng.exec()
+        untuned.append((time.perf_counter() - t) * 1e3)
+    tune = None
+    if not args.no_tune and not opts:
+        t0 = time.perf_counter()
+        tune = eng.tune(3)
+        log(f"tuned hand-off threshold / prefix on this instance in {time.perf_counter() - t0:.2f}s: {tune}")
+        for _ in range(args.warmup):
+            eng.exec()
     log(f"warmup done ({args.warmup} solves)")
     walls, profs = [], []
     t_all = time.perf_counter()
@@ -74,7 +100,7 @@ def single_instance(smp, args):
     match = eng.result()
     log(f"timed {args.steps} solves: mean {statistics.mean(walls):.3f} ms")
     eng.close()
-    return pm, pw, match, walls, profs, elapsed
+    return pm, pw, match, walls, profs, elapsed, untuned, tune
 
 
 def cpu_baseline_single(args, n, pm, pw, gpu_match):
@@ -228,6 +254,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-rate", type=float, default=1500.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the measured choice of hand-off threshold and prefix width")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run the 16-byte streaming copy kernel over this many MiB "
@@ -252,7 +280,7 @@ def main() -> int:
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
         return 0
 
-    pm, pw, match, walls, profs, elapsed = single_instance(smp, args)
+    pm, pw, match, walls, profs, elapsed, untuned, tune = single_instance(smp, args)
     n = args.n
     ms_per_solve = elapsed * 1e3 / args.steps
     rank_ms = statistics.mean(p["rank_kernel_ms"] for p in profs)
@@ -281,14 +309,18 @@ def main() -> int:
         "solve_ms": {"mean": round(statistics.mean(walls), 4), "median": round(statistics.median(walls), 4),
                      "min": round(min(walls), 4), "max": round(max(walls), 4)},
         "phases_ms": phases,
+        "untuned_defaults_ms": {"median": round(statistics.median(untuned), 4), "runs": len(untuned)},
+        "tune": tune,
         "proposals": last["proposals"],
         "atomics": last["atomics"],
+        "segment_steps": last["steps"],
         "handoff_free": last["handoff_free"],
         "host_proposals": last["host_proposals"],
+        "bailed_to_gpu": last["bailed"],
         "roofline": {"kernel": "k_invert_rows (women's rank matrix)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "algorithmic_bytes": rank_bytes},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(n),
+                     "algorithmic_bytes": rank_bytes, "kernel_ms": round(rank_ms, 4)},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

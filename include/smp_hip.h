@@ -110,9 +110,14 @@ typedef struct smp_opts {
                              costs failed atomics, never correctness)             */
   int32_t poll_interval;  /* proposal-kernel steps between reads of the shared
                              active-wave counter that triggers the hand-off; 0 = 16 */
-  int32_t host_spin;      /* 1 = the host polls for device completion (default),
-                             0 = it blocks in the runtime                         */
-  int32_t reserved[6];
+  int32_t host_spin;      /* 1 = the host polls for device completion, 0 = it blocks
+                             in the runtime (default; polling was measured slower) */
+  int32_t prefix_copy;    /* how the PR prefix reaches the host for the tail:
+                             0 = copied beside the proposal kernel on copy streams
+                             (default, measured fastest), 1 = copied after it on
+                             the engine's stream, 2 = block 0 after it and later
+                             blocks on a copy stream                              */
+  int32_t reserved[5];
 } smp_opts;
 
 typedef struct smp_times {

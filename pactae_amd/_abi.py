@@ -45,7 +45,8 @@ class SmpOpts(ctypes.Structure):
         ("coherent_hint", i32),
         ("poll_interval", i32),
         ("host_spin", i32),
-        ("reserved", i32 * 6),
+        ("prefix_copy", i32),
+        ("reserved", i32 * 5),
     ]
 
 

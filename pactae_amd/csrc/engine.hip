@@ -30,7 +30,8 @@ void smp_default_opts(smp_opts* o) {
   o->validate = 1;
   o->coherent_hint = 0;
   o->poll_interval = 16;
-  o->host_spin = 1;
+  o->host_spin = 0;
+  o->prefix_copy = 0;
 }
 
 static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {

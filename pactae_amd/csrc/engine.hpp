@@ -35,6 +35,12 @@ inline double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
+// Pinned host buffers the host tail reads are placed by the calling thread's
+// NUMA policy (its local node) rather than next to the GPU: the host tail's
+// random reads then stay local, while the copies into them still run at full
+// PCIe rate.
+constexpr unsigned kHostAllocFlags = hipHostMallocNumaUser;
+
 struct DeviceInfo {
   int cus = 256;
   int max_lds = 65536;  // bytes a workgroup may allocate
@@ -129,6 +135,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   unsigned long long* d_wstats = nullptr;  // per proposal wave: atomics, steps
 
   uint32_t* h_pr = nullptr;  // pinned copy of the PR prefix for the host tail
+  uint32_t* h_pr_dev = nullptr;  // the same buffer as seen from the device
   unsigned long long* h_partner = nullptr;
   Scalars* h_sc = nullptr;
   int2* h_parked = nullptr;
@@ -141,6 +148,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   static constexpr int kMaxBlocks = 8;  // PR prefix blocks copied to the host (K <= 256)
   hipEvent_t ev_blk[kMaxBlocks] = {};
   int blocks_copied = 0, blocks_ready = 0;
+  int requeues = 0;  // copy-stream replacements made by check_copy_overlap
+  bool state_copied = false;  // control block, partner words and parked list are on the host
   hipEvent_t ev[8] = {};
 
   smp_times times{};
@@ -211,7 +220,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream2, hipStreamNonBlocking));
     for (auto& e : ev) SMP_HIP_TRY(hipEventCreate(&e));
-    for (auto& e : ev_blk) SMP_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : ev_blk) SMP_HIP_TRY(hipEventCreate(&e));
     if (hipMalloc(&d_rank_w, nn * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&d_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_partner, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
@@ -225,13 +234,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error("device allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
     }
-    if (hipHostMalloc(&h_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t), hipHostMallocDefault) !=
+    if (hipHostMalloc(&h_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t), pactae::kHostAllocFlags) !=
             hipSuccess ||
-        hipHostMalloc(&h_partner, (size_t)n * sizeof(unsigned long long), hipHostMallocDefault) !=
+        hipHostMalloc(&h_partner, (size_t)n * sizeof(unsigned long long), pactae::kHostAllocFlags) !=
             hipSuccess ||
-        hipHostMalloc(&h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&h_parked, (size_t)n * sizeof(int2), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&h_rowbuf, kRowFetch * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&h_sc, sizeof(Scalars), pactae::kHostAllocFlags) != hipSuccess ||
+        hipHostMalloc(&h_parked, (size_t)n * sizeof(int2), pactae::kHostAllocFlags) != hipSuccess ||
+        hipHostMalloc(&h_rowbuf, kRowFetch * sizeof(uint32_t), pactae::kHostAllocFlags) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("pinned host allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
@@ -240,7 +249,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
-    if (hipHostMalloc(&h_match, (size_t)n * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h_pr_dev), h_pr, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      pactae::set_error("pinned PR prefix buffer is not visible to the device");
+      return SMP_ERR_HIP;
+    }
+    if (hipHostMalloc(&h_match, (size_t)n * sizeof(int32_t), pactae::kHostAllocFlags) != hipSuccess ||
         hipMalloc(&d_starts, (size_t)n * sizeof(int2)) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("allocation of the result buffers failed");
@@ -274,6 +288,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   void configure(int ni) {
     n = ni;
     K = prefix_for(ni);
+    // Defaults from the measured sweeps at n = 2,000 and 20,000 (uniform): the
+    // host takes over at 32 free men; smp_tune replaces this with a measurement.
     if (opts.tail_mode == SMP_TAIL_GPU)
       T = 0;
     else
@@ -414,7 +430,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipEventRecord(ev[2], stream));
     blocks_copied = 0;
     blocks_ready = 0;
-    if (T > 0) {
+    if (T > 0 && opts.prefix_copy == 0) {
       // PR prefix to the host, overlapped with the proposal kernel, one
       // 32-position block per copy (block 0, where nearly all late proposals fall,
       // first; the others alternate between two streams), each with an event so
@@ -424,7 +440,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       SMP_HIP_TRY(hipStreamWaitEvent(copy_stream, ev[2], 0));
       SMP_HIP_TRY(hipStreamWaitEvent(copy_stream2, ev[2], 0));
       for (int b = 0; b < nb && b < kMaxBlocks; ++b) {
-        hipStream_t cs = (b & 1) ? copy_stream2 : copy_stream;
+        hipStream_t cs = b == 0 ? copy_stream : copy_stream2;
         SMP_HIP_TRY(hipMemcpyAsync(h_pr + b * blk, d_pr + b * blk, blk * sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, cs));
         SMP_HIP_TRY(hipEventRecord(ev_blk[b], cs));
@@ -463,23 +479,63 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(pactae::launch_propose(propose_args(T, nullptr, 0), grid, stream));
     SMP_HIP_TRY(hipEventRecord(ev[4], stream));
     SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
-    if (T > 0) {
+    state_copied = false;
+    if (T > 0 && opts.prefix_copy >= 1) {
+      // Everything the host tail needs follows the kernel on the engine's own
+      // stream: first the small state (control block, partner words, parked
+      // chains), then the PR prefix block by block, each with an event.
+      SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
+      SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, stream));
+      SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost,
+                                 stream));
+      SMP_HIP_TRY(hipEventRecord(ev[5], stream));
+      const size_t blk = (size_t)n * pactae::kPrBlock;
+      const int nb = (K + pactae::kPrBlock - 1) / pactae::kPrBlock;
+      for (int b = 0; b < nb && b < kMaxBlocks; ++b) {
+        // prefix_copy 2: blocks past the first go on the copy stream, so that any
+        // the tail never reads cannot delay the engine's stream.
+        hipStream_t cs = (b > 0 && opts.prefix_copy == 2) ? copy_stream : stream;
+        if (b == 1 && cs == copy_stream) SMP_HIP_TRY(hipStreamWaitEvent(copy_stream, ev_blk[0], 0));
+        SMP_HIP_TRY(hipMemcpyAsync(h_pr + b * blk, d_pr + b * blk, blk * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, cs));
+        SMP_HIP_TRY(hipEventRecord(ev_blk[b], cs));
+        blocks_copied = b + 1;
+      }
+      auto t0 = Clock::now();
+      SMP_HIP_TRY(wait_event(ev[5]));
+      times.copy_wait_ms = ms_since(t0);
+      state_copied = true;
+      if (int st = check_device_error("proposal kernel")) return st;
+      if (h_sc->parked_count > 0) {
+        t0 = Clock::now();
+        SMP_HIP_TRY(wait_event(ev_blk[0]));
+        times.copy_wait_ms += ms_since(t0);
+        blocks_ready = 1;
+        t0 = Clock::now();
+        touch_block(0, false);
+        times.warm_ms = ms_since(t0);
+      }
+    } else if (T > 0) {
       // While the kernel runs: pull the PR prefix blocks into the host caches as
       // they land (a DMA-written buffer starts out in DRAM only).  Block 0 always;
       // later blocks only while the kernel is still running.
-      for (int b = 0; b < blocks_copied; ++b) {
+      bool kernel_done = false;
+      for (int b = 0; b < blocks_copied && !kernel_done; ++b) {
         if (b > 0 && hipEventQuery(ev[4]) == hipSuccess) break;
         auto t0 = Clock::now();
         SMP_HIP_TRY(wait_event(ev_blk[b]));
         times.copy_wait_ms += ms_since(t0);
         blocks_ready = b + 1;
         t0 = Clock::now();
-        touch_block(b);
+        kernel_done = !touch_block(b, /*stop_when_done=*/b > 0);
         times.warm_ms += ms_since(t0);
       }
     }
-    if (int st = read_scalars()) return st;
-    if (int st = check_device_error("proposal kernel")) return st;
+    if (!state_copied) {
+      if (int st = read_scalars()) return st;
+      if (int st = check_device_error("proposal kernel")) return st;
+    }
     const int handoff = h_sc->parked_count;
     times.handoff_free = handoff;
     if (handoff > 0) {
@@ -488,12 +544,24 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return SMP_OK;
   }
 
-  void touch_block(int b) {
+  // Reads one line in every 64 bytes of block b.  With stop_when_done it checks
+  // every 256 KB whether the proposal kernel has finished and then stops (the
+  // host tail must not wait for a warm-up); returns false if it stopped early.
+  bool touch_block(int b, bool stop_when_done) {
     const size_t blk = (size_t)n * pactae::kPrBlock;
     const uint64_t* p = reinterpret_cast<const uint64_t*>(h_pr + b * blk);
+    const size_t words = blk / 2, chunk = 32768;
     uint64_t acc = 0;
-    for (size_t i = 0; i < blk / 2; i += 8) acc += p[i];  // one load per 64-byte line
+    for (size_t c = 0; c < words; c += chunk) {
+      if (stop_when_done && hipEventQuery(ev[4]) == hipSuccess) {
+        touch_sink = acc;
+        return false;
+      }
+      const size_t e = std::min(words, c + chunk);
+      for (size_t i = c; i < e; i += 8) acc += p[i];
+    }
     touch_sink = acc;
+    return true;
   }
 
   // Host tail: McVitie-Wilson chains (smp_engine_mw_1_f9.cu:15-79, and the
@@ -504,12 +572,14 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // from the shared partner words (the host never stalls on single-row fetches).
   int host_tail() {
     auto t0 = Clock::now();
-    SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost, stream));
     const int parked = h_sc->parked_count;
-    SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)parked * sizeof(int2),
-                               hipMemcpyDeviceToHost, stream));
-    SMP_HIP_TRY(wait_stream(stream));
+    if (!state_copied) {
+      SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, stream));
+      SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)parked * sizeof(int2),
+                                 hipMemcpyDeviceToHost, stream));
+      SMP_HIP_TRY(wait_stream(stream));
+    }
     times.handoff_ms = ms_since(t0);
 
     t0 = Clock::now();
@@ -602,10 +672,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // -- post (smp_engine_bamboo.cu:289-306) ----------------------------------
   int PostProc(bool perfect) override {
     if (host_finished) {
-      // The host holds the matching; mirror it to the device result buffer.
+      // The host holds the matching; mirror it to the device result buffer (on
+      // the copy stream: it needs nothing still queued on the engine's stream).
       SMP_HIP_TRY(hipMemcpyAsync(d_match, h_match, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
-                                 stream));
-      SMP_HIP_TRY(wait_stream(stream));
+                                 copy_stream));
+      SMP_HIP_TRY(wait_stream(copy_stream));
       times.proposals = host_proposals_total;
       times.atomics = (int64_t)h_sc->atomics;
       times.steps = (int64_t)h_sc->steps;
@@ -629,6 +700,34 @@ struct smp_engine : public pactae::AbsSmpEngine {
       times.steps = (int64_t)h_sc->steps;
     }
     return SMP_OK;
+  }
+
+  // The prefix copies must run beside the proposal kernel.  Streams share the
+  // device's few hardware queues; when a copy stream lands on the compute
+  // stream's queue, its copies wait for the kernel instead.  Seen from the events
+  // (block 0 landing only after a kernel that ran well past one block's copy
+  // time), the copy streams are replaced, which moves them to other queues.
+  void check_copy_overlap() {
+    float copy_ms = 0, kern_ms = 0;
+    if (hipEventQuery(ev_blk[0]) != hipSuccess) return;
+    if (hipEventElapsedTime(&copy_ms, ev[2], ev_blk[0]) != hipSuccess) return;
+    if (hipEventElapsedTime(&kern_ms, ev[2], ev[4]) != hipSuccess) return;
+    if (kern_ms > 0.12f && copy_ms >= kern_ms && requeues < 8) {
+      (void)hipStreamSynchronize(copy_stream);
+      (void)hipStreamSynchronize(copy_stream2);
+      hipStream_t a = nullptr, b = nullptr;
+      if (hipStreamCreateWithFlags(&a, hipStreamNonBlocking) == hipSuccess &&
+          hipStreamCreateWithFlags(&b, hipStreamNonBlocking) == hipSuccess) {
+        (void)hipStreamDestroy(copy_stream);
+        (void)hipStreamDestroy(copy_stream2);
+        copy_stream = a;
+        copy_stream2 = b;
+        ++requeues;
+      } else {
+        if (a) (void)hipStreamDestroy(a);
+        (void)hipGetLastError();
+      }
+    }
   }
 
   // Init-only run: rank matrix, optionally the PR matrix, then the D2H copy of
@@ -681,6 +780,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.total_ms = precheck_time_ + init_time_ + core_time_ + post_time_;
     times.prefix_cols = K;
     times.tail_threshold = T;
+    if (!times.is_perfect && T > 0 && blocks_copied > 0 && opts.prefix_copy == 0) check_copy_overlap();
     if (!times.is_perfect) {
       float ms = 0;
       if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) times.rank_kernel_ms = ms;
