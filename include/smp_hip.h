@@ -203,8 +203,30 @@ typedef struct smp_batch_opts {
   int32_t use_graph;      /* 1 = capture the launch sequence in a hipGraph (default) */
   int32_t block_threads;  /* workgroup size of the batched kernel; 0 = autotuned     */
   int32_t wgs_per_cu;     /* persistent workgroups per CU; 0 = autotuned            */
-  int32_t reserved[12];
+  int32_t mode;           /* SMP_BATCH_AUTO (autotuned), _FUSED or _POOL            */
+  int32_t group_lanes;    /* pool mode: lanes per proposal chain (8, 16 or 64);
+                             0 = autotuned                                           */
+  int32_t pool_slots;     /* pool mode: instances open per workgroup (1, 2 or 4);
+                             0 = autotuned                                           */
+  int32_t rank_layout;    /* pool mode: SMP_RANK_WOMEN_ROWS or SMP_RANK_MEN_ROWS;
+                             0 = autotuned                                           */
+  int32_t reserved[8];
 } smp_batch_opts;
+
+enum smp_batch_mode {
+  SMP_BATCH_AUTO = 0,
+  SMP_BATCH_FUSED = 1, /* one kernel: each workgroup inverts its instance's rows, then
+                          runs its chains                                          */
+  SMP_BATCH_POOL = 2   /* two kernels in one graph: the rank matrices of all
+                          instances as one HBM-bound pass, then workgroups that each
+                          keep pool_slots instances open in LDS and run several
+                          chains per wave                                          */
+};
+
+enum smp_rank_layout {
+  SMP_RANK_WOMEN_ROWS = 1, /* rank[w][m]: row w is woman w's inverted list          */
+  SMP_RANK_MEN_ROWS = 2    /* rank[m][w]: a chain's lookups stay in its man's row   */
+};
 
 typedef struct smp_batch_times {
   double exec_ms;         /* wall time of the last smp_batch_exec / _looped         */
@@ -212,8 +234,14 @@ typedef struct smp_batch_times {
   double solves_per_s;
   int64_t proposals;
   int32_t count;
-  int32_t block_threads, wgs_per_cu;
-  int32_t reserved[9];
+  int32_t block_threads, wgs_per_cu;  /* of the chain (or fused) kernel           */
+  int32_t mode;                       /* SMP_BATCH_FUSED or SMP_BATCH_POOL          */
+  int32_t group_lanes;                /* pool mode: lanes per proposal chain        */
+  int32_t pool_slots;                 /* pool mode: instances open per workgroup    */
+  int32_t rank_layout;                /* pool mode: SMP_RANK_*                      */
+  double rank_ms;                     /* pool mode: the rank pass alone, measured
+                                         while tuning (min of two runs)             */
+  int32_t reserved[4];
 } smp_batch_times;
 
 void smp_batch_default_opts(smp_batch_opts* opts);

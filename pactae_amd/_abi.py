@@ -78,7 +78,7 @@ class SmpTimes(ctypes.Structure):
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("reserved")}
 
 
 class SmpInitTimes(ctypes.Structure):
@@ -94,7 +94,7 @@ class SmpInitTimes(ctypes.Structure):
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("reserved")}
 
 
 class SmpTuneResult(ctypes.Structure):
@@ -108,7 +108,7 @@ class SmpTuneResult(ctypes.Structure):
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("reserved")}
 
 
 class SmpBatchOpts(ctypes.Structure):
@@ -117,7 +117,11 @@ class SmpBatchOpts(ctypes.Structure):
         ("use_graph", i32),
         ("block_threads", i32),
         ("wgs_per_cu", i32),
-        ("reserved", i32 * 12),
+        ("mode", i32),
+        ("group_lanes", i32),
+        ("pool_slots", i32),
+        ("rank_layout", i32),
+        ("reserved", i32 * 8),
     ]
 
 
@@ -130,11 +134,16 @@ class SmpBatchTimes(ctypes.Structure):
         ("count", i32),
         ("block_threads", i32),
         ("wgs_per_cu", i32),
-        ("reserved", i32 * 9),
+        ("mode", i32),
+        ("group_lanes", i32),
+        ("pool_slots", i32),
+        ("rank_layout", i32),
+        ("rank_ms", ctypes.c_double),
+        ("reserved", i32 * 4),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("reserved")}
 
 
 class SmpServerOpts(ctypes.Structure):
@@ -163,7 +172,7 @@ class SmpRequestInfo(ctypes.Structure):
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("reserved")}
 
 
 class SmpServerStats(ctypes.Structure):
@@ -179,9 +188,14 @@ class SmpServerStats(ctypes.Structure):
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("reserved")}
 
 
+SMP_BATCH_AUTO = 0
+SMP_BATCH_FUSED = 1
+SMP_BATCH_POOL = 2
+SMP_RANK_WOMEN_ROWS = 1
+SMP_RANK_MEN_ROWS = 2
 SMP_SERVE_BATCHED = 0
 SMP_SERVE_PLAIN = 1
 

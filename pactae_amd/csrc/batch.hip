@@ -1,20 +1,24 @@
-// batch.hip — many independent instances solved by one persistent kernel.
+// batch.hip — many independent instances solved by one graph launch.
 //
 // The reference solves one instance per SmpEngineExecutor call
-// (benchmark/smp_engine_exec.h:6-32) and has no batched path.  Here one launch
-// (captured once in a hipGraph and replayed) solves all instances:
-//   * workgroups take instances from a global counter (work stealing), largest
-//     first, so the long instances start early and the short ones fill the end;
-//   * a workgroup solves its instance alone: it inverts the women's rows
-//     (rank[w][pref_w[w][r]] = r, the F9RankMatrixInitKernel step,
-//     src/smp/kernel.cu:672-686) into an int16 rank matrix, rows staged in LDS;
-//     then its waves run proposal chains (the La3F9CoreKernel walk,
-//     src/smp/kernel.cu:633-670) with the women's partner words in LDS and
-//     64-bit LDS atomicMin; then it writes the matching (PostProc,
-//     src/smp/smp_engine_bamboo.cu:299-303);
-//   * nothing leaves the workgroup except the rank rows (HBM) and the result.
-// Block size and workgroups per CU are chosen by a measured sweep at creation
-// (autotune) unless the caller fixes them.
+// (benchmark/smp_engine_exec.h:6-32) and has no batched path.  Here one hipGraph
+// (captured once, replayed) solves all instances, in one of two modes:
+//   FUSED  one persistent kernel; workgroups take instances from a global
+//          counter, largest first, and each solves its instance alone: it
+//          inverts the women's rows (rank[w][pref_w[w][r]] = r, the
+//          F9RankMatrixInitKernel step, src/smp/kernel.cu:672-686) into an
+//          int16 rank matrix, rows staged in LDS, then its waves run proposal
+//          chains (the La3F9CoreKernel walk, src/smp/kernel.cu:633-670) with the
+//          women's partner words in LDS and 64-bit LDS atomicMin, then it writes
+//          the matching (PostProc, src/smp/smp_engine_bamboo.cu:299-303);
+//   POOL   two kernels: the rank matrices of all instances as one HBM-bound
+//          pass (women's rows: k_rank_rows16, or men's rows: k_rankT_tiles),
+//          then a persistent kernel whose workgroups each keep `slots`
+//          instances open in LDS and run several chains per wave
+//          (wg_pool_chains), taking instances largest first.
+// The mode and its shape (block size, workgroups per CU, lanes per chain,
+// slots, rank layout) are chosen by a measured sweep at creation unless the
+// caller fixes them.
 
 #include <algorithm>
 #include <numeric>
@@ -58,6 +62,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch_solve(const BatchDesc* __restri
   if (bad) atomicOr(error, bad);
 }
 
+// Pool-mode chains: at most 64 VGPRs (8 waves per SIMD), so two 1024-thread
+// workgroups share a CU.
+template <int BLOCK, int G, int NS, bool TR>
+__global__ __launch_bounds__(BLOCK, 8) void k_batch_pool(const BatchDesc* __restrict__ desc, int count,
+                                                         const int32_t* __restrict__ pm_all,
+                                                         const int16_t* __restrict__ rank_all,
+                                                         int32_t* __restrict__ match_all, int* next_inst,
+                                                         unsigned long long* proposals, int* error, int max_n) {
+  extern __shared__ unsigned long long partner[];
+  __shared__ PoolSlot slots[NS];
+  __shared__ int qdone;
+  const int bad = wg_pool_chains<BLOCK, G, 8, NS, TR>(desc, count, next_inst, pm_all, rank_all, match_all,
+                                                       proposals, max_n, partner, slots, &qdone);
+  if (bad) atomicOr(error, bad);
+}
+
 __global__ void k_batch_reset(int* next_inst, unsigned long long* proposals, int* error) {
   if (threadIdx.x == 0) {
     *next_inst = 0;
@@ -67,10 +87,9 @@ __global__ void k_batch_reset(int* next_inst, unsigned long long* proposals, int
 }
 
 template <int BLOCK>
-static hipError_t launch_batch_t(int grid, size_t lds, hipStream_t s, const BatchDesc* desc, int count,
-                                 const int32_t* pm, const int32_t* pw, int16_t* rank, int32_t* match,
-                                 int* next_inst, unsigned long long* props, int* err, int max_n,
-                                 int rows) {
+static hipError_t launch_fused(int grid, size_t lds, hipStream_t s, const BatchDesc* desc, int count,
+                               const int32_t* pm, const int32_t* pw, int16_t* rank, int32_t* match,
+                               int* next_inst, unsigned long long* props, int* err, int max_n, int rows) {
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_solve<BLOCK>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -79,11 +98,74 @@ static hipError_t launch_batch_t(int grid, size_t lds, hipStream_t s, const Batc
   return hipGetLastError();
 }
 
+struct PoolArgs {
+  int grid, max_n, count;
+  hipStream_t s;
+  const BatchDesc* desc;
+  const int32_t* pm;
+  const int16_t* rank;
+  int32_t* match;
+  int* next_inst;
+  unsigned long long* props;
+  int* err;
+};
+
+template <int BLOCK, int G, int NS, bool TR>
+static hipError_t launch_pool4(const PoolArgs& a) {
+  const size_t lds = (size_t)NS * a.max_n * 8;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_pool<BLOCK, G, NS, TR>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_batch_pool<BLOCK, G, NS, TR>), dim3(a.grid), dim3(BLOCK), lds, a.s, a.desc, a.count,
+                     a.pm, a.rank, a.match, a.next_inst, a.props, a.err, a.max_n);
+  return hipGetLastError();
+}
+
+template <int BLOCK, int G, bool TR>
+static hipError_t launch_pool3(int ns, const PoolArgs& a) {
+  switch (ns) {
+    case 1: return launch_pool4<BLOCK, G, 1, TR>(a);
+    case 2: return launch_pool4<BLOCK, G, 2, TR>(a);
+    case 4: return launch_pool4<BLOCK, G, 4, TR>(a);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int BLOCK, bool TR>
+static hipError_t launch_pool2(int g, int ns, const PoolArgs& a) {
+  switch (g) {
+    case 8: return launch_pool3<BLOCK, 8, TR>(ns, a);
+    case 16: return launch_pool3<BLOCK, 16, TR>(ns, a);
+    case 64: return launch_pool3<BLOCK, 64, TR>(ns, a);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Pool shapes built into the library: block 512 or 1024, 8/16/64 lanes per
+// chain, 1/2/4 slots, either rank layout.
+static bool pool_shape_ok(int blk, int g, int ns) {
+  return (blk == 512 || blk == 1024) && (g == 8 || g == 16 || g == 64) && (ns == 1 || ns == 2 || ns == 4);
+}
+
+static hipError_t launch_pool(int blk, int g, int ns, bool transposed, const PoolArgs& a) {
+  if (blk == 512) return transposed ? launch_pool2<512, true>(g, ns, a) : launch_pool2<512, false>(g, ns, a);
+  if (blk == 1024) return transposed ? launch_pool2<1024, true>(g, ns, a) : launch_pool2<1024, false>(g, ns, a);
+  return hipErrorInvalidValue;
+}
+
 }  // namespace pactae
 
 using pactae::BatchDesc;
 
 struct smp_batch {
+  // One measured configuration of the batched solve.
+  struct Cfg {
+    int mode = 0, block = 0, wpc = 0;
+    int rows = 0;               // fused: rows in flight per workgroup
+    int group = 0, slots = 0;   // pool: lanes per chain, instances open per workgroup
+    int layout = 0;             // pool: SMP_RANK_WOMEN_ROWS or SMP_RANK_MEN_ROWS
+  };
+
   smp_batch_opts opts{};
   pactae::DeviceInfo dev;
   int count = 0;
@@ -97,14 +179,20 @@ struct smp_batch {
   int16_t* d_rank = nullptr;
   int32_t* d_match = nullptr;
   BatchDesc* d_desc = nullptr;
-  int* d_ctl = nullptr;  // [0] next instance, [1] error
+  pactae::RankTask* d_tasks = nullptr;  // pool: one per instance, instance order
+  int2* d_tiles = nullptr;              // pool, men's-rows layout: rankT tiles
+  int ntiles = 0;
+  int* d_row_task = nullptr;            // pool, women's-rows layout: flat row -> instance
+  int* d_ctl = nullptr;                 // [0] next instance, [1] error
   unsigned long long* d_props = nullptr;
 
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
+  hipEvent_t mid_ev = nullptr;  // recorded between the rank pass and the chains when set
+  float rank_ms = 0;            // rank pass of the chosen configuration, measured while tuning
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  int block = 0, wpc = 0, rows = 0;
+  Cfg cfg;
   bool have_result = false;
   smp_batch_times times{};
   std::unique_ptr<smp_engine> looped;  // single-instance engine for the looped baseline
@@ -119,29 +207,44 @@ struct smp_batch {
     (void)hipFree(d_rank);
     (void)hipFree(d_match);
     (void)hipFree(d_desc);
+    (void)hipFree(d_tasks);
+    (void)hipFree(d_tiles);
+    (void)hipFree(d_row_task);
     (void)hipFree(d_ctl);
     (void)hipFree(d_props);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_mid) (void)hipEventDestroy(ev_mid);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  hipError_t enqueue(int blk, int wgs_per_cu, int rows) {
+  hipError_t enqueue(const Cfg& c) {
     hipLaunchKernelGGL(pactae::k_batch_reset, dim3(1), dim3(64), 0, stream, d_ctl, d_props, d_ctl + 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int grid = std::max(1, std::min(count, dev.cus * wgs_per_cu));
-    const size_t lds = pactae::wg_lds_bytes(rows, max_n);
-    switch (blk) {
+    if (c.mode == SMP_BATCH_POOL) {
+      const bool tr = c.layout == SMP_RANK_MEN_ROWS;
+      e = tr ? pactae::launch_rankT(stream, d_tasks, d_tiles, ntiles, max_n, d_ctl + 1, 0, dev.cus)
+             : pactae::launch_rank_rows16(stream, d_tasks, count, d_row_task, (int)total_men, max_n, d_ctl + 1,
+                                          0, dev.cus, dev.max_lds);
+      if (e != hipSuccess) return e;
+      if (mid_ev && (e = hipEventRecord(mid_ev, stream)) != hipSuccess) return e;
+      pactae::PoolArgs a{std::max(1, std::min((count + c.slots - 1) / c.slots, dev.cus * c.wpc)),
+                         max_n, count, stream, d_desc, d_pm, d_rank, d_match, d_ctl, d_props, d_ctl + 1};
+      return pactae::launch_pool(c.block, c.group, c.slots, tr, a);
+    }
+    const int grid = std::max(1, std::min(count, dev.cus * c.wpc));
+    const size_t lds = pactae::wg_lds_bytes(c.rows, max_n);
+    switch (c.block) {
       case 256:
-        return pactae::launch_batch_t<256>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank, d_match,
-                                           d_ctl, d_props, d_ctl + 1, max_n, rows);
+        return pactae::launch_fused<256>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank, d_match, d_ctl,
+                                         d_props, d_ctl + 1, max_n, c.rows);
       case 512:
-        return pactae::launch_batch_t<512>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank, d_match,
-                                           d_ctl, d_props, d_ctl + 1, max_n, rows);
+        return pactae::launch_fused<512>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank, d_match, d_ctl,
+                                         d_props, d_ctl + 1, max_n, c.rows);
       default:
-        return pactae::launch_batch_t<1024>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank,
-                                            d_match, d_ctl, d_props, d_ctl + 1, max_n, rows);
+        return pactae::launch_fused<1024>(grid, lds, stream, d_desc, count, d_pm, d_pw, d_rank, d_match,
+                                          d_ctl, d_props, d_ctl + 1, max_n, c.rows);
     }
   }
 
@@ -155,51 +258,112 @@ struct smp_batch {
     return 0;
   }
 
-  // One timed run of a configuration (events around the enqueued work).
-  int time_config(int blk, int wgs, int rows, float* ms) {
+  // Whether a configuration fits this batch (LDS per workgroup and per CU).
+  bool fits(Cfg& c) const {
+    if (c.block * c.wpc > 2048) return false;
+    if (c.mode == SMP_BATCH_FUSED) {
+      c.rows = rows_for(c.block, c.wpc);
+      return c.rows > 0;
+    }
+    if (!pactae::pool_shape_ok(c.block, c.group, c.slots)) return false;
+    const size_t lds = (size_t)max_n * 8 * c.slots + 64 * (c.slots + 1);
+    if (lds * c.wpc > 160 * 1024 || lds > (size_t)dev.max_lds) return false;
+    if (c.layout == SMP_RANK_MEN_ROWS &&
+        pactae::rankT_tile_lds(pactae::rankT_tile_width(max_n), max_n) > (size_t)dev.max_lds)
+      return false;
+    if (c.layout == SMP_RANK_WOMEN_ROWS && pactae::rank_rows16_lds(64, max_n) > (size_t)dev.max_lds)
+      return false;
+    return true;
+  }
+
+  // One timed run of a configuration (events around the enqueued work; *rms
+  // receives the rank pass alone in the pool mode).
+  int time_config(const Cfg& c, float* ms, float* rms) {
+    const bool two = c.mode == SMP_BATCH_POOL;
+    mid_ev = two ? ev_mid : nullptr;
     SMP_HIP_TRY(hipEventRecord(ev0, stream));
-    SMP_HIP_TRY(enqueue(blk, wgs, rows));
+    hipError_t e = enqueue(c);
+    mid_ev = nullptr;
+    SMP_HIP_TRY(e);
     SMP_HIP_TRY(hipEventRecord(ev1, stream));
     SMP_HIP_TRY(hipStreamSynchronize(stream));
     SMP_HIP_TRY(hipEventElapsedTime(ms, ev0, ev1));
+    *rms = 0;
+    if (two) SMP_HIP_TRY(hipEventElapsedTime(rms, ev0, ev_mid));
     return SMP_OK;
   }
 
-  // Autotuner: measure each (block, workgroups per CU) candidate that fits the
-  // LDS budget on this batch and keep the fastest (best of two runs each).
+  // Autotuner: time each candidate configuration that fits this batch (best of
+  // two runs) and keep the fastest.  The default search covers the shapes that
+  // have won on uniform, congested, correlated and mixed batches; any fixed
+  // option restricts the search to the full grid of shapes matching it.
   int tune() {
-    const int blocks[] = {256, 512, 1024};
-    const int wgss[] = {1, 2, 4, 8};
-    float best = 1e30f;
-    int bb = 0, bw = 0, br = 0;
-    for (int blk : blocks) {
-      if (opts.block_threads > 0 && blk != opts.block_threads) continue;
-      for (int w : wgss) {
-        if (opts.wgs_per_cu > 0 && w != opts.wgs_per_cu) continue;
-        const int rows = rows_for(blk, w);
-        if (blk * w > 2048 || rows == 0) continue;
-        float t1 = 0, t2 = 0;
-        if (int st = time_config(blk, w, rows, &t1)) return st;
-        if (int st = time_config(blk, w, rows, &t2)) return st;
-        const float t = std::min(t1, t2);
-        if (t < best) best = t, bb = blk, bw = w, br = rows;
+    std::vector<Cfg> cands;
+    const bool fixed = opts.mode || opts.block_threads > 0 || opts.wgs_per_cu > 0 || opts.group_lanes > 0 ||
+                       opts.pool_slots > 0 || opts.rank_layout > 0;
+    if (fixed) {
+      for (int blk : {256, 512, 1024})
+        for (int w : {1, 2, 4, 8}) {
+          Cfg f;
+          f.mode = SMP_BATCH_FUSED, f.block = blk, f.wpc = w, f.group = 64;
+          cands.push_back(f);
+          for (int g : {8, 16, 64})
+            for (int ns : {1, 2, 4})
+              for (int lay : {SMP_RANK_MEN_ROWS, SMP_RANK_WOMEN_ROWS}) {
+                Cfg p;
+                p.mode = SMP_BATCH_POOL, p.block = blk, p.wpc = w, p.group = g, p.slots = ns, p.layout = lay;
+                cands.push_back(p);
+              }
+        }
+    } else {
+      const int shapes[][6] = {
+          // mode, block, wgs/CU, group, slots, layout
+          {SMP_BATCH_FUSED, 1024, 2, 64, 0, 0},
+          {SMP_BATCH_FUSED, 512, 4, 64, 0, 0},
+          {SMP_BATCH_POOL, 1024, 2, 64, 1, SMP_RANK_MEN_ROWS},
+          {SMP_BATCH_POOL, 1024, 2, 64, 2, SMP_RANK_MEN_ROWS},
+          {SMP_BATCH_POOL, 1024, 2, 16, 1, SMP_RANK_MEN_ROWS},
+          {SMP_BATCH_POOL, 512, 4, 16, 1, SMP_RANK_MEN_ROWS},
+          {SMP_BATCH_POOL, 1024, 2, 64, 1, SMP_RANK_WOMEN_ROWS},
+          {SMP_BATCH_POOL, 1024, 2, 16, 1, SMP_RANK_WOMEN_ROWS},
+      };
+      for (const auto& sh : shapes) {
+        Cfg c;
+        c.mode = sh[0], c.block = sh[1], c.wpc = sh[2], c.group = sh[3], c.slots = sh[4], c.layout = sh[5];
+        cands.push_back(c);
       }
     }
-    if (!bb) {
-      pactae::set_error("batched kernel: no block configuration fits the LDS budget for n=" +
+    float best = 1e30f;
+    Cfg bc;
+    for (Cfg c : cands) {
+      if (opts.mode && c.mode != opts.mode) continue;
+      if (opts.block_threads > 0 && c.block != opts.block_threads) continue;
+      if (opts.wgs_per_cu > 0 && c.wpc != opts.wgs_per_cu) continue;
+      if (c.mode == SMP_BATCH_POOL) {
+        if (opts.group_lanes > 0 && c.group != opts.group_lanes) continue;
+        if (opts.pool_slots > 0 && c.slots != opts.pool_slots) continue;
+        if (opts.rank_layout > 0 && c.layout != opts.rank_layout) continue;
+      }
+      if (!fits(c)) continue;
+      float t1 = 0, t2 = 0, r1 = 0, r2 = 0;
+      if (int st = time_config(c, &t1, &r1)) return st;
+      if (int st = time_config(c, &t2, &r2)) return st;
+      const float t = std::min(t1, t2);
+      if (t < best) best = t, bc = c, rank_ms = std::min(r1, r2);
+    }
+    if (!bc.block) {
+      pactae::set_error("batched solve: no configuration fits the options and the LDS budget for n=" +
                         std::to_string(max_n));
       return SMP_ERR_INVALID;
     }
-    block = bb;
-    wpc = bw;
-    rows = br;
+    cfg = bc;
     return SMP_OK;
   }
 
   int build_graph() {
     if (!opts.use_graph) return SMP_OK;
     SMP_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue(block, wpc, rows);
+    hipError_t e = enqueue(cfg);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -232,6 +396,10 @@ void smp_batch_default_opts(smp_batch_opts* o) {
   o->use_graph = 1;
   o->block_threads = 0;
   o->wgs_per_cu = 0;
+  o->mode = SMP_BATCH_AUTO;
+  o->group_lanes = 0;
+  o->pool_slots = 0;
+  o->rank_layout = 0;
 }
 
 int smp_batch_create(int32_t count, const int32_t* sizes, const int64_t* offsets,
@@ -296,9 +464,42 @@ int smp_batch_create(int32_t count, const int32_t* sizes, const int64_t* offsets
     desc[i] = BatchDesc{offsets[j], b->moffsets[j], sizes[j], 0};
   }
   SMP_HIP_TRY(hipMemcpy(b->d_desc, desc.data(), desc.size() * sizeof(BatchDesc), hipMemcpyHostToDevice));
+  // Pool mode: the instances as rank tasks, their transposed-rank tiles and the
+  // flat row -> instance map of the women's-rows pass.
+  if (b->opts.mode != SMP_BATCH_FUSED) {
+    std::vector<pactae::RankTask> tasks(count);
+    long long row0 = 0;
+    for (int i = 0; i < count; ++i) {
+      tasks[i] = pactae::RankTask{b->d_pw + offsets[i], b->d_rank + offsets[i], sizes[i], (int)row0};
+      row0 += sizes[i];
+    }
+    if (row0 > INT32_MAX) {
+      pactae::set_error("smp_batch_create: more than 2^31 rows in one batch");
+      return SMP_ERR_INVALID;
+    }
+    std::vector<int> row_task((size_t)row0);
+    for (int i = 0; i < count; ++i)
+      std::fill(row_task.begin() + tasks[i].row0, row_task.begin() + tasks[i].row0 + sizes[i], i);
+    std::vector<int2> tiles;
+    pactae::rankT_tiles(sizes, count, b->max_n, &tiles);
+    b->ntiles = (int)tiles.size();
+    if (hipMalloc(&b->d_tasks, tasks.size() * sizeof(pactae::RankTask)) != hipSuccess ||
+        hipMalloc(&b->d_tiles, tiles.size() * sizeof(int2)) != hipSuccess ||
+        hipMalloc(&b->d_row_task, row_task.size() * sizeof(int)) != hipSuccess) {
+      (void)hipGetLastError();
+      pactae::set_error("smp_batch_create: device allocation failed");
+      return SMP_ERR_NOMEM;
+    }
+    SMP_HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(pactae::RankTask),
+                          hipMemcpyHostToDevice));
+    SMP_HIP_TRY(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
+    SMP_HIP_TRY(hipMemcpy(b->d_row_task, row_task.data(), row_task.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+  }
   SMP_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   SMP_HIP_TRY(hipEventCreate(&b->ev0));
   SMP_HIP_TRY(hipEventCreate(&b->ev1));
+  SMP_HIP_TRY(hipEventCreate(&b->ev_mid));
   if (int st = b->tune()) return st;
   if (int st = b->build_graph()) return st;
   *out = b.release();
@@ -313,7 +514,7 @@ int smp_batch_exec(smp_batch* b) {
   if (b->graph_exec)
     SMP_HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
   else
-    SMP_HIP_TRY(b->enqueue(b->block, b->wpc, b->rows));
+    SMP_HIP_TRY(b->enqueue(b->cfg));
   SMP_HIP_TRY(hipEventRecord(b->ev1, b->stream));
   SMP_HIP_TRY(hipStreamSynchronize(b->stream));
   b->times = smp_batch_times{};
@@ -325,8 +526,13 @@ int smp_batch_exec(smp_batch* b) {
   SMP_HIP_TRY(hipMemcpy(&p, b->d_props, sizeof(p), hipMemcpyDeviceToHost));
   b->times.proposals = (int64_t)p;
   b->times.count = b->count;
-  b->times.block_threads = b->block;
-  b->times.wgs_per_cu = b->wpc;
+  b->times.block_threads = b->cfg.block;
+  b->times.wgs_per_cu = b->cfg.wpc;
+  b->times.mode = b->cfg.mode;
+  b->times.group_lanes = b->cfg.group;
+  b->times.pool_slots = b->cfg.slots;
+  b->times.rank_layout = b->cfg.layout;
+  b->times.rank_ms = b->rank_ms;
   b->times.solves_per_s = b->count / (b->times.exec_ms / 1e3);
   b->have_result = true;
   return SMP_OK;

@@ -8,8 +8,8 @@
 // (stream) is free; with several lanes, a request that arrives while others are on
 // the GPU starts as soon as a lane and memory free up, and its kernels overlap
 // with the ones already running.  One launch is two kernels:
-//   k_serve_rank    all women's rank rows of the launch, one row per workgroup
-//                   over a flat row index (so a large request is spread over the
+//   k_rank_rows16   all women's rank rows of the launch over a flat row index,
+//                   one wave per row (so a large request is spread over the
 //                   whole GPU), staged in LDS, written as int16;
 //   k_serve_chains  one workgroup per request: proposal chains with the partner
 //                   words in LDS, then the matching, written straight into pinned
@@ -37,39 +37,6 @@ struct ServeReq {
   int n;
   int row0;        // first flat row index of this request in the launch
 };
-
-__global__ __launch_bounds__(256) void k_serve_rank(const ServeReq* __restrict__ reqs, int count,
-                                                    int total_rows, int* errs) {
-  extern __shared__ int16_t rbuf[];
-  for (int g = blockIdx.x; g < total_rows; g += gridDim.x) {
-    int lo = 0, hi = count - 1;  // last request with row0 <= g
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (reqs[mid].row0 <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    const ServeReq q = reqs[lo];
-    const int n = q.n, row = g - q.row0;
-    int bad = 0;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) rbuf[c] = -1;
-    __syncthreads();
-    const int32_t* src = q.pw + (size_t)row * n;
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-      const int m = src[r];
-      if ((unsigned)m < (unsigned)n) rbuf[m] = (int16_t)r;
-      else bad |= kErrEntryRange;
-    }
-    __syncthreads();
-    int16_t* dst = q.rank + (size_t)row * n;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-      const int16_t v = rbuf[c];
-      if (v < 0) bad |= kErrNotPerm;
-      dst[c] = v;
-    }
-    __syncthreads();
-    if (bad) atomicOr(&errs[lo], bad);
-  }
-}
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_serve_chains(const ServeReq* __restrict__ reqs, int count,
@@ -357,10 +324,9 @@ struct smp_server {
                                   hipMemcpyHostToDevice, L.stream);
     if (e != hipSuccess) return e;
     if ((e = pactae::launch_fill_i32(L.d_errs, count, 0, L.stream)) != hipSuccess) return e;
-    const int rank_grid = std::min(rows, dev.cus * 8);
-    hipLaunchKernelGGL(pactae::k_serve_rank, dim3(rank_grid), dim3(256), (size_t)bmax * sizeof(int16_t),
-                       L.stream, L.d_reqs, count, rows, L.d_errs);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = pactae::launch_rank_rows16(L.stream, L.d_reqs, count, static_cast<const int*>(nullptr), rows,
+                                        bmax, L.d_errs, 1, dev.cus, dev.max_lds)) != hipSuccess)
+      return e;
     const size_t lds = (size_t)bmax * 8;
     if (opts.block_threads == 256)
       hipLaunchKernelGGL(pactae::k_serve_chains<256>, dim3(count), dim3(256), lds, L.stream, L.d_reqs,

@@ -307,6 +307,12 @@ def smp_engine_executor(workload_type: WorkloadType, workload_size: int, group_s
 class SmpBatch:
     """Many independent instances solved in one persistent launch (smp_batch_*)."""
 
+    AUTO = _abi.SMP_BATCH_AUTO
+    FUSED = _abi.SMP_BATCH_FUSED
+    POOL = _abi.SMP_BATCH_POOL
+    WOMEN_ROWS = _abi.SMP_RANK_WOMEN_ROWS
+    MEN_ROWS = _abi.SMP_RANK_MEN_ROWS
+
     def __init__(self, sizes, pref_m_list, pref_w_list, **opts):
         sizes = np.ascontiguousarray(sizes, dtype=np.int32)
         self.count = int(sizes.size)

@@ -16,11 +16,13 @@ def make_batch(smp, count, n_min, n_max, seed):
     return sizes, kinds, pms, pws
 
 
-def test_batch_matches_oracle_and_looped(gpu, oracle):
+@pytest.mark.parametrize("mode", ["auto", "fused", "pool"])
+def test_batch_matches_oracle_and_looped(gpu, oracle, mode):
     smp = gpu
     sizes, kinds, pms, pws = make_batch(smp, 96, 1, 700, seed=11)
     want = [oracle.gs_seq(int(n), pm, pw)[0] for n, pm, pw in zip(sizes, pms, pws)]
-    with smp.SmpBatch(sizes, pms, pws) as b:
+    md = {"auto": smp.SmpBatch.AUTO, "fused": smp.SmpBatch.FUSED, "pool": smp.SmpBatch.POOL}[mode]
+    with smp.SmpBatch(sizes, pms, pws, mode=md) as b:
         b.exec()
         for i, got in enumerate(b.result()):
             assert np.array_equal(got, want[i]), (i, sizes[i], kinds[i])
@@ -38,11 +40,57 @@ def test_batch_matches_oracle_and_looped(gpu, oracle):
 
 
 @pytest.mark.parametrize("block", [256, 512, 1024])
-def test_batch_fixed_configs(gpu, oracle, block):
+def test_batch_fused_configs(gpu, oracle, block):
     smp = gpu
     sizes, kinds, pms, pws = make_batch(smp, 40, 64, 2048, seed=block)
     want = [oracle.mw_seq(int(n), pm, pw)[0] for n, pm, pw in zip(sizes, pms, pws)]
-    with smp.SmpBatch(sizes, pms, pws, block_threads=block, wgs_per_cu=1, use_graph=0) as b:
+    with smp.SmpBatch(sizes, pms, pws, block_threads=block, wgs_per_cu=1, use_graph=0,
+                      mode=smp.SmpBatch.FUSED) as b:
         b.exec()
+        assert b.profile()["mode"] == smp.SmpBatch.FUSED
         for i, got in enumerate(b.result()):
             assert np.array_equal(got, want[i]), (i, sizes[i], kinds[i])
+
+
+@pytest.mark.parametrize("layout", ["women_rows", "men_rows"])
+@pytest.mark.parametrize("group", [8, 16, 64])
+@pytest.mark.parametrize("slots", [1, 2, 4])
+def test_batch_pool_configs(gpu, oracle, slots, group, layout):
+    """Every pool shape, on a batch with more instances than slots in the grid."""
+    smp = gpu
+    lay = {"women_rows": smp.SmpBatch.WOMEN_ROWS, "men_rows": smp.SmpBatch.MEN_ROWS}[layout]
+    sizes, kinds, pms, pws = make_batch(smp, 300, 1, 400, seed=slots * 31 + group)
+    want = [oracle.gs_seq(int(n), pm, pw)[0] for n, pm, pw in zip(sizes, pms, pws)]
+    with smp.SmpBatch(sizes, pms, pws, mode=smp.SmpBatch.POOL, block_threads=512, wgs_per_cu=1,
+                      group_lanes=group, pool_slots=slots, rank_layout=lay, use_graph=0) as b:
+        for _ in range(2):
+            b.exec()
+            p = b.profile()
+            assert (p["mode"], p["pool_slots"], p["group_lanes"], p["rank_layout"]) == (
+                smp.SmpBatch.POOL, slots, group, lay)
+            for i, got in enumerate(b.result()):
+                assert np.array_equal(got, want[i]), (i, sizes[i], kinds[i])
+
+
+def test_batch_pool_large_instances(gpu, oracle):
+    """Pool mode on instances up to n = 2048 of every kind, both layouts."""
+    smp = gpu
+    sizes, kinds, pms, pws = make_batch(smp, 24, 900, 2048, seed=77)
+    want = [oracle.mw_seq(int(n), pm, pw)[0] for n, pm, pw in zip(sizes, pms, pws)]
+    for lay in (smp.SmpBatch.WOMEN_ROWS, smp.SmpBatch.MEN_ROWS):
+        with smp.SmpBatch(sizes, pms, pws, mode=smp.SmpBatch.POOL, rank_layout=lay) as b:
+            b.exec()
+            for i, got in enumerate(b.result()):
+                assert np.array_equal(got, want[i]), (lay, i, sizes[i], kinds[i])
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_invalid_lists_raise(gpu, mode):
+    smp = gpu
+    sizes, kinds, pms, pws = make_batch(smp, 12, 16, 300, seed=4)
+    bad = [pw.copy() for pw in pws]
+    n = int(sizes[5])
+    bad[5][2 * n] = bad[5][2 * n + 1]  # a woman's row that is not a permutation
+    with smp.SmpBatch(sizes, pms, bad, mode=mode, use_graph=0) as b:
+        with pytest.raises(smp.SmpError):
+            b.exec()
