@@ -187,7 +187,8 @@ def serving(smp, args):
 
     def run(mode):
         results = [None] * len(picks)
-        with smp.SmpServer(mode=mode, lanes=3, max_batch=64, max_n=8192) as srv:
+        extra = {"large_n": args.serve_large_n} if args.serve_large_n > 0 else {}
+        with smp.SmpServer(mode=mode, lanes=3, max_batch=64, max_n=8192, **extra) as srv:
             # warm-up request (kernel load, first allocations)
             n0, d0m, d0w, _, _ = pool[0]
             srv.wait(srv.submit(d0m, d0w, n0), timeout_ms=120000)
@@ -224,6 +225,7 @@ def serving(smp, args):
             "peak_inflight": int(st["peak_inflight"]),
             "peak_lanes_busy": int(st["peak_lanes_busy"]),
             "peak_arena_mb": round(st["peak_arena_bytes"] / 2**20, 1),
+            "large_solved": int(st["large_solved"]),
             "wall_s": round(end - start, 3)}
 
     res_b, batched_stats = run(smp.SmpServer.BATCHED)
@@ -253,6 +255,8 @@ def main() -> int:
     ap.add_argument("--serve-pool", type=int, default=48)
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-rate", type=float, default=1500.0)
+    ap.add_argument("--serve-large-n", type=int, default=0,
+                    help="server large_n (requests this large use the single-instance pipeline); 0 = default")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the measured choice of hand-off threshold and prefix width")

@@ -282,7 +282,11 @@ typedef struct smp_server_opts {
   int32_t max_n;            /* largest request accepted; 0 = 8192 (int16 ranks)      */
   int32_t block_threads;    /* workgroup size of the serving kernel; 0 = auto        */
   int64_t hbm_budget_bytes; /* device arena for request state; 0 = 4 GiB             */
-  int32_t reserved[8];
+  int32_t large_n;          /* batched mode: requests with n >= large_n are solved one
+                               at a time by the multi-workgroup single-instance
+                               pipeline on a worker of their own, beside the lanes;
+                               0 = 4096; above max_n = never                       */
+  int32_t reserved[7];
 } smp_server_opts;
 
 typedef struct smp_request_info {
@@ -292,7 +296,7 @@ typedef struct smp_request_info {
   int64_t proposals;
   int32_t n;
   int32_t batch_size; /* requests in the launch that carried this one            */
-  int32_t lane;
+  int32_t lane;       /* launch lane, or -1 for the large-request worker          */
   int32_t reserved[5];
 } smp_request_info;
 
@@ -301,7 +305,8 @@ typedef struct smp_server_stats {
   int64_t arena_bytes, peak_arena_bytes;
   int32_t peak_inflight;  /* admitted and not completed, maximum seen            */
   int32_t peak_lanes_busy;
-  int32_t reserved[8];
+  int32_t large_solved;   /* requests solved by the large-request worker           */
+  int32_t reserved[7];
 } smp_server_stats;
 
 void smp_server_default_opts(smp_server_opts* opts);

@@ -155,7 +155,8 @@ class SmpServerOpts(ctypes.Structure):
         ("max_n", i32),
         ("block_threads", i32),
         ("hbm_budget_bytes", i64),
-        ("reserved", i32 * 8),
+        ("large_n", i32),
+        ("reserved", i32 * 7),
     ]
 
 
@@ -184,7 +185,8 @@ class SmpServerStats(ctypes.Structure):
         ("peak_arena_bytes", i64),
         ("peak_inflight", i32),
         ("peak_lanes_busy", i32),
-        ("reserved", i32 * 8),
+        ("large_solved", i32),
+        ("reserved", i32 * 7),
     ]
 
     def as_dict(self) -> dict:

@@ -11,11 +11,15 @@
 //   k_rank_rows16   all women's rank rows of the launch over a flat row index,
 //                   one wave per row (so a large request is spread over the
 //                   whole GPU), staged in LDS, written as int16;
-//   k_serve_chains  one workgroup per request: proposal chains with the partner
-//                   words in LDS, then the matching, written straight into pinned
-//                   host memory.
+//   k_serve_chains  one workgroup per request: proposal chains, 16 lanes per
+//                   chain, with the partner words in LDS, then the matching,
+//                   written straight into pinned host memory.
+// Requests with n >= large_n skip the lanes: a worker thread of their own solves
+// them one at a time with the multi-workgroup single-instance pipeline
+// (engine.hpp), whose chains run over the whole GPU instead of one workgroup, so a
+// large congested request does not hold a lane for its whole serial chain.
 // The plain mode is the baseline: one request at a time through the plain
-// single-instance pipeline (engine.hpp).
+// single-instance pipeline.
 
 #include <atomic>
 #include <condition_variable>
@@ -39,7 +43,7 @@ struct ServeReq {
 };
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_serve_chains(const ServeReq* __restrict__ reqs, int count,
+__global__ __launch_bounds__(BLOCK, 8) void k_serve_chains(const ServeReq* __restrict__ reqs, int count,
                                                         unsigned long long* props, int* errs) {
   extern __shared__ unsigned long long partner[];
   __shared__ int cursor;
@@ -49,7 +53,7 @@ __global__ __launch_bounds__(BLOCK) void k_serve_chains(const ServeReq* __restri
   const ServeReq q = reqs[i];
   InstanceRef ref{q.pm, q.pw, q.rank, q.match, q.n};
   unsigned long long p = 0;
-  const int bad = wg_chains<BLOCK>(ref, partner, &cursor, red, &p);
+  const int bad = wg_chains_g<BLOCK, 16, 8, false>(ref, partner, &cursor, red, &p);
   if (bad) atomicOr(&errs[i], bad);
   if (threadIdx.x == 0) props[i] = p;
 }
@@ -147,25 +151,33 @@ struct smp_server {
   char* d_arena = nullptr;
   std::vector<Lane> lanes;
   hipStream_t copy_stream = nullptr;
-  std::unique_ptr<smp_engine> plain;
+  std::unique_ptr<smp_engine> plain;  // plain mode: the whole path; batched: large requests
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Request*> pending, ready;
+  std::deque<Request*> pending, ready, ready_large;
+  std::condition_variable cv_large;
+  std::thread large_worker;
   std::map<uint64_t, std::unique_ptr<Request>> table;
   uint64_t next_ticket = 1;
-  bool stop = false;
+  bool stop = false, stop_large = false;
   bool fatal = false;
   std::string fatal_msg;
   std::thread worker;
   smp_server_stats stats{};
   int inflight = 0;
 
+  bool is_large(const Request& r) const { return opts.mode == SMP_SERVE_BATCHED && r.n >= opts.large_n; }
+
+  // Arena bytes of the int16 rank matrix (lane requests only: the large-request
+  // pipeline has its own) and of the list copies (host-submitted requests).
+  size_t rank_bytes(const Request& r) const {
+    return is_large(r) ? 0 : (((size_t)r.n * r.n * 2 + 255) & ~size_t(255));
+  }
   size_t footprint(const Request& r) const {
-    const size_t nn = (size_t)r.n * r.n;
-    size_t b = ((nn * 2 + 255) & ~size_t(255));            // int16 rank matrix
-    if (!r.on_device) b += 2 * ((nn * 4 + 255) & ~size_t(255));  // copies of both lists
-    return b;
+    size_t b = rank_bytes(r);
+    if (!r.on_device) b += 2 * (((size_t)r.n * r.n * 4 + 255) & ~size_t(255));
+    return std::max<size_t>(b, 256);
   }
 
   int setup() {
@@ -178,7 +190,7 @@ struct smp_server {
     }
     arena.init((size_t)opts.hbm_budget_bytes);
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
-    if (opts.mode == SMP_SERVE_PLAIN) {
+    if (opts.mode == SMP_SERVE_PLAIN || opts.large_n <= opts.max_n) {
       smp_opts o;
       smp_default_opts(&o);
       o.device = opts.device;
@@ -188,7 +200,8 @@ struct smp_server {
       e->opts = o;
       if (int st = e->setup()) return st;
       plain = std::move(e);
-    } else {
+    }
+    if (opts.mode == SMP_SERVE_BATCHED) {
       lanes.resize(opts.lanes);
       const size_t mb = (size_t)opts.max_batch;
       for (auto& L : lanes) {
@@ -207,9 +220,51 @@ struct smp_server {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
       SMP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&pactae::k_serve_chains<512>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      SMP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&pactae::k_serve_chains<1024>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      if (plain) large_worker = std::thread([this] { large_loop(); });
     }
     worker = std::thread([this] { loop(); });
     return SMP_OK;
+  }
+
+  // Solves one request with the single-instance pipeline; `lane` is recorded.
+  void solve_plain(std::unique_lock<std::mutex>& lk, Request* r, int lane) {
+    r->state = kLaunched;
+    r->t_launch = Clock::now();
+    r->batch_size = 1;
+    r->lane = lane;
+    ++stats.launches;
+    lk.unlock();
+    int st = plain->run_view(r->d_pm, r->d_pw, r->n, nullptr);
+    std::vector<int32_t> res;
+    int64_t props = 0;
+    std::string err;
+    if (st == SMP_OK) {
+      res.assign(plain->h_match, plain->h_match + r->n);
+      props = plain->times.proposals;
+    } else {
+      err = pactae::last_error();
+    }
+    lk.lock();
+    r->result = std::move(res);
+    r->proposals = props;
+    finish(r, st, err);
+  }
+
+  // Batched mode: the large-request worker.
+  void large_loop() {
+    (void)hipSetDevice(opts.device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_large.wait(lk, [this] { return stop_large || !ready_large.empty(); });
+      if (ready_large.empty()) break;
+      Request* r = ready_large.front();
+      ready_large.pop_front();
+      solve_plain(lk, r, -1);
+      ++stats.large_solved;
+      cv_work.notify_one();  // its arena block is free for admission
+    }
   }
 
   ~smp_server() {
@@ -219,6 +274,12 @@ struct smp_server {
     }
     cv_work.notify_all();
     if (worker.joinable()) worker.join();
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop_large = true;
+    }
+    cv_large.notify_all();
+    if (large_worker.joinable()) large_worker.join();
     for (auto& L : lanes) {
       if (L.stream) (void)hipStreamSynchronize(L.stream);
       (void)hipHostFree(L.h_reqs);
@@ -249,6 +310,7 @@ struct smp_server {
     }
     pending.clear();
     ready.clear();
+    ready_large.clear();
     cv_done.notify_all();
   }
 
@@ -277,7 +339,7 @@ struct smp_server {
       *progressed = true;
       if (!r->on_device) {
         const size_t nn = (size_t)r->n * r->n;
-        const size_t rank_b = (nn * 2 + 255) & ~size_t(255);
+        const size_t rank_b = rank_bytes(*r);
         const size_t list_b = (nn * 4 + 255) & ~size_t(255);
         int32_t* dm = reinterpret_cast<int32_t*>(d_arena + r->off + rank_b);
         int32_t* dw = reinterpret_cast<int32_t*>(d_arena + r->off + rank_b + list_b);
@@ -294,7 +356,12 @@ struct smp_server {
         std::vector<int32_t>().swap(r->host_pw);
       }
       r->state = kReady;
-      ready.push_back(r);
+      if (is_large(*r) && large_worker.joinable()) {
+        ready_large.push_back(r);
+        cv_large.notify_one();
+      } else {
+        ready.push_back(r);
+      }
     }
     return hipSuccess;
   }
@@ -331,8 +398,11 @@ struct smp_server {
     if (opts.block_threads == 256)
       hipLaunchKernelGGL(pactae::k_serve_chains<256>, dim3(count), dim3(256), lds, L.stream, L.d_reqs,
                          count, L.d_props, L.d_errs);
-    else
+    else if (opts.block_threads == 512)
       hipLaunchKernelGGL(pactae::k_serve_chains<512>, dim3(count), dim3(512), lds, L.stream, L.d_reqs,
+                         count, L.d_props, L.d_errs);
+    else
+      hipLaunchKernelGGL(pactae::k_serve_chains<1024>, dim3(count), dim3(1024), lds, L.stream, L.d_reqs,
                          count, L.d_props, L.d_errs);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(L.h_props, L.d_props, count * sizeof(unsigned long long),
@@ -375,26 +445,7 @@ struct smp_server {
         if (!ready.empty()) {
           Request* r = ready.front();
           ready.pop_front();
-          r->state = kLaunched;
-          r->t_launch = Clock::now();
-          r->batch_size = 1;
-          r->lane = 0;
-          ++stats.launches;
-          lk.unlock();
-          int st = plain->run_view(r->d_pm, r->d_pw, r->n, nullptr);
-          std::vector<int32_t> res;
-          int64_t props = 0;
-          std::string err;
-          if (st == SMP_OK) {
-            res.assign(plain->h_match, plain->h_match + r->n);
-            props = plain->times.proposals;
-          } else {
-            err = pactae::last_error();
-          }
-          lk.lock();
-          r->result = std::move(res);
-          r->proposals = props;
-          finish(r, st, err);
+          solve_plain(lk, r, 0);
           progressed = true;
         }
       } else if (!fatal) {
@@ -453,6 +504,7 @@ void smp_server_default_opts(smp_server_opts* o) {
   o->max_n = 8192;
   o->block_threads = 0;
   o->hbm_budget_bytes = 4ll << 30;
+  o->large_n = 4096;
 }
 
 int smp_server_create(const smp_server_opts* opts, smp_server** out) {
@@ -468,7 +520,8 @@ int smp_server_create(const smp_server_opts* opts, smp_server** out) {
   if (o.max_batch <= 0) o.max_batch = 64;
   if (o.max_n <= 0) o.max_n = 8192;
   if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = 4ll << 30;
-  if (o.block_threads != 256 && o.block_threads != 512) o.block_threads = 512;
+  if (o.block_threads != 256 && o.block_threads != 512 && o.block_threads != 1024) o.block_threads = 1024;
+  if (o.large_n <= 0) o.large_n = 4096;
   if (o.max_n > 20000 || o.lanes > 16 || o.max_batch > 4096 ||
       (o.mode != SMP_SERVE_BATCHED && o.mode != SMP_SERVE_PLAIN)) {
     pactae::set_error("server: max_n must be <= 20000 (LDS partner words), lanes <= 16, max_batch <= 4096");

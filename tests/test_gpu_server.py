@@ -54,6 +54,23 @@ def test_device_resident_requests_and_admission(gpu, oracle):
     assert st["peak_arena_bytes"] <= budget
 
 
+@pytest.mark.parametrize("large_n", [300, 100000])
+def test_large_request_worker(gpu, oracle, large_n):
+    """Requests at or above large_n go to the single-instance pipeline worker
+    (lane -1); the rest to the lanes; every result matches the oracle."""
+    smp = gpu
+    reqs = requests(smp, 30, seed=13, n_min=64, n_max=900)
+    want = [oracle.gs_seq(n, pm, pw)[0] for n, (pm, pw) in reqs]
+    with smp.SmpServer(lanes=2, max_batch=8, max_n=1024, large_n=large_n) as srv:
+        tickets = [srv.submit(pm, pw, n) for n, (pm, pw) in reqs]
+        out = [srv.wait(t, timeout_ms=60000) for t in tickets]
+        st = srv.stats()
+    for (n, _), (got, info), w in zip(reqs, out, want):
+        assert np.array_equal(got, w), n
+        assert (info["lane"] == -1) == (n >= large_n), (n, info["lane"])
+    assert st["large_solved"] == sum(n >= large_n for n, _ in reqs)
+
+
 def test_plain_mode_matches(gpu, oracle):
     smp = gpu
     reqs = requests(smp, 12, seed=9)
