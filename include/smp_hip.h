@@ -142,7 +142,10 @@ typedef struct smp_times {
   int32_t tail_threshold;  /* threshold used                                      */
   int32_t bailed;          /* chains the host handed back to the GPU because they
                               walked past the PR prefix                           */
-  int32_t reserved[7];
+  int32_t master_list;     /* 1 if every man's list was the same permutation and
+                              the women-proposing walk answered (the stable
+                              matching is then unique)                            */
+  int32_t reserved[6];
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;

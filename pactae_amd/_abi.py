@@ -74,7 +74,8 @@ class SmpTimes(ctypes.Structure):
         ("prefix_cols", i32),
         ("tail_threshold", i32),
         ("bailed", i32),
-        ("reserved", i32 * 7),
+        ("master_list", i32),
+        ("reserved", i32 * 6),
     ]
 
     def as_dict(self) -> dict:

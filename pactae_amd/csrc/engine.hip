@@ -171,6 +171,10 @@ int smp_rank_matrix(smp_engine* e, int32_t* rank_out) {
     pactae::set_error("null argument");
     return SMP_ERR_INVALID;
   }
+  if (!e->matrices_valid) {
+    pactae::set_error("smp_rank_matrix: the last run built no rank matrix (first-choice or master-list path)");
+    return SMP_ERR_STATE;
+  }
   SMP_HIP_TRY(hipStreamSynchronize(e->stream));
   SMP_HIP_TRY(hipMemcpy(rank_out, e->d_rank_w, (size_t)e->n * e->n * sizeof(int32_t),
                         hipMemcpyDeviceToHost));
@@ -181,6 +185,10 @@ int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols) {
   if (!e || !pr_out || !cols) {
     pactae::set_error("null argument");
     return SMP_ERR_INVALID;
+  }
+  if (!e->matrices_valid) {
+    pactae::set_error("smp_pr_matrix: the last run built no PR prefix (first-choice or master-list path)");
+    return SMP_ERR_STATE;
   }
   SMP_HIP_TRY(hipStreamSynchronize(e->stream));
   const int n = e->n, K = e->K;

@@ -128,7 +128,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
   uint32_t* d_pr = nullptr;
   unsigned long long* d_partner = nullptr;
   int32_t* d_match = nullptr;
-  int* d_seen = nullptr;
+  int* d_seen = nullptr;     // precheck counts; the master-list path's Linv afterwards
+  bool master = false;       // this run takes the master-list path
+  bool matrices_valid = false;  // d_rank_w / d_pr hold the last run's matrices
   Scalars* d_sc = nullptr;
   int2* d_parked = nullptr;
   uint32_t* d_rowbuf = nullptr;
@@ -404,6 +406,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int IsPerfect(bool* perfect) override {
     times = smp_times{};
     have_result = false;
+    master = false;
+    matrices_valid = false;
     SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
     if (!opts.precheck) {
       *perfect = false;
@@ -415,12 +419,34 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (int st = check_device_error("precheck")) return st;
     *perfect = h_sc->dup_first == 0;
     times.is_perfect = *perfect ? 1 : 0;
+    // Every man names the same first choice: is every list the same permutation?
+    master = false;
+    if (!*perfect && n > 1 && h_sc->first_differs == 0) {
+      SMP_HIP_TRY(pactae::launch_rows_equal(d_pref_m, n, d_sc, dev.cus, stream));
+      if (int st = read_scalars()) return st;
+      master = h_sc->not_master == 0;
+    }
+    times.master_list = master ? 1 : 0;
     return SMP_OK;
   }
 
   // -- init: rank matrix + PR prefix gather (smp_engine_bamboo.cu:33-106) -----
   int InitProc() override {
+    if (master) {
+      // Master list: the inverse of the shared list (in d_seen, free after the
+      // precheck) and, when validating, a permutation check of the women's rows;
+      // neither the rank matrix nor the PR prefix is needed.
+      blocks_copied = 0;
+      SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[0], stream));
+      SMP_HIP_TRY(pactae::launch_master_inverse(d_pref_m, n, d_seen, d_sc, stream));
+      if (opts.validate) SMP_HIP_TRY(pactae::launch_check_rows(d_pref_w, n, d_sc, dev.cus, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+      SMP_HIP_TRY(hipEventRecord(ev[2], stream));
+      return SMP_OK;
+    }
     const int chunk = invert_chunk();
+    matrices_valid = true;
     SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
     SMP_HIP_TRY(hipEventRecord(ev[0], stream));
     SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream,
@@ -475,6 +501,15 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int CoreProc() override {
     host_finished = false;
     const int grid = propose_grid();
+    if (master) {
+      SMP_HIP_TRY(hipEventRecord(ev[3], stream));
+      SMP_HIP_TRY(pactae::launch_propose_master(d_pref_w, d_seen, d_partner, n, d_sc, grid, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+      if (int st = read_scalars()) return st;
+      if (int st = check_device_error("women-proposing walk")) return st;
+      times.handoff_free = 0;
+      return SMP_OK;
+    }
     SMP_HIP_TRY(hipEventRecord(ev[3], stream));
     SMP_HIP_TRY(pactae::launch_propose(propose_args(T, nullptr, 0), grid, stream));
     SMP_HIP_TRY(hipEventRecord(ev[4], stream));
@@ -684,6 +719,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     if (perfect) {
       SMP_HIP_TRY(pactae::launch_first_match(d_pref_m, n, d_match, stream));
+    } else if (master) {
+      SMP_HIP_TRY(pactae::launch_post_master(d_partner, d_seen, n, d_match, d_sc, stream));
     } else {
       SMP_HIP_TRY(pactae::launch_post(d_partner, n, d_match, d_sc, stream));
     }
@@ -734,6 +771,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // the last matrix built (AbsSmpInitEngine::Init, smp_init_abs.h:16-28).
   int init_only(bool init_pr, smp_init_times* t) {
     *t = smp_init_times{};
+    matrices_valid = true;
     SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
     const int chunk = invert_chunk();
     auto t0 = Clock::now();

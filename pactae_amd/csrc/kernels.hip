@@ -8,6 +8,8 @@
 //   k_gather_pr                       F9PRMatrixInitKernel   src/smp/kernel.cu:688-709
 //   k_propose                         La3F9CoreKernel        src/smp/kernel.cu:633-670
 //   k_post                            PostProc               src/smp/smp_engine_bamboo.cu:299-303
+//   k_rows_equal .. k_post_master     master-list path (every man holds the same list):
+//                                     detection, women-proposing walk, matching
 //   k_gather_row                      rows the host tail needs beyond the gathered prefix
 //                                     (the reference copies the whole PR matrix to the host
 //                                     instead, smp_engine_bamboo.h:53-61)
@@ -42,6 +44,8 @@ __global__ void k_reset_scalars(Scalars* sc, int active) {
     sc->parked_count = 0;
     sc->error = 0;
     sc->dup_first = 0;
+    sc->first_differs = 0;
+    sc->not_master = 0;
     sc->atomics = 0;
     sc->steps = 0;
     sc->proposals = 0;
@@ -60,6 +64,7 @@ __global__ void k_first_choices(const int32_t* __restrict__ pref_m, int n, int* 
     return;
   }
   if (atomicAdd(&seen[w], 1) != 0) atomicOr(&sc->dup_first, 1);
+  if (w != pref_m[0]) atomicOr(&sc->first_differs, 1);
 }
 
 // Perfect case: every man keeps his first choice.
@@ -426,6 +431,178 @@ __global__ __launch_bounds__(256) void k_post(const unsigned long long* __restri
   if (threadIdx.x == 0) atomicAdd(&sc->proposals, red[0] + red[1] + red[2] + red[3]);
 }
 
+// ---------------------------------------------------------------------------
+// Master list (every man's list is the same permutation L).  With one side
+// ranking the other identically the stable matching is unique, so the
+// women-proposing deferred acceptance returns the man-optimal matching the
+// reference's proposal walk computes (smp_engine_bamboo.cu:108-306), in about
+// n*H_n proposals instead of n(n+1)/2.  A man ranks woman w at Linv[w], the same
+// for every man, so a proposal needs no rank gather.
+
+// One wave per man m >= 1 compares his list with man 0's, 64 entries a step, and
+// stops at the first difference (or when another wave has found one).
+__global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ pref_m, int n, Scalars* sc) {
+  const int lane = threadIdx.x & 63;
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  for (int m = 1 + (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); m < n; m += waves) {
+    if (__hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const int32_t* row = pref_m + (size_t)m * n;
+    for (int k0 = 0; k0 < n; k0 += 256) {
+      bool diff = false;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = k0 + 64 * j + lane;
+        if (k < n && row[k] != pref_m[k]) diff = true;
+      }
+      if (__ballot(diff)) {
+        if (lane == 0) atomicOr(&sc->not_master, 1);
+        return;
+      }
+    }
+  }
+}
+
+// linv[L[k]] = k over the shared list L (linv pre-filled with -1).
+__global__ void k_master_inverse(const int32_t* __restrict__ L, int n, int32_t* linv, Scalars* sc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int w = L[k];
+  if ((unsigned)w >= (unsigned)n) {
+    atomicOr(&sc->error, kErrEntryRange);
+  } else if (atomicCAS(&linv[w], -1, k) != -1) {
+    atomicOr(&sc->error, kErrNotPerm);
+  }
+}
+
+// Permutation check of every row (one workgroup per row, an n-bit LDS set).
+__global__ __launch_bounds__(256) void k_check_rows(const int32_t* __restrict__ pref, int n, Scalars* sc) {
+  extern __shared__ uint32_t bits[];
+  const int words = (n + 31) >> 5;
+  int bad = 0;
+  for (int row = blockIdx.x; row < n; row += gridDim.x) {
+    for (int i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0;
+    __syncthreads();
+    const int32_t* src = pref + (size_t)row * n;
+    for (int r0 = threadIdx.x; r0 < n; r0 += 256 * 8) {
+      int v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int r = r0 + 256 * j;
+        v[j] = r < n ? __builtin_nontemporal_load(src + r) : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (r0 + 256 * j < n) {
+          if ((unsigned)v[j] >= (unsigned)n) {
+            bad |= kErrEntryRange;
+          } else {
+            const uint32_t bit = 1u << (v[j] & 31);
+            if (atomicOr(&bits[v[j] >> 5], bit) & bit) bad |= kErrNotPerm;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (bad) atomicOr(&sc->error, bad);
+}
+
+// Women-proposing walk: one wave per chain (woman), 64 list positions a step.
+// Woman w's proposal to the man at position k of her list is the packed word
+// (Linv[w], w, k); the man keeps the smallest (atomicMin on his partner word), so
+// he holds the woman who comes first in L.  A displaced woman continues at her
+// stored position + 1.  Hints read before the atomic can only be stale upward
+// (words only decrease), so they never hide a candidate.
+__global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restrict__ pref_w,
+                                                        const int32_t* __restrict__ linv,
+                                                        unsigned long long* partner_m, int n, Scalars* sc) {
+  const int lane = threadIdx.x & 63;
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  int next_init = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  int w = -1, k = 0;
+  uint32_t rw = 0;
+  unsigned long long n_atomics = 0, n_steps = 0;
+  if (__hip_atomic_load(&sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) next_init = n;
+  for (;;) {
+    if (w < 0) {
+      if (next_init >= n) break;
+      w = next_init;
+      k = 0;
+      next_init += waves;
+    }
+    if (k >= n) {
+      if (lane == 0) atomicOr(&sc->error, kErrListEnd);
+      break;
+    }
+    rw = (uint32_t)linv[w];
+    const int kk = k + lane;
+    bool valid = kk < n;
+    int m = 0;
+    if (valid) {
+      m = pref_w[(size_t)w * n + kk];
+      if ((unsigned)m >= (unsigned)n) {
+        valid = false;
+        atomicOr(&sc->error, kErrEntryRange);
+      }
+    }
+    const unsigned long long mine = pw_pack(rw, (uint32_t)w, (uint32_t)kk);
+    const unsigned long long hint =
+        valid ? __hip_atomic_load(&partner_m[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+    unsigned long long cand = __ballot(valid && mine < hint);
+    ++n_steps;
+    bool moved = false;
+    while (cand) {
+      const int j = __ffsll((long long)cand) - 1;
+      unsigned long long old = 0;
+      if (lane == j) old = atomicMin(&partner_m[m], mine);
+      old = __shfl(old, j);
+      ++n_atomics;
+      if (old > __shfl(mine, j)) {
+        if (old == kEmpty) {
+          w = -1;
+        } else {
+          w = (int)pw_man(old);
+          k = (int)pw_pos(old) + 1;
+        }
+        moved = true;
+        break;
+      }
+      cand &= cand - 1;
+    }
+    if (!moved) k += 64;
+  }
+  if (lane == 0) {
+    atomicAdd(&sc->atomics, n_atomics);
+    atomicAdd(&sc->steps, n_steps);
+  }
+}
+
+// match[m] = the woman in man m's partner word; proposals (the man-proposing
+// count, the same for every schedule) = sum over men of Linv[match[m]] + 1.
+__global__ __launch_bounds__(256) void k_post_master(const unsigned long long* __restrict__ partner_m,
+                                                     const int32_t* __restrict__ linv, int n,
+                                                     int32_t* __restrict__ match, Scalars* sc) {
+  __shared__ unsigned long long red[4];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long props = 0;
+  if (m < n) {
+    const unsigned long long v = partner_m[m];
+    if (v == kEmpty) {
+      atomicOr(&sc->error, kErrUnmatched);
+    } else {
+      const uint32_t w = pw_man(v);
+      if (w < (uint32_t)n) {
+        match[m] = (int32_t)w;
+        props = (unsigned long long)linv[w] + 1ull;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) props += __shfl_down(props, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = props;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&sc->proposals, red[0] + red[1] + red[2] + red[3]);
+}
+
 // Streaming copy, 16 bytes per thread per iteration.
 __global__ void k_copy16(int4* __restrict__ dst, const int4* __restrict__ src, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
@@ -461,6 +638,38 @@ hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s) {
 
 hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, active);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_equal(const int32_t* pref_m, int n, Scalars* sc, int cus, hipStream_t s) {
+  const int grid = std::max(1, std::min((n + 3) / 4, cus * 8));
+  hipLaunchKernelGGL(k_rows_equal, dim3(grid), dim3(256), 0, s, pref_m, n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars* sc, hipStream_t s) {
+  hipError_t e = launch_fill_i32(linv, n, -1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_master_inverse, dim3((n + 255) / 256), dim3(256), 0, s, L, n, linv, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_rows(const int32_t* pref, int n, Scalars* sc, int cus, hipStream_t s) {
+  const size_t lds = (size_t)((n + 31) / 32) * 4;
+  const int grid = std::max(1, std::min(n, cus * 8));
+  hipLaunchKernelGGL(k_check_rows, dim3(grid), dim3(256), lds, s, pref, n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_propose_master(const int32_t* pref_w, const int32_t* linv, unsigned long long* partner_m,
+                                 int n, Scalars* sc, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_propose_master, dim3(grid), dim3(256), 0, s, pref_w, linv, partner_m, n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
+                              Scalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_post_master, dim3((n + 255) / 256), dim3(256), 0, s, partner_m, linv, n, match, sc);
   return hipGetLastError();
 }
 

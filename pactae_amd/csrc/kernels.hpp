@@ -27,6 +27,13 @@ hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long lo
 hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s);
 hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s);
 hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars* sc, hipStream_t s);
+hipError_t launch_rows_equal(const int32_t* pref_m, int n, Scalars* sc, int cus, hipStream_t s);
+hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars* sc, hipStream_t s);
+hipError_t launch_check_rows(const int32_t* pref, int n, Scalars* sc, int cus, hipStream_t s);
+hipError_t launch_propose_master(const int32_t* pref_w, const int32_t* linv, unsigned long long* partner_m,
+                                 int n, Scalars* sc, int grid, hipStream_t s);
+hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
+                              Scalars* sc, hipStream_t s);
 hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s);
 hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
                               Scalars* sc, hipStream_t s, int nontemporal);

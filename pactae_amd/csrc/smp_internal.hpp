@@ -68,7 +68,9 @@ struct Scalars {
   int parked_count;  // chains parked for the host tail
   int error;         // kErr* bits
   int dup_first;     // precheck: 1 if two men share a first choice
-  int pad[3];
+  int first_differs; // precheck: 1 if two men have different first choices
+  int not_master;    // precheck: 1 if some man's list differs from man 0's
+  int pad1;
   unsigned long long atomics;    // atomicMin attempts
   unsigned long long steps;      // segment scans
   unsigned long long proposals;  // sum over women of (partner's list position + 1)

@@ -96,12 +96,61 @@ def test_config3_congested_and_shifted_20000(gpu, oracle):
     pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=0x5EED0003)
     want = oracle.serial_dictatorship(n, pm[:n], pw)
     got, prof = solve(smp, n, pm, pw, tail_mode=1)
+    assert prof["master_list"] == 1
+    assert np.array_equal(got, want)
+    assert prof["proposals"] == n * (n + 1) // 2
+    # the man-proposing walk on the same input (precheck off: no fast paths)
+    got, prof = solve(smp, n, pm, pw, tail_mode=1, precheck=0)
+    assert prof["master_list"] == 0
     assert np.array_equal(got, want)
     assert prof["proposals"] == n * (n + 1) // 2
     pm, pw = smp.generate(smp.Workload.SHIFTED, n, seed=0x5EED0003)
     got, prof = solve(smp, n, pm, pw)
     assert prof["is_perfect"] == 1
     assert np.array_equal(got, pm.reshape(n, n)[:, 0])
+
+
+@pytest.mark.parametrize("n", [2, 3, 64, 65, 1000, 2048])
+def test_master_list_path(gpu, oracle, n):
+    """Every man holds the same list: the women-proposing walk answers, bit-exact,
+    with the man-proposing proposal count n(n+1)/2; a list that differs from the
+    others in its last two entries (or only in a later row) keeps the general path."""
+    smp = gpu
+    for kind in (smp.Workload.CONGESTED, smp.Workload.REF_CONGESTED):
+        pm, pw = smp.generate(kind, n, seed=4242 + n)
+        want, t = oracle.gs_seq(n, pm, pw)
+        assert t["proposals"] == n * (n + 1) // 2
+        got, prof = solve(smp, n, pm, pw)
+        assert prof["master_list"] == 1 and prof["is_perfect"] == 0
+        assert np.array_equal(got, want), kind
+        assert prof["proposals"] == t["proposals"]
+        assert oracle.count_blocking_pairs(n, pm, pw, got) == 0
+        near = pm.copy().reshape(n, n)
+        row = n - 1
+        near[row, -2], near[row, -1] = near[row, -1], near[row, -2]
+        near = near.reshape(-1)
+        want2, t2 = oracle.gs_seq(n, near, pw)
+        got2, prof2 = solve(smp, n, near, pw)
+        assert prof2["master_list"] == 0
+        assert np.array_equal(got2, want2)
+        assert prof2["proposals"] == t2["proposals"]
+
+
+def test_master_list_invalid_and_state(gpu):
+    smp = gpu
+    n = 300
+    pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=9)
+    bad_w = pw.copy()
+    bad_w[11 * n + 5] = bad_w[11 * n + 6]  # woman 11's row is not a permutation
+    with pytest.raises(smp.SmpError) as ei:
+        solve(smp, n, pm, bad_w)
+    assert ei.value.status == 1
+    obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+    with smp.SmpEngineBamboo(obj, n) as e:
+        e.find_stable_matching()
+        assert e.profile()["master_list"] == 1
+        with pytest.raises(smp.SmpError):
+            e.rank_matrix()  # no rank matrix was built on this path
 
 
 def test_invalid_lists_are_rejected(gpu):
