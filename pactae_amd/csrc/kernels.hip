@@ -54,17 +54,27 @@ __global__ void k_reset_scalars(Scalars* sc, int active) {
 
 // ---------------------------------------------------------------------------
 // Precheck: do all men have distinct first choices?  seen[] must be zero.
+// The flags are combined per wave (ballot) before one atomic each: on uniform
+// input thousands of men repeat a first choice, and one same-address atomic per
+// man would serialize the kernel.
 __global__ void k_first_choices(const int32_t* __restrict__ pref_m, int n, int* seen, Scalars* sc) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  int w = pref_m[(size_t)m * n];
-  if ((unsigned)w >= (unsigned)n) {
-    atomicOr(&sc->error, kErrEntryRange);
-    atomicOr(&sc->dup_first, 1);
-    return;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  bool range = false, dup = false, diff = false;
+  if (m < n) {
+    const int w = pref_m[(size_t)m * n];
+    if ((unsigned)w >= (unsigned)n) {
+      range = true;
+    } else {
+      dup = atomicAdd(&seen[w], 1) != 0;
+      diff = w != pref_m[0];
+    }
   }
-  if (atomicAdd(&seen[w], 1) != 0) atomicOr(&sc->dup_first, 1);
-  if (w != pref_m[0]) atomicOr(&sc->first_differs, 1);
+  const unsigned long long br = __ballot(range), bd = __ballot(dup || range), bf = __ballot(diff);
+  if ((threadIdx.x & 63) == 0) {
+    if (br) atomicOr(&sc->error, kErrEntryRange);
+    if (bd) atomicOr(&sc->dup_first, 1);
+    if (bf) atomicOr(&sc->first_differs, 1);
+  }
 }
 
 // Perfect case: every man keeps his first choice.
@@ -455,7 +465,8 @@ __global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ 
         if (k < n && row[k] != pref_m[k]) diff = true;
       }
       if (__ballot(diff)) {
-        if (lane == 0) atomicOr(&sc->not_master, 1);
+        if (lane == 0 && !__hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+          atomicOr(&sc->not_master, 1);
         return;
       }
     }
