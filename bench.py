@@ -288,7 +288,7 @@ def main() -> int:
     n = args.n
     ms_per_solve = elapsed * 1e3 / args.steps
     rank_ms = statistics.mean(p["rank_kernel_ms"] for p in profs)
-    rank_bytes = 8 * n * n
+    rank_bytes = 6 * n * n  # 4 B read from pref_w + 2 B written to the 16-bit rank matrix per entry
     achieved = rank_bytes / (rank_ms * 1e-3) / 1e9
     phases = {k: round(statistics.mean(p[k] for p in profs), 4) for k in
               ("precheck_ms", "init_ms", "core_ms", "post_ms", "total_ms", "rank_kernel_ms",

@@ -176,8 +176,10 @@ int smp_rank_matrix(smp_engine* e, int32_t* rank_out) {
     return SMP_ERR_STATE;
   }
   SMP_HIP_TRY(hipStreamSynchronize(e->stream));
-  SMP_HIP_TRY(hipMemcpy(rank_out, e->d_rank_w, (size_t)e->n * e->n * sizeof(int32_t),
-                        hipMemcpyDeviceToHost));
+  const size_t nn = (size_t)e->n * e->n;
+  std::vector<uint16_t> r16(nn);
+  SMP_HIP_TRY(hipMemcpy(r16.data(), e->d_rank_w, nn * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < nn; ++i) rank_out[i] = (int32_t)r16[i];
   return SMP_OK;
 }
 

@@ -124,7 +124,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   const int32_t* d_pref_m = nullptr;
   const int32_t* d_pref_w = nullptr;
-  int32_t* d_rank_w = nullptr;
+  uint16_t* d_rank_w = nullptr;  // women's ranks, 16-bit (n <= SMP_MAX_N = 65535)
   uint32_t* d_pr = nullptr;
   unsigned long long* d_partner = nullptr;
   int32_t* d_match = nullptr;
@@ -223,7 +223,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream2, hipStreamNonBlocking));
     for (auto& e : ev) SMP_HIP_TRY(hipEventCreate(&e));
     for (auto& e : ev_blk) SMP_HIP_TRY(hipEventCreate(&e));
-    if (hipMalloc(&d_rank_w, nn * sizeof(int32_t)) != hipSuccess ||
+    if (hipMalloc(&d_rank_w, nn * sizeof(uint16_t)) != hipSuccess ||
         hipMalloc(&d_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_partner, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&d_match, (size_t)n * sizeof(int32_t)) != hipSuccess ||
@@ -247,7 +247,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error("pinned host allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
     }
-    if (hipError_t e = pactae::set_invert_lds_limit(((invert_chunk() + 3) & ~3) * 2); e != hipSuccess) {
+    if (hipError_t e = pactae::set_invert_lds_limit(((invert_chunk() + 7) & ~7) * 2); e != hipSuccess) {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
@@ -374,7 +374,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // staging fits in a workgroup's LDS (n up to 81,920 on gfx950), else a multiple
   // of 4 that does.
   int invert_chunk() const {
-    const int cap = (dev.max_lds / 2) & ~3;
+    const int cap = (dev.max_lds / 2) & ~7;
     return n <= cap ? n : cap;
   }
 
@@ -796,8 +796,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
       host.resize(pactae::pr_entries(n, K));
       SMP_HIP_TRY(hipMemcpy(host.data(), d_pr, host.size() * 4, hipMemcpyDeviceToHost));
     } else {
-      host.resize((size_t)n * n);
-      SMP_HIP_TRY(hipMemcpy(host.data(), d_rank_w, host.size() * 4, hipMemcpyDeviceToHost));
+      host.resize(((size_t)n * n + 1) / 2);
+      SMP_HIP_TRY(hipMemcpy(host.data(), d_rank_w, (size_t)n * n * sizeof(uint16_t), hipMemcpyDeviceToHost));
     }
     t->transfer_ms = ms_since(t0);
     t->total_ms = t->rank_ms + t->pr_ms + t->transfer_ms;

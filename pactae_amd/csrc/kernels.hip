@@ -98,7 +98,7 @@ constexpr int kBlk = 8;
 
 template <int VEC, int NT>
 __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __restrict__ pref,
-                                                           int32_t* __restrict__ rank, int n,
+                                                           uint16_t* __restrict__ rank, int n,
                                                            int chunk, int validate, Scalars* sc) {
   extern __shared__ uint16_t lds16[];
   uint32_t* lds32 = reinterpret_cast<uint32_t*>(lds16);
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __rest
   int bad = 0;
   for (int row = blockIdx.x; row < n; row += gridDim.x) {
     const int32_t* src = pref + (size_t)row * n;
-    int32_t* dst = rank + (size_t)row * n;
+    uint16_t* dst = rank + (size_t)row * n;
     for (int c0 = 0; c0 < n; c0 += chunk) {
       const int clen = min(chunk, n - c0);
       if (validate)
@@ -165,29 +165,36 @@ __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __rest
       }
       __syncthreads();
       if constexpr (VEC == 4) {
-        const uint2* l2 = reinterpret_cast<const uint2*>(lds16);
-        int4* dst4 = reinterpret_cast<int4*>(dst + c0);
-        for (int q = tid; q < (clen >> 2); q += kInvBlock) {
-          const uint2 p = l2[q];
-          const int4 o = make_int4((int)(p.x & 0xFFFFu), (int)(p.x >> 16), (int)(p.y & 0xFFFFu),
-                                   (int)(p.y >> 16));
-          if (validate && ((p.x & 0xFFFFu) == 0xFFFFu || (p.x >> 16) == 0xFFFFu ||
-                           (p.y & 0xFFFFu) == 0xFFFFu || (p.y >> 16) == 0xFFFFu))
-            bad |= 2;
-          if constexpr (NT) {
-            __builtin_nontemporal_store(o.x, &dst4[q].x);
-            __builtin_nontemporal_store(o.y, &dst4[q].y);
-            __builtin_nontemporal_store(o.z, &dst4[q].z);
-            __builtin_nontemporal_store(o.w, &dst4[q].w);
-          } else {
-            dst4[q] = o;
+        // The LDS row is already the 16-bit output: 16-byte stores (8 ranks) when
+        // n is a multiple of 8 (rows then start 16-byte aligned), 8-byte stores
+        // otherwise (n is a multiple of 4 here).
+        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+        typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+        auto has_hole = [](unsigned x) { return (x & 0xFFFFu) == 0xFFFFu || (x >> 16) == 0xFFFFu; };
+        if ((n & 7) == 0) {
+          const v4u* l4 = reinterpret_cast<const v4u*>(lds16);
+          v4u* dst8 = reinterpret_cast<v4u*>(dst + c0);
+          for (int q = tid; q < (clen >> 3); q += kInvBlock) {
+            const v4u p = l4[q];
+            if (validate && (has_hole(p.x) || has_hole(p.y) || has_hole(p.z) || has_hole(p.w))) bad |= 2;
+            if constexpr (NT) __builtin_nontemporal_store(p, dst8 + q);
+            else dst8[q] = p;
+          }
+        } else {
+          const v2u* l2 = reinterpret_cast<const v2u*>(lds16);
+          v2u* dst4 = reinterpret_cast<v2u*>(dst + c0);
+          for (int q = tid; q < (clen >> 2); q += kInvBlock) {
+            const v2u p = l2[q];
+            if (validate && (has_hole(p.x) || has_hole(p.y))) bad |= 2;
+            if constexpr (NT) __builtin_nontemporal_store(p, dst4 + q);
+            else dst4[q] = p;
           }
         }
       } else {
         for (int c = tid; c < clen; c += kInvBlock) {
-          const uint32_t v = lds16[c];
+          const uint16_t v = lds16[c];
           if (validate && v == 0xFFFFu) bad |= 2;
-          dst[c0 + c] = (int32_t)v;
+          dst[c0 + c] = v;
         }
       }
       __syncthreads();
@@ -208,7 +215,7 @@ constexpr int kGatherMen = 32;
 constexpr int kGatherCols = 256;
 
 __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ pref_m,
-                                                   const int32_t* __restrict__ rank_w,
+                                                   const uint16_t* __restrict__ rank_w,
                                                    uint32_t* __restrict__ pr, int n, int K,
                                                    int tiles, int per_xcd, Scalars* sc) {
   const int b = blockIdx.x;
@@ -237,7 +244,7 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
 }
 
 // Entries [k0, k0 + count) of man m's PR row, for the host tail.
-__global__ void k_gather_row(const int32_t* __restrict__ pref_m, const int32_t* __restrict__ rank_w,
+__global__ void k_gather_row(const int32_t* __restrict__ pref_m, const uint16_t* __restrict__ rank_w,
                              int n, int m, int k0, int count, uint32_t* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -327,7 +334,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         const int wi = a.pref_m[(size_t)m * n + kk];
         if ((unsigned)wi < (unsigned)n) {
           w = (uint32_t)wi;
-          r = (uint32_t)a.rank_w[(size_t)w * n + m] & 0xFFFFu;
+          r = (uint32_t)a.rank_w[(size_t)w * n + m];
         } else {
           valid = false;
           atomicOr(&a.sc->error, kErrEntryRange);
@@ -696,10 +703,10 @@ hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipS
 
 // chunk: columns of one LDS pass (<= max LDS bytes / 4).
 // chunk: columns of one LDS pass (a multiple of 4, <= max LDS bytes / 2).
-hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
+hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int chunk, int validate,
                               Scalars* sc, hipStream_t s, int nontemporal) {
-  const size_t lds = (size_t)((chunk + 3) & ~3) * sizeof(uint16_t);
-  const bool vec = (n % 4 == 0) && (chunk % 4 == 0);
+  const size_t lds = (size_t)((chunk + 7) & ~7) * sizeof(uint16_t);
+  const bool vec = (n % 4 == 0) && (chunk % 8 == 0 || chunk == n);
   if (vec && nontemporal)
     hipLaunchKernelGGL((k_invert_rows<4, 1>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
                        validate, sc);  // streaming: rank rows are written once, read much later
@@ -723,7 +730,7 @@ hipError_t set_invert_lds_limit(int bytes) {
   return hipSuccess;
 }
 
-hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
+hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
                             Scalars* sc, hipStream_t s) {
   const int men_blocks = (n + kGatherMen - 1) / kGatherMen;
   const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
@@ -735,7 +742,7 @@ hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32
   return hipGetLastError();
 }
 
-hipError_t launch_gather_row(const int32_t* pref_m, const int32_t* rank_w, int n, int m, int k0,
+hipError_t launch_gather_row(const int32_t* pref_m, const uint16_t* rank_w, int n, int m, int k0,
                              int count, uint32_t* out, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_row, dim3((count + 255) / 256), dim3(256), 0, s, pref_m, rank_w, n, m,
                      k0, count, out);

@@ -10,7 +10,7 @@ namespace pactae {
 struct ProposeArgs {
   const uint32_t* pr;
   const int32_t* pref_m;
-  const int32_t* rank_w;
+  const uint16_t* rank_w;     // women's ranks, rank_w[w * n + m]
   unsigned long long* partner;
   int2* parked;
   Scalars* sc;
@@ -35,12 +35,12 @@ hipError_t launch_propose_master(const int32_t* pref_w, const int32_t* linv, uns
 hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
                               Scalars* sc, hipStream_t s);
 hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s);
-hipError_t launch_invert_rows(const int32_t* pref, int32_t* rank, int n, int chunk, int validate,
+hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int chunk, int validate,
                               Scalars* sc, hipStream_t s, int nontemporal);
 hipError_t set_invert_lds_limit(int bytes);
-hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
+hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
                             Scalars* sc, hipStream_t s);
-hipError_t launch_gather_row(const int32_t* pref_m, const int32_t* rank_w, int n, int m, int k0,
+hipError_t launch_gather_row(const int32_t* pref_m, const uint16_t* rank_w, int n, int m, int k0,
                              int count, uint32_t* out, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);

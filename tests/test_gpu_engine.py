@@ -20,6 +20,7 @@ TAIL_CONFIGS = [
     dict(tail_mode=1, prefix_cols=100000),      # full n x n PR matrix (reference layout)
     dict(tail_mode=2, tail_threshold=37, prefix_cols=1),  # every host chain is handed back
     dict(tail_mode=2, tail_threshold=5, poll_interval=1, coherent_hint=1),
+    dict(tail_mode=2, tail_threshold=512, prefix_cols=32),  # a long host tail over a small prefix
 ]
 
 
@@ -83,7 +84,8 @@ def test_config2_uniform_20000(gpu, oracle):
     n = 20000
     pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=0x5EED0002)
     want, t = oracle.gs_seq(n, pm, pw)
-    for cfg in (dict(tail_mode=1), dict(tail_mode=2, tail_threshold=16), dict(tail_mode=0)):
+    for cfg in (dict(tail_mode=1), dict(tail_mode=2, tail_threshold=16), dict(tail_mode=0),
+                dict(tail_mode=2, tail_threshold=256)):
         got, prof = solve(smp, n, pm, pw, **cfg)
         assert np.array_equal(got, want), cfg
         assert prof["proposals"] == t["proposals"]
