@@ -100,7 +100,22 @@ def single_instance(smp, args):
     match = eng.result()
     log(f"timed {args.steps} solves: mean {statistics.mean(walls):.3f} ms")
     eng.close()
-    return pm, pw, match, walls, profs, elapsed, untuned, tune
+    # The reference's own GPU baseline (MW-Par-GPU-CAS) on the same lists.
+    base = smp.SmpEngineBamboo.from_device(dm, dw, n, schedule=smp.Schedule.MW_CAS)
+    base.exec()
+    bwalls = []
+    for _ in range(3):
+        t = time.perf_counter()
+        base.exec()
+        bwalls.append((time.perf_counter() - t) * 1e3)
+    same = bool(np.array_equal(base.result(), match))
+    bprof = base.profile()
+    base.close()
+    gpu_base = {"engine": "MW-Par-GPU-CAS (src/smp/smp_engine_mw_3_f9.cu:37-97), this build's kernels",
+                "ms_per_solve": round(statistics.median(bwalls), 3),
+                "core_kernel_ms": round(bprof["propose_kernel_ms"], 3), "bit_exact": same}
+    log(f"gpu baseline MW-CAS: {gpu_base}")
+    return pm, pw, match, walls, profs, elapsed, untuned, tune, gpu_base
 
 
 def cpu_baseline_single(args, n, pm, pw, gpu_match):
@@ -284,7 +299,7 @@ def main() -> int:
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
         return 0
 
-    pm, pw, match, walls, profs, elapsed, untuned, tune = single_instance(smp, args)
+    pm, pw, match, walls, profs, elapsed, untuned, tune, gpu_base = single_instance(smp, args)
     n = args.n
     ms_per_solve = elapsed * 1e3 / args.steps
     rank_ms = statistics.mean(p["rank_kernel_ms"] for p in profs)
@@ -326,6 +341,8 @@ def main() -> int:
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(n),
                      "algorithmic_bytes": rank_bytes, "kernel_ms": round(rank_ms, 4)},
         "cpu_baseline": None,
+        "gpu_baseline": gpu_base,
+        "speedup_vs_gpu_baseline": round(gpu_base["ms_per_solve"] / ms_per_solve, 2),
     }
     if not args.no_cpu:
         cpu, ok = cpu_baseline_single(args, n, pm, pw, match)

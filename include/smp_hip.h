@@ -117,8 +117,16 @@ typedef struct smp_opts {
                              (default, measured fastest), 1 = copied after it on
                              the engine's stream, 2 = block 0 after it and later
                              blocks on a copy stream                              */
-  int32_t reserved[5];
+  int32_t schedule;       /* enum smp_schedule                                     */
+  int32_t reserved[4];
 } smp_opts;
+
+enum smp_schedule {
+  SMP_SCHEDULE_BAMBOO = 0, /* this build's pipeline (default)                       */
+  SMP_SCHEDULE_MW_CAS = 1  /* the reference's GPU baseline MW-Par-GPU-CAS
+                              (src/smp/smp_engine_mw_3_f9.cu:37-97): one thread per
+                              man, 32-bit CAS on the partner rank; no precheck     */
+};
 
 typedef struct smp_times {
   /* Wall clock of the timed phases.  precheck/init/core are the three phases
@@ -145,7 +153,8 @@ typedef struct smp_times {
   int32_t master_list;     /* 1 if every man's list was the same permutation and
                               the women-proposing walk answered (the stable
                               matching is then unique)                            */
-  int32_t reserved[6];
+  int32_t schedule;        /* enum smp_schedule of this run                       */
+  int32_t reserved[5];
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;

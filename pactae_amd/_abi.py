@@ -46,7 +46,8 @@ class SmpOpts(ctypes.Structure):
         ("poll_interval", i32),
         ("host_spin", i32),
         ("prefix_copy", i32),
-        ("reserved", i32 * 5),
+        ("schedule", i32),
+        ("reserved", i32 * 4),
     ]
 
 
@@ -75,7 +76,8 @@ class SmpTimes(ctypes.Structure):
         ("tail_threshold", i32),
         ("bailed", i32),
         ("master_list", i32),
-        ("reserved", i32 * 6),
+        ("schedule", i32),
+        ("reserved", i32 * 5),
     ]
 
     def as_dict(self) -> dict:
@@ -194,6 +196,8 @@ class SmpServerStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("reserved")}
 
 
+SMP_SCHEDULE_BAMBOO = 0
+SMP_SCHEDULE_MW_CAS = 1
 SMP_BATCH_AUTO = 0
 SMP_BATCH_FUSED = 1
 SMP_BATCH_POOL = 2

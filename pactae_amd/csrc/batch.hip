@@ -32,7 +32,7 @@ struct BatchDesc {
   long long off;   // element offset of the instance's n x n lists
   long long moff;  // offset of its result in the concatenated matching
   int n;
-  int pad;
+  int idx;         // the instance's position in the caller's order
 };
 
 template <int BLOCK>
@@ -69,12 +69,95 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch_pool(const BatchDesc* __rest
                                                          const int32_t* __restrict__ pm_all,
                                                          const int16_t* __restrict__ rank_all,
                                                          int32_t* __restrict__ match_all, int* next_inst,
-                                                         unsigned long long* proposals, int* error, int max_n) {
+                                                         unsigned long long* proposals, int* error, int max_n,
+                                                         const int32_t* __restrict__ pw_all,
+                                                         const int* __restrict__ master_flags,
+                                                         const int32_t* __restrict__ linv_all) {
   extern __shared__ unsigned long long partner[];
   __shared__ PoolSlot slots[NS];
   __shared__ int qdone;
   const int bad = wg_pool_chains<BLOCK, G, 8, NS, TR>(desc, count, next_inst, pm_all, rank_all, match_all,
-                                                       proposals, max_n, partner, slots, &qdone);
+                                                       proposals, max_n, partner, slots, &qdone, pw_all,
+                                                       master_flags, linv_all);
+  if (bad) atomicOr(error, bad);
+}
+
+// Master-list detection (does every man of an instance hold the same list?),
+// in three passes so that no instance is compared by one workgroup alone:
+//   k_master_probe   one wave per instance compares the first 64 entries of
+//                    rows 1..4 with row 0; most instances differ there
+//                    (differs[i] = 1) and are done;
+//   k_master_rows    the remaining candidates' rows, one wave per row over the
+//                    flat row space of the batch, 512 entries a step, each row
+//                    stopping at its first difference or when its instance is
+//                    already known to differ;
+//   k_master_linv    for every instance still equal (n >= 2): linv[moff + L[k]] = k
+//                    with L = row 0 checked to be a permutation.
+// differs[] is indexed by the instance's position in the caller's order.
+__global__ __launch_bounds__(256) void k_master_probe(const RankTask* __restrict__ tasks, int count,
+                                                      const int32_t* __restrict__ pw_all,
+                                                      const int32_t* __restrict__ pm_all, int* differs) {
+  const int lane = threadIdx.x & 63;
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  for (int i = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); i < count; i += waves) {
+    const int n = tasks[i].n;
+    const int32_t* pm = pm_all + (tasks[i].pw - pw_all);
+    bool diff = n < 2;
+    for (int m = 1; m < min(n, 5) && !diff; ++m)
+      diff = __ballot(lane < n && pm[(size_t)m * n + lane] != pm[lane]) != 0;
+    if (lane == 0) differs[i] = diff ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_master_rows(const RankTask* __restrict__ tasks,
+                                                     const int* __restrict__ row_task, int total_rows,
+                                                     const int32_t* __restrict__ pw_all,
+                                                     const int32_t* __restrict__ pm_all, int* differs) {
+  const int lane = threadIdx.x & 63;
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  for (int g = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); g < total_rows; g += waves) {
+    const int i = row_task[g];
+    if (__hip_atomic_load(&differs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+    const int n = tasks[i].n, m = g - tasks[i].row0;
+    if (m == 0) continue;
+    const int32_t* pm = pm_all + (tasks[i].pw - pw_all);
+    const int32_t* row = pm + (size_t)m * n;
+    for (int k0 = 0; k0 < n; k0 += 512) {
+      bool diff = false;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + 64 * j + lane;
+        if (k < n && row[k] != pm[k]) diff = true;
+      }
+      if (__ballot(diff)) {
+        if (lane == 0) atomicOr(&differs[i], 1);
+        break;
+      }
+      if (__hip_atomic_load(&differs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_master_linv(const RankTask* __restrict__ tasks, int count,
+                                                     const long long* __restrict__ moffs,
+                                                     const int32_t* __restrict__ pw_all,
+                                                     const int32_t* __restrict__ pm_all, const int* differs,
+                                                     int32_t* linv_all, int* error) {
+  int bad = 0;
+  for (int i = blockIdx.x; i < count; i += gridDim.x) {
+    if (differs[i]) continue;
+    const int n = tasks[i].n;
+    const int32_t* L = pm_all + (tasks[i].pw - pw_all);
+    int32_t* linv = linv_all + moffs[i];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) linv[k] = -1;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const int w = L[k];
+      if ((unsigned)w >= (unsigned)n) bad |= kErrEntryRange;
+      else if (atomicCAS(&linv[w], -1, k) != -1) bad |= kErrNotPerm;
+    }
+    __syncthreads();
+  }
   if (bad) atomicOr(error, bad);
 }
 
@@ -108,6 +191,9 @@ struct PoolArgs {
   int* next_inst;
   unsigned long long* props;
   int* err;
+  const int32_t* pw;
+  const int* master;  // per-instance master-list flags, or null
+  const int32_t* linv;
 };
 
 template <int BLOCK, int G, int NS, bool TR>
@@ -117,7 +203,7 @@ static hipError_t launch_pool4(const PoolArgs& a) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_batch_pool<BLOCK, G, NS, TR>), dim3(a.grid), dim3(BLOCK), lds, a.s, a.desc, a.count,
-                     a.pm, a.rank, a.match, a.next_inst, a.props, a.err, a.max_n);
+                     a.pm, a.rank, a.match, a.next_inst, a.props, a.err, a.max_n, a.pw, a.master, a.linv);
   return hipGetLastError();
 }
 
@@ -183,6 +269,9 @@ struct smp_batch {
   int2* d_tiles = nullptr;              // pool, men's-rows layout: rankT tiles
   int ntiles = 0;
   int* d_row_task = nullptr;            // pool, women's-rows layout: flat row -> instance
+  int* d_master = nullptr;              // pool: per-instance master-list flags
+  int32_t* d_linv = nullptr;            // pool: Linv of master-list instances, at matching offsets
+  long long* d_moffs = nullptr;         // pool: matching offsets in the caller's order
   int* d_ctl = nullptr;                 // [0] next instance, [1] error
   unsigned long long* d_props = nullptr;
 
@@ -210,6 +299,9 @@ struct smp_batch {
     (void)hipFree(d_tasks);
     (void)hipFree(d_tiles);
     (void)hipFree(d_row_task);
+    (void)hipFree(d_master);
+    (void)hipFree(d_linv);
+    (void)hipFree(d_moffs);
     (void)hipFree(d_ctl);
     (void)hipFree(d_props);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -224,13 +316,22 @@ struct smp_batch {
     if (e != hipSuccess) return e;
     if (c.mode == SMP_BATCH_POOL) {
       const bool tr = c.layout == SMP_RANK_MEN_ROWS;
+      const int cgrid = std::max(1, std::min((count + 3) / 4, dev.cus * 8));
+      hipLaunchKernelGGL(pactae::k_master_probe, dim3(cgrid), dim3(256), 0, stream, d_tasks, count, d_pw, d_pm,
+                         d_master);
+      hipLaunchKernelGGL(pactae::k_master_rows, dim3(dev.cus * 8), dim3(256), 0, stream, d_tasks, d_row_task,
+                         (int)total_men, d_pw, d_pm, d_master);
+      hipLaunchKernelGGL(pactae::k_master_linv, dim3(std::max(1, std::min(count, dev.cus * 8))), dim3(256), 0,
+                         stream, d_tasks, count, d_moffs, d_pw, d_pm, d_master, d_linv, d_ctl + 1);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
       e = tr ? pactae::launch_rankT(stream, d_tasks, d_tiles, ntiles, max_n, d_ctl + 1, 0, dev.cus)
              : pactae::launch_rank_rows16(stream, d_tasks, count, d_row_task, (int)total_men, max_n, d_ctl + 1,
                                           0, dev.cus, dev.max_lds);
       if (e != hipSuccess) return e;
       if (mid_ev && (e = hipEventRecord(mid_ev, stream)) != hipSuccess) return e;
       pactae::PoolArgs a{std::max(1, std::min((count + c.slots - 1) / c.slots, dev.cus * c.wpc)),
-                         max_n, count, stream, d_desc, d_pm, d_rank, d_match, d_ctl, d_props, d_ctl + 1};
+                         max_n, count, stream, d_desc, d_pm, d_rank, d_match, d_ctl, d_props, d_ctl + 1,
+                         d_pw, d_master, d_linv};
       return pactae::launch_pool(c.block, c.group, c.slots, tr, a);
     }
     const int grid = std::max(1, std::min(count, dev.cus * c.wpc));
@@ -461,7 +562,7 @@ int smp_batch_create(int32_t count, const int32_t* sizes, const int64_t* offsets
   std::vector<BatchDesc> desc(count);
   for (int i = 0; i < count; ++i) {
     const int j = order[i];
-    desc[i] = BatchDesc{offsets[j], b->moffsets[j], sizes[j], 0};
+    desc[i] = BatchDesc{offsets[j], b->moffsets[j], sizes[j], j};
   }
   SMP_HIP_TRY(hipMemcpy(b->d_desc, desc.data(), desc.size() * sizeof(BatchDesc), hipMemcpyHostToDevice));
   // Pool mode: the instances as rank tasks, their transposed-rank tiles and the
@@ -485,7 +586,10 @@ int smp_batch_create(int32_t count, const int32_t* sizes, const int64_t* offsets
     b->ntiles = (int)tiles.size();
     if (hipMalloc(&b->d_tasks, tasks.size() * sizeof(pactae::RankTask)) != hipSuccess ||
         hipMalloc(&b->d_tiles, tiles.size() * sizeof(int2)) != hipSuccess ||
-        hipMalloc(&b->d_row_task, row_task.size() * sizeof(int)) != hipSuccess) {
+        hipMalloc(&b->d_row_task, row_task.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->d_master, (size_t)count * sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->d_linv, (size_t)b->total_men * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&b->d_moffs, (size_t)count * sizeof(long long)) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("smp_batch_create: device allocation failed");
       return SMP_ERR_NOMEM;
@@ -494,6 +598,8 @@ int smp_batch_create(int32_t count, const int32_t* sizes, const int64_t* offsets
                           hipMemcpyHostToDevice));
     SMP_HIP_TRY(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
     SMP_HIP_TRY(hipMemcpy(b->d_row_task, row_task.data(), row_task.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+    SMP_HIP_TRY(hipMemcpy(b->d_moffs, b->moffsets.data(), (size_t)count * sizeof(long long),
                           hipMemcpyHostToDevice));
   }
   SMP_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));

@@ -32,6 +32,7 @@ void smp_default_opts(smp_opts* o) {
   o->poll_interval = 16;
   o->host_spin = 0;
   o->prefix_copy = 0;
+  o->schedule = SMP_SCHEDULE_BAMBOO;
 }
 
 static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {

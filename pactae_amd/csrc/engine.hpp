@@ -130,6 +130,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int32_t* d_match = nullptr;
   int* d_seen = nullptr;     // precheck counts; the master-list path's Linv afterwards
   bool master = false;       // this run takes the master-list path
+  int* d_cas_next = nullptr; // MW-CAS schedule: list position after each man's last claim
   bool matrices_valid = false;  // d_rank_w / d_pr hold the last run's matrices
   Scalars* d_sc = nullptr;
   int2* d_parked = nullptr;
@@ -176,6 +177,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipFree(d_partner);
     (void)hipFree(d_match);
     (void)hipFree(d_seen);
+    (void)hipFree(d_cas_next);
     (void)hipFree(d_sc);
     (void)hipFree(d_parked);
     (void)hipFree(d_rowbuf);
@@ -195,6 +197,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     d_partner = nullptr;
     d_match = nullptr;
     d_seen = nullptr;
+    d_cas_next = nullptr;
     d_sc = nullptr;
     d_parked = nullptr;
     d_rowbuf = nullptr;
@@ -228,6 +231,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
         hipMalloc(&d_partner, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&d_match, (size_t)n * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&d_seen, (size_t)n * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_cas_next, (size_t)n * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&d_parked, (size_t)n * sizeof(int2)) != hipSuccess ||
         hipMalloc(&d_rowbuf, kRowFetch * sizeof(uint32_t)) != hipSuccess ||
@@ -408,8 +412,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
     have_result = false;
     master = false;
     matrices_valid = false;
+    times.schedule = opts.schedule;
     SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
-    if (!opts.precheck) {
+    if (!opts.precheck || opts.schedule == SMP_SCHEDULE_MW_CAS) {
       *perfect = false;
       return SMP_OK;
     }
@@ -432,6 +437,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   // -- init: rank matrix + PR prefix gather (smp_engine_bamboo.cu:33-106) -----
   int InitProc() override {
+    if (opts.schedule == SMP_SCHEDULE_MW_CAS) {
+      // The baseline needs the women's rank matrix only.
+      blocks_copied = 0;
+      SMP_HIP_TRY(hipEventRecord(ev[0], stream));
+      SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, invert_chunk(), opts.validate, d_sc, stream,
+                                             /*nontemporal=*/1));
+      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+      SMP_HIP_TRY(hipEventRecord(ev[2], stream));
+      return SMP_OK;
+    }
     if (master) {
       // Master list: the inverse of the shared list (in d_seen, free after the
       // precheck) and, when validating, a permutation check of the women's rows;
@@ -501,6 +516,15 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int CoreProc() override {
     host_finished = false;
     const int grid = propose_grid();
+    if (opts.schedule == SMP_SCHEDULE_MW_CAS) {
+      SMP_HIP_TRY(hipEventRecord(ev[3], stream));
+      SMP_HIP_TRY(pactae::launch_mw_cas(d_pref_m, d_pref_w, d_rank_w, d_seen, d_cas_next, n, d_sc, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+      if (int st = read_scalars()) return st;
+      if (int st = check_device_error("MW-CAS walk")) return st;
+      times.handoff_free = 0;
+      return SMP_OK;
+    }
     if (master) {
       SMP_HIP_TRY(hipEventRecord(ev[3], stream));
       SMP_HIP_TRY(pactae::launch_propose_master(d_pref_w, d_seen, d_partner, n, d_sc, grid, stream));
@@ -719,6 +743,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     if (perfect) {
       SMP_HIP_TRY(pactae::launch_first_match(d_pref_m, n, d_match, stream));
+    } else if (opts.schedule == SMP_SCHEDULE_MW_CAS) {
+      SMP_HIP_TRY(pactae::launch_post_cas(d_pref_w, d_seen, d_cas_next, n, d_match, d_sc, stream));
     } else if (master) {
       SMP_HIP_TRY(pactae::launch_post_master(d_partner, d_seen, n, d_match, d_sc, stream));
     } else {

@@ -10,6 +10,8 @@
 //   k_post                            PostProc               src/smp/smp_engine_bamboo.cu:299-303
 //   k_rows_equal .. k_post_master     master-list path (every man holds the same list):
 //                                     detection, women-proposing walk, matching
+//   k_mw_cas / k_post_cas             MW-Par-GPU-CAS baseline schedule
+//                                     src/smp/smp_engine_mw_3_f9.cu:37-97
 //   k_gather_row                      rows the host tail needs beyond the gathered prefix
 //                                     (the reference copies the whole PR matrix to the host
 //                                     instead, smp_engine_bamboo.h:53-61)
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(256) void k_post(const unsigned long long* __restri
 // n*H_n proposals instead of n(n+1)/2.  A man ranks woman w at Linv[w], the same
 // for every man, so a proposal needs no rank gather.
 
-// One wave per man m >= 1 compares his list with man 0's, 64 entries a step, and
+// One wave per man m >= 1 compares his list with man 0's, 512 entries a step, and
 // stops at the first difference (or when another wave has found one).
 __global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ pref_m, int n, Scalars* sc) {
   const int lane = threadIdx.x & 63;
@@ -464,10 +466,10 @@ __global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ 
   for (int m = 1 + (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); m < n; m += waves) {
     if (__hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const int32_t* row = pref_m + (size_t)m * n;
-    for (int k0 = 0; k0 < n; k0 += 256) {
+    for (int k0 = 0; k0 < n; k0 += 512) {
       bool diff = false;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < 8; ++j) {
         const int k = k0 + 64 * j + lane;
         if (k < n && row[k] != pref_m[k]) diff = true;
       }
@@ -621,6 +623,100 @@ __global__ __launch_bounds__(256) void k_post_master(const unsigned long long* _
   if (threadIdx.x == 0) atomicAdd(&sc->proposals, red[0] + red[1] + red[2] + red[3]);
 }
 
+// ---------------------------------------------------------------------------
+// MW-Par-GPU-CAS, the reference's GPU baseline (SmpEngineMw3F9::CoreProc,
+// src/smp/smp_engine_mw_3_f9.cu:37-82): one thread per man walks his chain; a
+// woman's partner is the 32-bit rank of her current man (n = free), claimed by
+// atomicCAS; the displaced man is pref_w[w][old rank] and resumes at the list
+// position recorded in next[] when he was accepted.  next[m] is written before
+// the claim, so a displacer that reads it after its own claim sees that value or
+// an earlier one (an earlier one only repeats refused proposals).
+__global__ __launch_bounds__(256) void k_mw_cas(const int32_t* __restrict__ pref_m,
+                                                const int32_t* __restrict__ pref_w,
+                                                const uint16_t* __restrict__ rank_w, int* partner_rank,
+                                                int* next, int n, Scalars* sc) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long tries = 0;
+  int bad = 0;
+  if (m < n) {
+    int k = 0;
+    for (;;) {
+      if (k >= n) {
+        bad |= kErrListEnd;
+        break;
+      }
+      const int w = pref_m[(size_t)m * n + k];
+      if ((unsigned)w >= (unsigned)n) {
+        bad |= kErrEntryRange;
+        break;
+      }
+      const int r = rank_w[(size_t)w * n + m];
+      ++k;
+      int cur = __hip_atomic_load(&partner_rank[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bool won = false;
+      while (cur > r) {
+        __hip_atomic_store(&next[m], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        ++tries;
+        const int prev = atomicCAS(&partner_rank[w], cur, r);
+        if (prev == cur) {
+          won = true;
+          break;
+        }
+        cur = prev;
+      }
+      if (!won) continue;
+      if (cur >= n) break;  // she was free
+      __threadfence();
+      m = pref_w[(size_t)w * n + cur];
+      if ((unsigned)m >= (unsigned)n) {
+        bad |= kErrEntryRange;
+        break;
+      }
+      k = __hip_atomic_load(&next[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) tries += __shfl_down(tries, off);
+  const unsigned long long bb = __ballot(bad != 0);
+  if ((threadIdx.x & 63) == 0) {
+    if (tries) atomicAdd(&sc->atomics, tries);
+    if (bb) atomicOr(&sc->error, kErrEntryRange | kErrListEnd);
+  }
+}
+
+// match[pref_w[w][partner_rank[w]]] = w (the reference's PostProc,
+// smp_engine_mw_3_f9.cu:85-97); proposals = sum over men of next[m], the list
+// position after each man's final proposal.
+__global__ __launch_bounds__(256) void k_post_cas(const int32_t* __restrict__ pref_w,
+                                                  const int* __restrict__ partner_rank,
+                                                  const int* __restrict__ next, int n,
+                                                  int32_t* __restrict__ match, Scalars* sc) {
+  __shared__ unsigned long long red[4];
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long props = 0;
+  bool unmatched = false;
+  if (w < n) {
+    const int r = partner_rank[w];
+    if ((unsigned)r >= (unsigned)n) {
+      unmatched = true;
+    } else {
+      const int m = pref_w[(size_t)w * n + r];
+      if ((unsigned)m < (unsigned)n) {
+        match[m] = w;
+        props = (unsigned long long)next[m];
+      }
+    }
+  }
+  const unsigned long long bu = __ballot(unmatched);
+  for (int off = 32; off > 0; off >>= 1) props += __shfl_down(props, off);
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = props;
+    if (bu) atomicOr(&sc->error, kErrUnmatched);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&sc->proposals, red[0] + red[1] + red[2] + red[3]);
+}
+
 // Streaming copy, 16 bytes per thread per iteration.
 __global__ void k_copy16(int4* __restrict__ dst, const int4* __restrict__ src, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
@@ -688,6 +784,22 @@ hipError_t launch_propose_master(const int32_t* pref_w, const int32_t* linv, uns
 hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
                               Scalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_post_master, dim3((n + 255) / 256), dim3(256), 0, s, partner_m, linv, n, match, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_mw_cas(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank_w, int* partner_rank,
+                         int* next, int n, Scalars* sc, hipStream_t s) {
+  hipError_t e = launch_fill_i32(partner_rank, n, n, s);
+  if (e != hipSuccess) return e;
+  if ((e = launch_fill_i32(next, n, 0, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mw_cas, dim3((n + 255) / 256), dim3(256), 0, s, pref_m, pref_w, rank_w, partner_rank, next,
+                     n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_post_cas(const int32_t* pref_w, const int* partner_rank, const int* next, int n, int32_t* match,
+                           Scalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_post_cas, dim3((n + 255) / 256), dim3(256), 0, s, pref_w, partner_rank, next, n, match, sc);
   return hipGetLastError();
 }
 

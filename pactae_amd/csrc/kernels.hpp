@@ -34,6 +34,10 @@ hipError_t launch_propose_master(const int32_t* pref_w, const int32_t* linv, uns
                                  int n, Scalars* sc, int grid, hipStream_t s);
 hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
                               Scalars* sc, hipStream_t s);
+hipError_t launch_mw_cas(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank_w, int* partner_rank,
+                         int* next, int n, Scalars* sc, hipStream_t s);
+hipError_t launch_post_cas(const int32_t* pref_w, const int* partner_rank, const int* next, int n, int32_t* match,
+                           Scalars* sc, hipStream_t s);
 hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s);
 hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int chunk, int validate,
                               Scalars* sc, hipStream_t s, int nontemporal);

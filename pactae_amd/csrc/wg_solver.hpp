@@ -251,6 +251,7 @@ __device__ int wg_chains(const InstanceRef& d, unsigned long long* partner, int*
 struct PoolSlot {
   int state, n, cursor, active;
   long long off, moff;  // element offset of the lists / of the matching
+  int master, pad;      // 1: every man holds the same list; the women propose
 };
 enum { kSlotFree = 0, kSlotOpening = 1, kSlotOpen = 2, kSlotFinal = 3 };
 
@@ -327,14 +328,23 @@ __device__ int pool_acquire(PoolSlot* slots, int* qdone, int* next_inst, int cou
 }
 
 // The batched chain walk over an instance pool (see wg_chains_g for the step).
-// Desc has fields off, moff, n.  `partner` holds NS * max_n words of LDS,
+// Desc has fields off, moff, n, idx.  `partner` holds NS * max_n words of LDS,
 // `slots` NS slots, `qdone` one int.  Returns this thread's error bits;
 // proposal counts go to *proposals.
+// Master-list instances (n >= 2 and master_flags[d.idx] = 0, i.e. no man's list
+// differs from man 0's: every man holds the same list L)
+// are solved from the women's side: the stable matching is then unique, the
+// proposers are the women with their lists in pw_all, a proposal of woman w
+// carries the rank Linv[w] (linv_all at the instance's matching offset) that
+// every man gives her, and the partner words belong to the men.  The reported
+// count is the men's Σ(Linv[match[m]] + 1), what the man-proposing walk makes.
 template <int BLOCK, int G, int JMAX, int NS, bool kTransposed, typename Desc>
 __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* next_inst,
                               const int32_t* __restrict__ pm_all, const int16_t* __restrict__ rank_all,
                               int32_t* __restrict__ match_all, unsigned long long* proposals, int max_n,
-                              unsigned long long* partner, PoolSlot* slots, int* qdone) {
+                              unsigned long long* partner, PoolSlot* slots, int* qdone,
+                              const int32_t* __restrict__ pw_all, const int* __restrict__ master_flags,
+                              const int32_t* __restrict__ linv_all) {
   static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "group size");
   static_assert(NS >= 1 && (NS & (NS - 1)) == 0, "slots");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -354,6 +364,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
         sl.n = d.n;
         sl.off = d.off;
         sl.moff = d.moff;
+        sl.master = master_flags && d.n >= 2 && !master_flags[d.idx];
         sl.state = kSlotOpen;
       } else {
         sl.n = 0;
@@ -376,14 +387,30 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
     const unsigned long long* pw_ = partner + (size_t)ss * max_n;
     int32_t* match = match_all + sl.moff;
     unsigned long long p = 0;
-    for (int w = gl; w < n; w += G) {
-      const unsigned long long v = pw_[w];
-      if (v == kEmpty) {
-        bad |= kErrUnmatched;
-      } else {
-        const uint32_t man = pw_man(v);
-        if (man < (uint32_t)n) match[man] = w;
-        p += pw_pos(v) + 1ull;
+    if (sl.master) {
+      const int32_t* linv = linv_all + sl.moff;
+      for (int a = gl; a < n; a += G) {  // the words belong to the men
+        const unsigned long long v = pw_[a];
+        if (v == kEmpty) {
+          bad |= kErrUnmatched;
+        } else {
+          const uint32_t w = pw_man(v);
+          if (w < (uint32_t)n) {
+            match[a] = (int32_t)w;
+            p += (unsigned long long)linv[w] + 1ull;
+          }
+        }
+      }
+    } else {
+      for (int w = gl; w < n; w += G) {
+        const unsigned long long v = pw_[w];
+        if (v == kEmpty) {
+          bad |= kErrUnmatched;
+        } else {
+          const uint32_t man = pw_man(v);
+          if (man < (uint32_t)n) match[man] = w;
+          p += pw_pos(v) + 1ull;
+        }
       }
     }
 #pragma unroll
@@ -404,6 +431,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       sl.n = d.n;
       sl.off = d.off;
       sl.moff = d.moff;
+      sl.master = master_flags && d.n >= 2 && !master_flags[d.idx];
       lds_st(&sl.cursor, 0);
       lds_st(&sl.state, kSlotOpen);
     }
@@ -411,8 +439,9 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
 
   const int start = (wave * (64 / G) + lane / G) & (NS - 1);
   int s = 0, n = 0, m = -1, k = 0, J = 1;
-  long long off = 0;
-  bool exited = false;
+  long long off = 0, moff = 0;
+  bool exited = false, mst = false;
+  uint32_t rprop = 0;  // master-list slot: the proposing woman's rank, Linv[m]
   for (;;) {
     int act = kActNone, a_s = 0, a_v = 0;
     if (!exited && m < 0 && gl == 0) act = pool_acquire<NS>(slots, qdone, next_inst, count, start, &a_s, &a_v);
@@ -432,6 +461,9 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       J = 1;
       n = slots[s].n;
       off = slots[s].off;
+      moff = slots[s].moff;
+      mst = slots[s].master != 0;
+      if (mst) rprop = (uint32_t)linv_all[moff + m];
     }
     if (__ballot(!exited) == 0) break;
 
@@ -447,7 +479,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
     uint32_t wf = 0;
     unsigned long long mf = 0;
     if (scan) {
-      const int32_t* pm = pm_all + off;
+      const int32_t* pm = (mst ? pw_all : pm_all) + off;  // the proposers' lists
       const int16_t* rk = rank_all + off;
       int wv[JMAX];
 #pragma unroll
@@ -465,7 +497,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       uint32_t rv[JMAX];
 #pragma unroll
       for (int j = 0; j < JMAX; ++j)
-        rv[j] = wv[j] >= 0 ? rank_of<kTransposed>(rk, n, (uint32_t)wv[j], (uint32_t)m) : 0u;
+        rv[j] = wv[j] < 0 ? 0u : mst ? rprop : rank_of<kTransposed>(rk, n, (uint32_t)wv[j], (uint32_t)m);
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) {
         if (wv[j] >= 0 && jf == 8) {
@@ -499,6 +531,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
         } else {
           m = (int)pw_man(old);
           k = (int)pw_pos(old) + 1;
+          if (mst) rprop = (uint32_t)linv_all[moff + m];
         }
         J = 1;
       } else {

@@ -190,6 +190,13 @@ def default_opts(**overrides) -> _abi.SmpOpts:
     return o
 
 
+class Schedule(enum.IntEnum):
+    """enum smp_schedule: this build's pipeline, or the reference's GPU baseline
+    MW-Par-GPU-CAS (SmpEngineMw3F9, src/smp/smp_engine_mw_3_f9.cu:37-97)."""
+    BAMBOO = _abi.SMP_SCHEDULE_BAMBOO
+    MW_CAS = _abi.SMP_SCHEDULE_MW_CAS
+
+
 class SmpEngineBamboo:
     """MI355X BambooSMP engine (SmpEngineBamboo, smp_engine_bamboo.h:13-119).
 

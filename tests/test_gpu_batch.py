@@ -84,6 +84,30 @@ def test_batch_pool_large_instances(gpu, oracle):
                 assert np.array_equal(got, want[i]), (lay, i, sizes[i], kinds[i])
 
 
+@pytest.mark.parametrize("layout", ["women_rows", "men_rows"])
+def test_batch_pool_master_list_instances(gpu, oracle, layout):
+    """All-congested batch (every man of an instance holds the same list): the pool
+    solves these from the women's side; results and the men's proposal counts
+    n(n+1)/2 must equal the oracle's, beside uniform instances in the same batch."""
+    smp = gpu
+    lay = {"women_rows": smp.SmpBatch.WOMEN_ROWS, "men_rows": smp.SmpBatch.MEN_ROWS}[layout]
+    sizes, kinds = smp.batch_plan(40, 1, 600, seed=21, mix=(0.5, 0.5, 0.0))
+    pms, pws, want, props = [], [], [], 0
+    for i, (n, k) in enumerate(zip(sizes, kinds)):
+        pm, pw = smp.generate(int(k), int(n), seed=5000 + i)
+        m, t = oracle.gs_seq(int(n), pm, pw)
+        pms.append(pm)
+        pws.append(pw)
+        want.append(m)
+        props += t["proposals"]
+    with smp.SmpBatch(sizes, pms, pws, mode=smp.SmpBatch.POOL, rank_layout=lay) as b:
+        for _ in range(2):
+            b.exec()
+            for i, got in enumerate(b.result()):
+                assert np.array_equal(got, want[i]), (i, sizes[i], kinds[i])
+            assert b.profile()["proposals"] == props
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_batch_invalid_lists_raise(gpu, mode):
     smp = gpu

@@ -57,6 +57,23 @@ def test_parity_mid_sizes(gpu, oracle, kind, cfg):
         assert prof["proposals"] == t["proposals"]
 
 
+@pytest.mark.parametrize("kind", range(8))
+def test_mw_cas_baseline_schedule(gpu, oracle, kind):
+    """The reference's GPU baseline (one thread per man, 32-bit CAS) through the
+    same engine: bit-exact, same proposal count, no fast paths taken."""
+    smp = gpu
+    for n in (1, 2, 5, 64, 65, 300, 1000):
+        if kind == 7 and n < 4:
+            continue
+        pm, pw = smp.generate(kind, n, seed=2000 + n)
+        want, t = oracle.gs_seq(n, pm, pw)
+        got, prof = solve(smp, n, pm, pw, schedule=smp.Schedule.MW_CAS)
+        assert prof["schedule"] == smp.Schedule.MW_CAS
+        assert prof["is_perfect"] == 0 and prof["master_list"] == 0
+        assert np.array_equal(got, want), (kind, n)
+        assert prof["proposals"] == t["proposals"], (kind, n)
+
+
 def test_config1_uniform_2000(gpu, oracle):
     smp = gpu
     n = 2000
