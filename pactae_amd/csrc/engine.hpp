@@ -576,6 +576,14 @@ struct smp_engine : public pactae::AbsSmpEngine {
         times.warm_ms = ms_since(t0);
       }
     } else if (T > 0) {
+      // The state the host tail reads (control block, partner words, parked
+      // chains) follows the kernel on its own stream, so that one wait covers all
+      // three copies.
+      SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
+      SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, stream));
+      SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[5], stream));
       // While the kernel runs: pull the PR prefix blocks into the host caches as
       // they land (a DMA-written buffer starts out in DRAM only).  Block 0 always;
       // later blocks only while the kernel is still running.
@@ -590,6 +598,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
         kernel_done = !touch_block(b, /*stop_when_done=*/b > 0);
         times.warm_ms += ms_since(t0);
       }
+      auto t0 = Clock::now();
+      SMP_HIP_TRY(wait_event(ev[5]));
+      times.copy_wait_ms += ms_since(t0);
+      state_copied = true;
+      if (int st = check_device_error("proposal kernel")) return st;
     }
     if (!state_copied) {
       if (int st = read_scalars()) return st;
