@@ -181,6 +181,10 @@ def batched(smp, args):
             "looped_ms": round(looped["exec_ms"], 3), "looped_solves_per_s": round(looped["solves_per_s"], 1),
             "speedup_vs_looped": round(best["solves_per_s"] / looped["solves_per_s"], 2),
             "block_threads": best["block_threads"], "wgs_per_cu": best["wgs_per_cu"],
+            "batch_mode": {1: "fused", 2: "pool"}.get(best["mode"], best["mode"]),
+            "group_lanes": best["group_lanes"], "pool_slots": best["pool_slots"],
+            "rank_layout": {0: None, 1: "women_rows", 2: "men_rows"}.get(best["rank_layout"]),
+            "rank_pass_ms": round(best["rank_ms"], 3),
             "bit_exact_batched_vs_looped": bool(same)}, (sizes, kinds, pm_all, pw_all, res_batched)
 
 
@@ -203,6 +207,8 @@ def serving(smp, args):
     def run(mode):
         results = [None] * len(picks)
         extra = {"large_n": args.serve_large_n} if args.serve_large_n > 0 else {}
+        if getattr(args, "serve_large_workers", 0) > 0:
+            extra["large_workers"] = args.serve_large_workers
         with smp.SmpServer(mode=mode, lanes=3, max_batch=64, max_n=8192, **extra) as srv:
             # warm-up request (kernel load, first allocations)
             n0, d0m, d0w, _, _ = pool[0]
@@ -272,6 +278,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=1500.0)
     ap.add_argument("--serve-large-n", type=int, default=0,
                     help="server large_n (requests this large use the single-instance pipeline); 0 = default")
+    ap.add_argument("--serve-large-workers", type=int, default=0,
+                    help="server large_workers (threads with their own pipeline); 0 = default")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the measured choice of hand-off threshold and prefix width")

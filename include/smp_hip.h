@@ -294,11 +294,13 @@ typedef struct smp_server_opts {
   int32_t max_n;            /* largest request accepted; 0 = 8192 (int16 ranks)      */
   int32_t block_threads;    /* workgroup size of the serving kernel; 0 = auto        */
   int64_t hbm_budget_bytes; /* device arena for request state; 0 = 4 GiB             */
-  int32_t large_n;          /* batched mode: requests with n >= large_n are solved one
-                               at a time by the multi-workgroup single-instance
-                               pipeline on a worker of their own, beside the lanes;
-                               0 = 4096; above max_n = never                       */
-  int32_t reserved[7];
+  int32_t large_n;          /* batched mode: requests with n >= large_n are solved by
+                               the multi-workgroup single-instance pipeline on
+                               workers of their own, beside the lanes;
+                               0 = 512; above max_n = never                        */
+  int32_t large_workers;    /* threads (each with its own pipeline) serving the large
+                               requests; 0 = 3                                     */
+  int32_t reserved[6];
 } smp_server_opts;
 
 typedef struct smp_request_info {

@@ -159,7 +159,8 @@ class SmpServerOpts(ctypes.Structure):
         ("block_threads", i32),
         ("hbm_budget_bytes", i64),
         ("large_n", i32),
-        ("reserved", i32 * 7),
+        ("large_workers", i32),
+        ("reserved", i32 * 6),
     ]
 
 

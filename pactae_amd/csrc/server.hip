@@ -14,8 +14,9 @@
 //   k_serve_chains  one workgroup per request: proposal chains, 16 lanes per
 //                   chain, with the partner words in LDS, then the matching,
 //                   written straight into pinned host memory.
-// Requests with n >= large_n skip the lanes: a worker thread of their own solves
-// them one at a time with the multi-workgroup single-instance pipeline
+// Requests with n >= large_n skip the lanes: large_workers threads, each with a
+// pipeline of its own, solve them one at a time each with the multi-workgroup
+// single-instance pipeline
 // (engine.hpp), whose chains run over the whole GPU instead of one workgroup, so a
 // large congested request does not hold a lane for its whole serial chain.
 // The plain mode is the baseline: one request at a time through the plain
@@ -151,13 +152,15 @@ struct smp_server {
   char* d_arena = nullptr;
   std::vector<Lane> lanes;
   hipStream_t copy_stream = nullptr;
-  std::unique_ptr<smp_engine> plain;  // plain mode: the whole path; batched: large requests
+  // Single-instance pipelines: plain mode uses engines[0] for every request; the
+  // batched mode gives one to each large-request worker.
+  std::vector<std::unique_ptr<smp_engine>> engines;
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Request*> pending, ready, ready_large;
   std::condition_variable cv_large;
-  std::thread large_worker;
+  std::vector<std::thread> large_workers;
   std::map<uint64_t, std::unique_ptr<Request>> table;
   uint64_t next_ticket = 1;
   bool stop = false, stop_large = false;
@@ -190,16 +193,16 @@ struct smp_server {
     }
     arena.init((size_t)opts.hbm_budget_bytes);
     SMP_HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
-    if (opts.mode == SMP_SERVE_PLAIN || opts.large_n <= opts.max_n) {
+    const int nengines = opts.mode == SMP_SERVE_PLAIN ? 1 : opts.large_n <= opts.max_n ? opts.large_workers : 0;
+    for (int i = 0; i < nengines; ++i) {
       smp_opts o;
       smp_default_opts(&o);
       o.device = opts.device;
-      o.tail_mode = SMP_TAIL_GPU;
       auto e = std::make_unique<smp_engine>();
       e->n = opts.max_n;
       e->opts = o;
       if (int st = e->setup()) return st;
-      plain = std::move(e);
+      engines.push_back(std::move(e));
     }
     if (opts.mode == SMP_SERVE_BATCHED) {
       lanes.resize(opts.lanes);
@@ -222,27 +225,28 @@ struct smp_server {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
       SMP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&pactae::k_serve_chains<1024>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-      if (plain) large_worker = std::thread([this] { large_loop(); });
+      for (size_t i = 0; i < engines.size(); ++i)
+        large_workers.emplace_back([this, i] { large_loop(*engines[i]); });
     }
     worker = std::thread([this] { loop(); });
     return SMP_OK;
   }
 
   // Solves one request with the single-instance pipeline; `lane` is recorded.
-  void solve_plain(std::unique_lock<std::mutex>& lk, Request* r, int lane) {
+  void solve_plain(std::unique_lock<std::mutex>& lk, Request* r, int lane, smp_engine& plain) {
     r->state = kLaunched;
     r->t_launch = Clock::now();
     r->batch_size = 1;
     r->lane = lane;
     ++stats.launches;
     lk.unlock();
-    int st = plain->run_view(r->d_pm, r->d_pw, r->n, nullptr);
+    int st = plain.run_view(r->d_pm, r->d_pw, r->n, nullptr);
     std::vector<int32_t> res;
     int64_t props = 0;
     std::string err;
     if (st == SMP_OK) {
-      res.assign(plain->h_match, plain->h_match + r->n);
-      props = plain->times.proposals;
+      res.assign(plain.h_match, plain.h_match + r->n);
+      props = plain.times.proposals;
     } else {
       err = pactae::last_error();
     }
@@ -253,7 +257,7 @@ struct smp_server {
   }
 
   // Batched mode: the large-request worker.
-  void large_loop() {
+  void large_loop(smp_engine& eng) {
     (void)hipSetDevice(opts.device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
@@ -261,7 +265,7 @@ struct smp_server {
       if (ready_large.empty()) break;
       Request* r = ready_large.front();
       ready_large.pop_front();
-      solve_plain(lk, r, -1);
+      solve_plain(lk, r, -1, eng);
       ++stats.large_solved;
       cv_work.notify_one();  // its arena block is free for admission
     }
@@ -279,7 +283,8 @@ struct smp_server {
       stop_large = true;
     }
     cv_large.notify_all();
-    if (large_worker.joinable()) large_worker.join();
+    for (auto& t : large_workers)
+      if (t.joinable()) t.join();
     for (auto& L : lanes) {
       if (L.stream) (void)hipStreamSynchronize(L.stream);
       (void)hipHostFree(L.h_reqs);
@@ -292,7 +297,7 @@ struct smp_server {
       if (L.done) (void)hipEventDestroy(L.done);
       if (L.stream) (void)hipStreamDestroy(L.stream);
     }
-    plain.reset();
+    engines.clear();
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
     (void)hipFree(d_arena);
   }
@@ -356,7 +361,7 @@ struct smp_server {
         std::vector<int32_t>().swap(r->host_pw);
       }
       r->state = kReady;
-      if (is_large(*r) && large_worker.joinable()) {
+      if (is_large(*r) && !large_workers.empty()) {
         ready_large.push_back(r);
         cv_large.notify_one();
       } else {
@@ -445,7 +450,7 @@ struct smp_server {
         if (!ready.empty()) {
           Request* r = ready.front();
           ready.pop_front();
-          solve_plain(lk, r, 0);
+          solve_plain(lk, r, 0, *engines[0]);
           progressed = true;
         }
       } else if (!fatal) {
@@ -504,7 +509,8 @@ void smp_server_default_opts(smp_server_opts* o) {
   o->max_n = 8192;
   o->block_threads = 0;
   o->hbm_budget_bytes = 4ll << 30;
-  o->large_n = 4096;
+  o->large_n = 512;
+  o->large_workers = 3;
 }
 
 int smp_server_create(const smp_server_opts* opts, smp_server** out) {
@@ -521,7 +527,12 @@ int smp_server_create(const smp_server_opts* opts, smp_server** out) {
   if (o.max_n <= 0) o.max_n = 8192;
   if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = 4ll << 30;
   if (o.block_threads != 256 && o.block_threads != 512 && o.block_threads != 1024) o.block_threads = 1024;
-  if (o.large_n <= 0) o.large_n = 4096;
+  if (o.large_n <= 0) o.large_n = 512;
+  if (o.large_workers <= 0) o.large_workers = 3;
+  if (o.large_workers > 8) {
+    pactae::set_error("server: large_workers must be <= 8");
+    return SMP_ERR_INVALID;
+  }
   if (o.max_n > 20000 || o.lanes > 16 || o.max_batch > 4096 ||
       (o.mode != SMP_SERVE_BATCHED && o.mode != SMP_SERVE_PLAIN)) {
     pactae::set_error("server: max_n must be <= 20000 (LDS partner words), lanes <= 16, max_batch <= 4096");
