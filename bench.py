@@ -46,11 +46,11 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def measured_traffic(n):
-    """HBM bytes per k_invert_rows launch from the committed rocprofv3 counter runs
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration), if
     they were taken at this n; else None."""
-    path = os.path.join(ROOT, "profiles", f"counters_k_invert_rows_n{n}.json")
+    path = os.path.join(ROOT, "profiles", f"counters_{kernel}_n{n}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -72,6 +72,8 @@ def single_instance(smp, args):
         opts["tail_threshold"] = args.tail_threshold
     if args.prefix_cols is not None:
         opts["prefix_cols"] = args.prefix_cols
+    if args.init_mode is not None:
+        opts["init_mode"] = args.init_mode
     eng = smp.SmpEngineBamboo.from_device(dm, dw, n, **opts)
     for _ in range(args.warmup):
         eng.exec()
@@ -272,6 +274,7 @@ def main() -> int:
     ap.add_argument("--tail-mode", type=int, default=None)
     ap.add_argument("--tail-threshold", type=int, default=None)
     ap.add_argument("--prefix-cols", type=int, default=None)
+    ap.add_argument("--init-mode", type=int, default=None, help="1 = full rank matrix, 2 = sparse prefix pass")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-pool", type=int, default=48)
     ap.add_argument("--serve-requests", type=int, default=1500)
@@ -310,13 +313,22 @@ def main() -> int:
     pm, pw, match, walls, profs, elapsed, untuned, tune, gpu_base = single_instance(smp, args)
     n = args.n
     ms_per_solve = elapsed * 1e3 / args.steps
+    last = profs[-1]
     rank_ms = statistics.mean(p["rank_kernel_ms"] for p in profs)
-    rank_bytes = 6 * n * n  # 4 B read from pref_w + 2 B written to the 16-bit rank matrix per entry
+    K = last["prefix_cols"]
+    if last["sparse_init"] and not last["rank_matrix"]:
+        # Sparse prefix pass: 4 B read per women's list entry, plus per prefix entry
+        # 4 B of bucket read and 4 B of PR prefix written.
+        rank_kernel = ("k_rank_prefix", "k_invert_rows<4,1,1> (sparse prefix pass over the women's rows)")
+        rank_bytes = 4 * n * n + 8 * n * K
+    else:
+        # Full rank matrix: 4 B read from pref_w + 2 B written to the 16-bit rank matrix per entry.
+        rank_kernel = ("k_invert_rows", "k_invert_rows<4,1,0> (women's rank matrix)")
+        rank_bytes = 6 * n * n
     achieved = rank_bytes / (rank_ms * 1e-3) / 1e9
     phases = {k: round(statistics.mean(p[k] for p in profs), 4) for k in
-              ("precheck_ms", "init_ms", "core_ms", "post_ms", "total_ms", "rank_kernel_ms",
-               "gather_kernel_ms", "propose_kernel_ms", "handoff_ms", "host_tail_ms")}
-    last = profs[-1]
+              ("precheck_ms", "init_ms", "core_ms", "post_ms", "total_ms", "bucket_kernel_ms",
+               "rank_kernel_ms", "gather_kernel_ms", "propose_kernel_ms", "handoff_ms", "host_tail_ms")}
     out = {
         "metric": "ms/solve at n=20k (bit-exact); batched solves/s vs looped plain launches",
         "value": round(ms_per_solve, 4),
@@ -344,9 +356,10 @@ def main() -> int:
         "handoff_free": last["handoff_free"],
         "host_proposals": last["host_proposals"],
         "bailed_to_gpu": last["bailed"],
-        "roofline": {"kernel": "k_invert_rows (women's rank matrix)", "bound": "hbm",
+        "init": {"sparse_prefix_pass": bool(last["sparse_init"]), "rank_matrix_built": bool(last["rank_matrix"])},
+        "roofline": {"kernel": rank_kernel[1], "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(n),
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(rank_kernel[0], n),
                      "algorithmic_bytes": rank_bytes, "kernel_ms": round(rank_ms, 4)},
         "cpu_baseline": None,
         "gpu_baseline": gpu_base,

@@ -118,8 +118,19 @@ typedef struct smp_opts {
                              the engine's stream, 2 = block 0 after it and later
                              blocks on a copy stream                              */
   int32_t schedule;       /* enum smp_schedule                                     */
-  int32_t reserved[4];
+  int32_t init_mode;      /* enum smp_init_mode                                    */
+  int32_t reserved[3];
 } smp_opts;
+
+enum smp_init_mode {
+  SMP_INIT_AUTO = 0,   /* sparse prefix pass when prefix_cols <= n / 4, else full  */
+  SMP_INIT_FULL = 1,   /* the n x n women's rank matrix, then the PR prefix gather
+                          (the reference's init, kernel.cu:672-709)                */
+  SMP_INIT_SPARSE = 2  /* PR prefix straight from the women's rows: the rank
+                          matrix is built only if a chain walks past the prefix
+                          or a woman is named in more prefixes than her bucket
+                          holds; results are identical in every mode              */
+};
 
 enum smp_schedule {
   SMP_SCHEDULE_BAMBOO = 0, /* this build's pipeline (default)                       */
@@ -140,6 +151,8 @@ typedef struct smp_times {
   double copy_wait_ms;     /* host wait for the PR-prefix copy after the kernel launch */
   double warm_ms;          /* host cache warm-up of that copy                          */
   double resume_ms;        /* proposal launch for chains the host handed back          */
+  double bucket_kernel_ms; /* sparse prefix pass: the bucket kernel (HIP events); the
+                              rank pass after it is in rank_kernel_ms               */
   int64_t proposals;       /* sum over men of (final list position + 1)           */
   int64_t atomics;         /* atomicMin attempts made by the proposal kernel(s)   */
   int64_t host_proposals;  /* proposals made by the host tail                     */
@@ -154,7 +167,10 @@ typedef struct smp_times {
                               the women-proposing walk answered (the stable
                               matching is then unique)                            */
   int32_t schedule;        /* enum smp_schedule of this run                       */
-  int32_t reserved[5];
+  int32_t sparse_init;     /* 1 if init took the sparse prefix pass               */
+  int32_t rank_matrix;     /* 1 if the n x n rank matrix was built (full init, a
+                              bucket overflow, or a chain past the prefix)        */
+  int32_t reserved[3];
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;

@@ -47,7 +47,8 @@ class SmpOpts(ctypes.Structure):
         ("host_spin", i32),
         ("prefix_copy", i32),
         ("schedule", i32),
-        ("reserved", i32 * 4),
+        ("init_mode", i32),
+        ("reserved", i32 * 3),
     ]
 
 
@@ -66,6 +67,7 @@ class SmpTimes(ctypes.Structure):
         ("copy_wait_ms", ctypes.c_double),
         ("warm_ms", ctypes.c_double),
         ("resume_ms", ctypes.c_double),
+        ("bucket_kernel_ms", ctypes.c_double),
         ("proposals", i64),
         ("atomics", i64),
         ("host_proposals", i64),
@@ -77,7 +79,9 @@ class SmpTimes(ctypes.Structure):
         ("bailed", i32),
         ("master_list", i32),
         ("schedule", i32),
-        ("reserved", i32 * 5),
+        ("sparse_init", i32),
+        ("rank_matrix", i32),
+        ("reserved", i32 * 3),
     ]
 
     def as_dict(self) -> dict:
@@ -199,6 +203,9 @@ class SmpServerStats(ctypes.Structure):
 
 SMP_SCHEDULE_BAMBOO = 0
 SMP_SCHEDULE_MW_CAS = 1
+SMP_INIT_AUTO = 0
+SMP_INIT_FULL = 1
+SMP_INIT_SPARSE = 2
 SMP_BATCH_AUTO = 0
 SMP_BATCH_FUSED = 1
 SMP_BATCH_POOL = 2

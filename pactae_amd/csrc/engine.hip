@@ -33,6 +33,7 @@ void smp_default_opts(smp_opts* o) {
   o->host_spin = 0;
   o->prefix_copy = 0;
   o->schedule = SMP_SCHEDULE_BAMBOO;
+  o->init_mode = SMP_INIT_AUTO;
 }
 
 static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
@@ -52,6 +53,11 @@ static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
     e->opts = *opts;
   else
     smp_default_opts(&e->opts);
+  if (e->opts.init_mode < SMP_INIT_AUTO || e->opts.init_mode > SMP_INIT_SPARSE ||
+      e->opts.schedule < SMP_SCHEDULE_BAMBOO || e->opts.schedule > SMP_SCHEDULE_MW_CAS) {
+    pactae::set_error("opts: unknown init_mode or schedule");
+    return SMP_ERR_INVALID;
+  }
   if (int st = e->setup()) return st;
   *out = e.release();
   return SMP_OK;
@@ -176,6 +182,7 @@ int smp_rank_matrix(smp_engine* e, int32_t* rank_out) {
     pactae::set_error("smp_rank_matrix: the last run built no rank matrix (first-choice or master-list path)");
     return SMP_ERR_STATE;
   }
+  if (int st = e->ensure_rank_matrix()) return st;  // a sparse prefix pass skips it
   SMP_HIP_TRY(hipStreamSynchronize(e->stream));
   const size_t nn = (size_t)e->n * e->n;
   std::vector<uint16_t> r16(nn);

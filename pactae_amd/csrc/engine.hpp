@@ -134,15 +134,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
   bool matrices_valid = false;  // d_rank_w / d_pr hold the last run's matrices
   Scalars* d_sc = nullptr;
   int2* d_parked = nullptr;
-  uint32_t* d_rowbuf = nullptr;
   unsigned long long* d_wstats = nullptr;  // per proposal wave: atomics, steps
+  uint32_t* d_bucket = nullptr;  // sparse prefix pass: per-woman buckets of (m, k)
+  unsigned long long* d_bins = nullptr;  // sparse prefix pass: bins of 256 women
+  int* d_bin_cnt = nullptr;      // entries per bin
+  int K_sparse_alloc = 0;        // widest prefix the sparse buffers hold
+  bool sparse = false;           // this run's init took the sparse prefix pass
+  bool rank_built = false;       // d_rank_w holds this run's women's rank matrix
 
   uint32_t* h_pr = nullptr;  // pinned copy of the PR prefix for the host tail
   uint32_t* h_pr_dev = nullptr;  // the same buffer as seen from the device
   unsigned long long* h_partner = nullptr;
   Scalars* h_sc = nullptr;
   int2* h_parked = nullptr;
-  uint32_t* h_rowbuf = nullptr;
   int32_t* h_match = nullptr;  // pinned result (man -> woman)
   int2* d_starts = nullptr;     // chains handed back by the host tail
   bool host_finished = false;   // the host tail completed every chain
@@ -158,8 +162,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
   smp_times times{};
   volatile uint64_t touch_sink = 0;
   long long host_proposals_total = 0;
-  std::vector<uint32_t> hr, hm, hp;  // host tail: partner rank / man / position per woman
-  static constexpr int kRowFetch = 2048;
+  // Host tail state, one word per woman: rank in bits 0-15 (0xFFFF = free), man
+  // in 16-31, list position in 32-47 (all < n <= 65535).  One word keeps each
+  // proposal on one cache line.
+  std::vector<uint64_t> hst;
+  static constexpr uint32_t kHostFree = 0xFFFFu;
+  static constexpr int kTailSlots = 4;  // chains the host walks interleaved
+  static uint64_t hs_pack(uint32_t r, uint32_t m, uint32_t p) {
+    return (uint64_t)r | ((uint64_t)m << 16) | ((uint64_t)p << 32);
+  }
+  static int tail_error(const char* what) {
+    pactae::set_error(what);
+    return SMP_ERR_INVALID;
+  }
 
   ~smp_engine() override { release(); }
 
@@ -180,7 +195,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipFree(d_cas_next);
     (void)hipFree(d_sc);
     (void)hipFree(d_parked);
-    (void)hipFree(d_rowbuf);
+    (void)hipFree(d_bucket);
+    d_bucket = nullptr;
+    (void)hipFree(d_bins);
+    d_bins = nullptr;
+    (void)hipFree(d_bin_cnt);
+    d_bin_cnt = nullptr;
     (void)hipFree(d_wstats);
     d_wstats = nullptr;
     (void)hipFree(d_starts);
@@ -191,7 +211,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipHostFree(h_partner);
     (void)hipHostFree(h_sc);
     (void)hipHostFree(h_parked);
-    (void)hipHostFree(h_rowbuf);
     d_rank_w = nullptr;
     d_pr = nullptr;
     d_partner = nullptr;
@@ -200,12 +219,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     d_cas_next = nullptr;
     d_sc = nullptr;
     d_parked = nullptr;
-    d_rowbuf = nullptr;
     h_pr = nullptr;
     h_partner = nullptr;
     h_sc = nullptr;
     h_parked = nullptr;
-    h_rowbuf = nullptr;
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     for (auto& e : ev_blk)
@@ -234,7 +251,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
         hipMalloc(&d_cas_next, (size_t)n * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&d_parked, (size_t)n * sizeof(int2)) != hipSuccess ||
-        hipMalloc(&d_rowbuf, kRowFetch * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_wstats, (size_t)dev.cus * 32 * 2 * sizeof(unsigned long long)) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("device allocation failed for n=" + std::to_string(n));
@@ -245,8 +261,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
         hipHostMalloc(&h_partner, (size_t)n * sizeof(unsigned long long), pactae::kHostAllocFlags) !=
             hipSuccess ||
         hipHostMalloc(&h_sc, sizeof(Scalars), pactae::kHostAllocFlags) != hipSuccess ||
-        hipHostMalloc(&h_parked, (size_t)n * sizeof(int2), pactae::kHostAllocFlags) != hipSuccess ||
-        hipHostMalloc(&h_rowbuf, kRowFetch * sizeof(uint32_t), pactae::kHostAllocFlags) != hipSuccess) {
+        hipHostMalloc(&h_parked, (size_t)n * sizeof(int2), pactae::kHostAllocFlags) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("pinned host allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
@@ -266,6 +281,40 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error("allocation of the result buffers failed");
       return SMP_ERR_NOMEM;
     }
+    // Sparse prefix pass buffers, for prefixes up to 256 columns (any width when
+    // n <= 2048, where n x n buckets are small).
+    K_sparse_alloc = n <= 2048 ? K_alloc : std::min(K_alloc, 256);
+    if (hipMalloc(&d_bucket, (size_t)n * bucket_cap(n, K_sparse_alloc) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_bins, (size_t)pactae::bin_count(n) * bin_cap(K_sparse_alloc) * sizeof(unsigned long long)) !=
+            hipSuccess ||
+        hipMalloc(&d_bin_cnt, 256 * sizeof(int)) != hipSuccess) {
+      (void)hipGetLastError();
+      pactae::set_error("allocation of the prefix buckets failed for n=" + std::to_string(n));
+      return SMP_ERR_NOMEM;
+    }
+    return SMP_OK;
+  }
+
+  // Sparse prefix pass: bucket capacity per woman for prefix width k.  A woman's
+  // suitor count over the first k positions of n uniform lists is about
+  // Poisson(k); k + max(64, k/2) is more than 5.6 standard deviations above it
+  // (overflow only falls back to the full rank matrix).  With k = ni every man
+  // names every woman, so ni slots always suffice.
+  static int bucket_cap(int ni, int k) { return std::min(ni, k + std::max(64, k / 2)); }
+  // Entries per bin of 256 women: 256 k on average for uniform lists (exactly
+  // that at most when k = n), plus 12.5% and 1024 of slack.
+  static int bin_cap(int k) { return 256 * k + 32 * k + 1024; }
+  bool use_sparse() const {
+    if (opts.init_mode == SMP_INIT_FULL || K > K_sparse_alloc) return false;
+    return opts.init_mode == SMP_INIT_SPARSE || 4ll * K <= n;
+  }
+  // The women's rank matrix, if this run has not built it (sparse prefix pass).
+  int ensure_rank_matrix() {
+    if (rank_built) return SMP_OK;
+    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, invert_chunk(), opts.validate, d_sc, stream,
+                                           /*nontemporal=*/1));
+    rank_built = true;
+    times.rank_matrix = 1;
     return SMP_OK;
   }
 
@@ -412,6 +461,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     have_result = false;
     master = false;
     matrices_valid = false;
+    sparse = false;
+    rank_built = false;
     times.schedule = opts.schedule;
     SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
     if (!opts.precheck || opts.schedule == SMP_SCHEDULE_MW_CAS) {
@@ -443,6 +494,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
       SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, invert_chunk(), opts.validate, d_sc, stream,
                                              /*nontemporal=*/1));
+      rank_built = true;
+      times.rank_matrix = 1;
       SMP_HIP_TRY(hipEventRecord(ev[1], stream));
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
       return SMP_OK;
@@ -462,12 +515,33 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     const int chunk = invert_chunk();
     matrices_valid = true;
+    sparse = use_sparse();
+    times.sparse_init = sparse ? 1 : 0;
     SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
-    SMP_HIP_TRY(hipEventRecord(ev[0], stream));
-    SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream,
-                                           /*nontemporal=*/1));
-    SMP_HIP_TRY(hipEventRecord(ev[1], stream));
-    SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
+    if (sparse) {
+      // Buckets of (man, position) per woman over the first K positions (counts
+      // in d_seen, free after the precheck), then one pass over the women's rows
+      // that writes the PR prefix directly.  If a bucket overflowed, that pass
+      // writes the full rank matrix instead and the gated gather builds the
+      // prefix from it.
+      const int cap = bucket_cap(n, K);
+      SMP_HIP_TRY(hipEventRecord(ev[6], stream));
+      SMP_HIP_TRY(pactae::launch_bucket_prefix(d_pref_m, n, K, d_bin_cnt, d_bins, bin_cap(K), d_seen, d_bucket, cap,
+                                               d_sc, dev.cus, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[0], stream));
+      SMP_HIP_TRY(pactae::launch_rank_prefix(d_pref_w, d_rank_w, n, chunk, opts.validate, d_seen, d_bucket, cap,
+                                             d_pr, d_sc, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, /*gated=*/1, d_sc, stream));
+    } else {
+      SMP_HIP_TRY(hipEventRecord(ev[0], stream));
+      SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream,
+                                             /*nontemporal=*/1));
+      rank_built = true;
+      times.rank_matrix = 1;
+      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, /*gated=*/0, d_sc, stream));
+    }
     SMP_HIP_TRY(hipEventRecord(ev[2], stream));
     blocks_copied = 0;
     blocks_ready = 0;
@@ -510,6 +584,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.poll = opts.poll_interval > 0 ? opts.poll_interval : 16;
     a.starts = starts;
     a.nstarts = nstarts;
+    a.rank_sparse = rank_built ? 0 : 1;
     return a;
   }
 
@@ -608,6 +683,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
       if (int st = read_scalars()) return st;
       if (int st = check_device_error("proposal kernel")) return st;
     }
+    if (sparse && h_sc->prefix_overflow != 0) {  // the sparse pass fell back to the full matrix
+      rank_built = true;
+      times.rank_matrix = 1;
+    }
     const int handoff = h_sc->parked_count;
     times.handoff_free = handoff;
     if (handoff > 0) {
@@ -638,10 +717,14 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   // Host tail: McVitie-Wilson chains (smp_engine_mw_1_f9.cu:15-79, and the
   // reference's host LAProcedure smp_engine_bamboo.cu:197-230) over the partner
-  // state the kernel left, split into rank / man / position arrays that stay in
-  // the host caches.  A chain that walks past the PR prefix is handed back: it
-  // is collected, and once every other chain is done the GPU resumes all of them
-  // from the shared partner words (the host never stalls on single-row fetches).
+  // state the kernel left, kept as one packed word per woman (hst) so that it
+  // stays in the host caches.  kTailSlots chains are walked interleaved: a chain
+  // runs until its next acceptance, prefetches the row of the man it displaced
+  // and yields to the next chain, so the row fetches of several chains overlap
+  // (the stable matching does not depend on the order of proposals).  A chain
+  // that walks past the host's PR prefix is handed back: it is collected, and
+  // once every other chain is done the GPU resumes all of them from the shared
+  // partner words (the host never stalls on single-row fetches).
   int host_tail() {
     auto t0 = Clock::now();
     const int parked = h_sc->parked_count;
@@ -655,30 +738,37 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.handoff_ms = ms_since(t0);
 
     t0 = Clock::now();
-    constexpr uint32_t kFree = 0xFFFFFFFFu;
-    hr.resize(n);
-    hm.resize(n);
-    hp.resize(n);
+    hst.resize(n);
     for (int w = 0; w < n; ++w) {
       const unsigned long long v = h_partner[w];
-      hr[w] = v == pactae::kEmpty ? kFree : pactae::pw_rank(v);
-      hm[w] = pactae::pw_man(v);
-      hp[w] = pactae::pw_pos(v);
+      hst[w] = v == pactae::kEmpty ? kHostFree : hs_pack(pactae::pw_rank(v), pactae::pw_man(v), pactae::pw_pos(v));
     }
     long long props = 0;
     int bailed = 0;
     const uint32_t* pr = h_pr;
+    uint64_t* st = hst.data();
     const int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
-    int2* bail = h_parked;  // handed-back chains overwrite the consumed parked list
-    for (int i = 0; i < parked; ++i) {
-      int m = h_parked[i].x, k = h_parked[i].y;
-      if ((unsigned)m >= (unsigned)n) {
-        pactae::set_error("host tail: parked man outside [0,n)");
-        return SMP_ERR_INVALID;
-      }
+    int2* bail = h_parked;  // handed-back chains overwrite the consumed parked list (bailed <= taken)
+    int cm[kTailSlots], ck[kTailSlots];
+    int act = 0, taken = 0;
+    // Puts parked chain `taken` into slot s; false if it names a man outside [0,n).
+    auto take = [&](int s) {
+      cm[s] = h_parked[taken].x;
+      ck[s] = h_parked[taken].y;
+      ++taken;
+      return (unsigned)cm[s] < (unsigned)n;
+    };
+    for (; act < kTailSlots && taken < parked; ++act)
+      if (!take(act)) return tail_error("host tail: parked man outside [0,n)");
+    int i = 0;
+    while (act > 0) {
+      if (i >= act) i = 0;
+      int m = cm[i], k = ck[i];
+      bool ended = false;  // chain reached a free woman, or was handed back
       for (;;) {
         if (k >= kcap) {
           bail[bailed++] = make_int2(m, k);
+          ended = true;
           break;
         }
         if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
@@ -687,21 +777,31 @@ struct smp_engine : public pactae::AbsSmpEngine {
         const uint32_t e = pr[pactae::pr_index(m, k, n)];
         ++props;
         const uint32_t w = pactae::pr_woman(e), r = pactae::pr_rank(e);
-        if (w >= (uint32_t)n) {
-          pactae::set_error("host tail: preference entry outside [0,n)");
-          return SMP_ERR_INVALID;
+        if (w >= (uint32_t)n) return tail_error("host tail: preference entry outside [0,n)");
+        const uint64_t v = st[w];
+        if (r < (uint32_t)(v & 0xFFFFu)) {
+          st[w] = hs_pack(r, (uint32_t)m, (uint32_t)k);
+          if ((v & 0xFFFFu) == kHostFree) {
+            ended = true;
+            break;
+          }
+          m = (int)((v >> 16) & 0xFFFFu);
+          k = (int)(v >> 32) + 1;
+          if (k < kcap) __builtin_prefetch(pr + pactae::pr_index(m, k, n));
+          break;
         }
-        if (r < hr[w]) {
-          const uint32_t old_r = hr[w], old_m = hm[w], old_p = hp[w];
-          hr[w] = r;
-          hm[w] = (uint32_t)m;
-          hp[w] = (uint32_t)k;
-          if (old_r == kFree) break;
-          m = (int)old_m;
-          k = (int)old_p + 1;
-        } else {
-          ++k;
-        }
+        ++k;
+      }
+      if (!ended) {
+        cm[i] = m;
+        ck[i] = k;
+        ++i;
+      } else if (taken < parked) {
+        if (!take(i)) return tail_error("host tail: parked man outside [0,n)");
+      } else {
+        --act;
+        cm[i] = cm[act];
+        ck[i] = ck[act];
       }
     }
     times.host_proposals = props;
@@ -710,24 +810,29 @@ struct smp_engine : public pactae::AbsSmpEngine {
       // Every chain finished on the host: assemble the matching here.
       long long total = 0;
       for (int w = 0; w < n; ++w) {
-        if (hr[w] == kFree) {
-          pactae::set_error("host tail: a woman was left unmatched");
-          return SMP_ERR_INVALID;
-        }
-        h_match[hm[w]] = w;
-        total += hp[w] + 1ll;
+        const uint64_t v = st[w];
+        if ((v & 0xFFFFu) == kHostFree) return tail_error("host tail: a woman was left unmatched");
+        h_match[(v >> 16) & 0xFFFFu] = w;
+        total += (long long)(v >> 32) + 1;
       }
       host_proposals_total = total;
       host_finished = true;
       times.host_tail_ms = ms_since(t0);
       return SMP_OK;
     }
-    for (int w = 0; w < n; ++w)
-      h_partner[w] = hr[w] == kFree ? pactae::kEmpty : pactae::pw_pack(hr[w], hm[w], hp[w]);
+    for (int w = 0; w < n; ++w) {
+      const uint64_t v = st[w];
+      h_partner[w] = (v & 0xFFFFu) == kHostFree
+                         ? pactae::kEmpty
+                         : pactae::pw_pack((uint32_t)(v & 0xFFFFu), (uint32_t)((v >> 16) & 0xFFFFu),
+                                           (uint32_t)(v >> 32));
+    }
     times.host_tail_ms = ms_since(t0);
 
-    // Resume the handed-back chains on the GPU (no hand-off this time).
+    // Resume the handed-back chains on the GPU (no hand-off this time), with the
+    // rank matrix for the positions past the prefix.
     t0 = Clock::now();
+    if (int st = ensure_rank_matrix()) return st;
     SMP_HIP_TRY(hipMemcpyAsync(d_partner, h_partner, (size_t)n * sizeof(unsigned long long),
                                hipMemcpyHostToDevice, stream));
     SMP_HIP_TRY(hipMemcpyAsync(d_starts, bail, (size_t)bailed * sizeof(int2), hipMemcpyHostToDevice,
@@ -811,6 +916,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int init_only(bool init_pr, smp_init_times* t) {
     *t = smp_init_times{};
     matrices_valid = true;
+    sparse = false;
+    rank_built = true;
     SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
     const int chunk = invert_chunk();
     auto t0 = Clock::now();
@@ -822,7 +929,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     t->rank_ms = ms_since(t0);
     if (init_pr) {
       t0 = Clock::now();
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, /*gated=*/0, d_sc, stream));
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
       SMP_HIP_TRY(wait_stream(stream));
       t->pr_ms = ms_since(t0);
@@ -861,6 +968,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (!times.is_perfect) {
       float ms = 0;
       if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) times.rank_kernel_ms = ms;
+      if (sparse && hipEventElapsedTime(&ms, ev[6], ev[0]) == hipSuccess) times.bucket_kernel_ms = ms;
       if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) times.gather_kernel_ms = ms;
       if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess) times.propose_kernel_ms = ms;
     }

@@ -12,9 +12,10 @@
 //                                     detection, women-proposing walk, matching
 //   k_mw_cas / k_post_cas             MW-Par-GPU-CAS baseline schedule
 //                                     src/smp/smp_engine_mw_3_f9.cu:37-97
-//   k_gather_row                      rows the host tail needs beyond the gathered prefix
-//                                     (the reference copies the whole PR matrix to the host
-//                                     instead, smp_engine_bamboo.h:53-61)
+//   k_bucket_prefix + k_invert_rows<.., SP=1>
+//                                     sparse prefix pass: F9RankMatrixInitKernel and
+//                                     F9PRMatrixInitKernel restricted to the first K list
+//                                     positions, without writing the n x n rank matrix
 //
 // All arithmetic is integer.  Every loop is bounded by n (list length) or by the
 // grid size; preference entries are range-checked before they index anything.
@@ -44,6 +45,7 @@ __global__ void k_reset_scalars(Scalars* sc, int active) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     sc->active = active;
     sc->parked_count = 0;
+    sc->prefix_overflow = 0;
     sc->error = 0;
     sc->dup_first = 0;
     sc->first_differs = 0;
@@ -89,26 +91,47 @@ __global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t
 // Rank inversion, one workgroup per row: rank[w][pref[w][r]] = r.
 // The row's inverse is assembled in LDS as 16-bit positions (n <= 65535, so a
 // position fits and 0xFFFF is free to mark an empty slot), `chunk` columns per
-// pass (one pass when the row fits: 40 KB at n = 20,000, so four 512-thread
-// workgroups share a CU).  Each thread first issues up to kBlk 16-byte loads of
-// the row and only then scatters them, so many loads are in flight per thread;
-// the read of pref and the write of rank are both coalesced 16-byte accesses and
-// only the LDS scatter is random.  With validate != 0 a slot left at 0xFFFF marks
-// a row that is not a permutation.
+// pass (one pass when the row fits: 40 KB at n = 20,000).  Each thread first
+// issues up to kBlk 16-byte loads of the row and only then scatters them, so
+// many loads are in flight per thread; the read of pref and the write of rank
+// are both coalesced 16-byte accesses and only the LDS scatter is random.  With
+// validate != 0 a slot left at 0xFFFF marks a row that is not a permutation.
+//
+// SP = 1 is the sparse prefix pass: instead of writing the row out, the
+// workgroup looks up only the men whose first K list positions name this woman
+// (her bucket, filled by k_bucket_prefix) and writes their PR prefix entries
+// pr[m, k] = (w, rank) straight into the column-blocked prefix.  The n x n rank
+// matrix is then never written (2n^2 bytes saved) and the separate gather is not
+// needed.  If some bucket overflowed (sc->prefix_overflow), every row is written
+// out in full as with SP = 0 and k_gather_pr (gated) builds the prefix instead.
 constexpr int kInvBlock = 1024;  // measured: 1024 > 512 ≈ 256 threads per row at n = 20,000
 constexpr int kBlk = 8;
 
-template <int VEC, int NT>
-__global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __restrict__ pref,
+struct PrefixBuckets {
+  const int* cnt;          // suitors per woman (may exceed cap: overflow)
+  const uint32_t* bucket;  // bucket[w * cap + j] = k << 16 | m
+  int cap;
+  uint32_t* pr;            // column-blocked PR prefix
+};
+
+// Two 1024-thread workgroups per CU (at most 64 VGPRs): with one, half the row
+// loads the CU could keep in flight are missing.
+template <int VEC, int NT, int SP>
+__global__ __launch_bounds__(kInvBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_invert_rows(const int32_t* __restrict__ pref,
                                                            uint16_t* __restrict__ rank, int n,
-                                                           int chunk, int validate, Scalars* sc) {
+                                                           int chunk, int validate, Scalars* sc,
+                                                           PrefixBuckets pb) {
   extern __shared__ uint16_t lds16[];
   uint32_t* lds32 = reinterpret_cast<uint32_t*>(lds16);
   const int tid = threadIdx.x;
   int bad = 0;
+  // Sparse pass: look up the bucket entries unless a bucket overflowed.
+  const bool lookup =
+      SP && __hip_atomic_load(&sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
   for (int row = blockIdx.x; row < n; row += gridDim.x) {
     const int32_t* src = pref + (size_t)row * n;
     uint16_t* dst = rank + (size_t)row * n;
+
     for (int c0 = 0; c0 < n; c0 += chunk) {
       const int clen = min(chunk, n - c0);
       if (validate)
@@ -166,7 +189,23 @@ __global__ __launch_bounds__(kInvBlock) void k_invert_rows(const int32_t* __rest
         }
       }
       __syncthreads();
-      if constexpr (VEC == 4) {
+      if (lookup) {
+        // Prefix entries of this woman's suitors whose man falls in this chunk.
+        const int cntw = min(pb.cnt[row], pb.cap);
+        const uint32_t* bk = pb.bucket + (size_t)row * pb.cap;
+        for (int j = tid; j < cntw; j += kInvBlock) {
+          const uint32_t e = bk[j];
+          const unsigned u = (e & 0xFFFFu) - (unsigned)c0;
+          if (u < (unsigned)clen)
+            pb.pr[pr_index((int)(e & 0xFFFFu), (int)(e >> 16), n)] = pr_pack((uint32_t)row, lds16[u]);
+        }
+        if (validate) {  // a hole in the LDS row marks a row that is not a permutation
+          for (int c = tid; c < (clen + 1) / 2; c += kInvBlock) {
+            const uint32_t x = lds32[c];
+            if ((x & 0xFFFFu) == 0xFFFFu || (2 * c + 1 < clen && (x >> 16) == 0xFFFFu)) bad |= 2;
+          }
+        }
+      } else if constexpr (VEC == 4) {
         // The LDS row is already the 16-bit output: 16-byte stores (8 ranks) when
         // n is a multiple of 8 (rows then start 16-byte aligned), 8-byte stores
         // otherwise (n is a multiple of 4 here).
@@ -219,7 +258,10 @@ constexpr int kGatherCols = 256;
 __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ pref_m,
                                                    const uint16_t* __restrict__ rank_w,
                                                    uint32_t* __restrict__ pr, int n, int K,
-                                                   int tiles, int per_xcd, Scalars* sc) {
+                                                   int tiles, int per_xcd, int gated, Scalars* sc) {
+  // gated: the fallback after a sparse pass, needed only if a bucket overflowed.
+  if (gated && __hip_atomic_load(&sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    return;
   const int b = blockIdx.x;
   const int t = (b & 7) * per_xcd + (b >> 3);
   if (t >= tiles) return;
@@ -245,18 +287,104 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
   if (bad) atomicOr(&sc->error, kErrEntryRange);
 }
 
-// Entries [k0, k0 + count) of man m's PR row, for the host tail.
-__global__ void k_gather_row(const int32_t* __restrict__ pref_m, const uint16_t* __restrict__ rank_w,
-                             int n, int m, int k0, int count, uint32_t* out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  int k = k0 + i;
-  uint32_t e = pr_pack(0, 0xFFFFu);
-  if (k < n) {
-    int w = pref_m[(size_t)m * n + k];
-    if ((unsigned)w < (unsigned)n) e = pr_pack((uint32_t)w, (uint32_t)rank_w[(size_t)w * n + m]);
+// Buckets of the sparse prefix pass: every entry (m, k < K) of the men's lists
+// goes to the bucket of woman w = pref_m[m][k] as k << 16 | m (both fit 16 bits:
+// n <= 65535).  A transpose of the n x K prefix, done as a two-level partition so
+// that global atomics are few:
+//   k_bin_prefix  each workgroup takes kBinEntries consecutive prefix entries
+//                 (coalesced along the men's rows), counts them per bin of 256
+//                 women in LDS, reserves room in each bin with one global
+//                 atomicAdd per bin, and writes (w << 32 | k << 16 | m) there;
+//   k_bin_sort    one workgroup per bin places its entries into the women's
+//                 buckets with LDS atomics and writes the per-woman counts.
+// The order inside a bucket varies from run to run, which changes nothing (each
+// entry names its own PR slot).  A bin or bucket that runs out of room sets
+// sc->prefix_overflow, and the pass falls back to the full rank matrix.
+constexpr int kBinWomen = 256;     // women per bin: bin = w >> 8 (at most 256 bins)
+constexpr int kBinEntries = 2048;  // prefix entries per k_bin_prefix workgroup (8 per thread)
+
+__global__ __launch_bounds__(256) void k_bin_prefix(const int32_t* __restrict__ pref_m, int n, int K, int nbins,
+                                                    int* __restrict__ bin_cnt, unsigned long long* __restrict__ bins,
+                                                    int bin_cap, Scalars* sc) {
+  __shared__ int hist[kBinWomen];
+  __shared__ int base[kBinWomen];
+  constexpr int kPer = kBinEntries / 256;
+  const int tid = threadIdx.x;
+  const unsigned long long total = (unsigned long long)n * (unsigned long long)K;  // <= 65535^2 < 2^32
+  int bad = 0, over = 0;
+  for (unsigned long long e0 = (unsigned long long)blockIdx.x * kBinEntries; e0 < total;
+       e0 += (unsigned long long)gridDim.x * kBinEntries) {
+    hist[tid] = 0;
+    __syncthreads();
+    uint32_t wv[kPer], mk[kPer];
+    int loc[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const unsigned long long e64 = e0 + (unsigned long long)(j * 256 + tid);
+      wv[j] = 0xFFFFFFFFu;
+      mk[j] = 0;
+      if (e64 < total) {
+        const unsigned e = (unsigned)e64;
+        const unsigned m = e / (unsigned)K, k = e - m * (unsigned)K;
+        const int w = pref_m[(size_t)m * n + k];
+        if ((unsigned)w < (unsigned)n) {
+          wv[j] = (uint32_t)w;
+          mk[j] = (k << 16) | m;
+        } else {
+          bad = 1;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) loc[j] = wv[j] != 0xFFFFFFFFu ? atomicAdd(&hist[wv[j] >> 8], 1) : 0;
+    __syncthreads();
+    if (tid < nbins && hist[tid] > 0) base[tid] = atomicAdd(&bin_cnt[tid], hist[tid]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (wv[j] == 0xFFFFFFFFu) continue;
+      const int bi = (int)(wv[j] >> 8);
+      const int pos = base[bi] + loc[j];
+      if (pos < bin_cap) bins[(size_t)bi * bin_cap + pos] = ((unsigned long long)wv[j] << 32) | mk[j];
+      else over = 1;
+    }
+    __syncthreads();  // hist and base are reused by the next chunk
   }
-  out[i] = e;
+  if (__ballot(bad) && (tid & 63) == 0) atomicOr(&sc->error, kErrEntryRange);
+  if (__ballot(over) && (tid & 63) == 0) atomicOr(&sc->prefix_overflow, 1);
+}
+
+__global__ __launch_bounds__(1024) void k_bin_sort(const unsigned long long* __restrict__ bins,
+                                                   const int* __restrict__ bin_cnt, int bin_cap, int n,
+                                                   int* __restrict__ cnt, uint32_t* __restrict__ bucket, int cap,
+                                                   Scalars* sc) {
+  __shared__ int c[kBinWomen];
+  const int tid = threadIdx.x, b = blockIdx.x;
+  if (tid < kBinWomen) c[tid] = 0;
+  __syncthreads();
+  const int total = min(bin_cnt[b], bin_cap);
+  const unsigned long long* src = bins + (size_t)b * bin_cap;
+  int over = 0;
+  constexpr int kLoads = 8;  // entries loaded per thread before any is placed
+  for (int i0 = tid; i0 < total; i0 += (int)blockDim.x * kLoads) {
+    unsigned long long v[kLoads];
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+      const int i = i0 + j * (int)blockDim.x;
+      v[j] = i < total ? src[i] : ~0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+      if (v[j] == ~0ull) continue;
+      const uint32_t w = (uint32_t)(v[j] >> 32);
+      const int slot = atomicAdd(&c[w & (kBinWomen - 1)], 1);
+      if (slot < cap) bucket[(size_t)w * cap + slot] = (uint32_t)v[j];
+      else over = 1;
+    }
+  }
+  __syncthreads();
+  if (tid < kBinWomen && b * kBinWomen + tid < n) cnt[b * kBinWomen + tid] = c[tid];
+  if (__ballot(over) && (tid & 63) == 0) atomicOr(&sc->prefix_overflow, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -276,7 +404,9 @@ __global__ void k_gather_row(const int32_t* __restrict__ pref_m, const uint16_t*
 // when its chain ends and it has no unstarted man left.  With T > 0, a wave that
 // sees active <= T after a step parks its chain (man, next position) and its
 // unstarted men for the host, and exits.  Near the end every unstarted man has
-// been started, so `active` equals the number of free men.
+// been started, so `active` equals the number of free men.  After a sparse prefix
+// pass (no rank matrix), a chain that reaches position K is parked at K in the
+// same list, and its wave starts its next man.
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   const int lane = threadIdx.x & 63;
   const int n = a.n, K = a.K, T = a.T;
@@ -294,6 +424,10 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   if (__hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
     next_init = total;  // init found malformed lists: do nothing
   }
+  // After a sparse prefix pass only positions < K have ranks (unless it fell back
+  // to the full rank matrix).
+  const bool have_rank =
+      !a.rank_sparse || __hip_atomic_load(&a.sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
   for (;;) {
     if (m < 0) {
       if (next_init >= total) {
@@ -325,7 +459,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       act = __hip_atomic_load(&a.sc->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ++n_steps;
     const int kk = k + lane;
-    bool valid = kk < n;
+    bool valid = kk < n && (have_rank || kk < K);
     uint32_t w = 0, r = 0;
     if (valid) {
       if (kk < K) {
@@ -383,7 +517,15 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       }
       cand &= cand - 1;
     }
-    if (!moved) k += 64;
+    if (!moved) {
+      k += 64;
+      if (!have_rank && k >= K) {
+        // Positions below K are settled; the rest needs the rank matrix, which the
+        // host builds before it resumes the chain at K.
+        if (lane == 0) a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(m, K);
+        m = -1;
+      }
+    }
     if (poll) {
       since_poll = 0;
       act = __shfl(act, 0);
@@ -819,22 +961,56 @@ hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int ch
                               Scalars* sc, hipStream_t s, int nontemporal) {
   const size_t lds = (size_t)((chunk + 7) & ~7) * sizeof(uint16_t);
   const bool vec = (n % 4 == 0) && (chunk % 8 == 0 || chunk == n);
+  const PrefixBuckets none{nullptr, nullptr, 0, nullptr};
   if (vec && nontemporal)
-    hipLaunchKernelGGL((k_invert_rows<4, 1>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
-                       validate, sc);  // streaming: rank rows are written once, read much later
+    hipLaunchKernelGGL((k_invert_rows<4, 1, 0>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
+                       validate, sc, none);  // streaming: rank rows are written once, read much later
   else if (vec)
-    hipLaunchKernelGGL((k_invert_rows<4, 0>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
-                       validate, sc);
+    hipLaunchKernelGGL((k_invert_rows<4, 0, 0>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
+                       validate, sc, none);
   else
-    hipLaunchKernelGGL((k_invert_rows<1, 0>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
-                       validate, sc);
+    hipLaunchKernelGGL((k_invert_rows<1, 0, 0>), dim3(n), dim3(kInvBlock), lds, s, pref, rank, n, chunk,
+                       validate, sc, none);
+  return hipGetLastError();
+}
+
+int bin_count(int n) { return (n + kBinWomen - 1) / kBinWomen; }
+
+hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
+                                int bin_cap, int* cnt, uint32_t* bucket, int cap, Scalars* sc, int cus,
+                                hipStream_t s) {
+  const int nbins = bin_count(n);
+  if (nbins > kBinWomen) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, bin_cnt, (size_t)nbins, 0);
+  const unsigned long long total = (unsigned long long)n * (unsigned long long)K;
+  const long long chunks = (long long)((total + kBinEntries - 1) / kBinEntries);
+  const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));
+  hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(256), 0, s, pref_m, n, K, nbins, bin_cnt, bins, bin_cap, sc);
+  hipLaunchKernelGGL(k_bin_sort, dim3(nbins), dim3(1024), 0, s, bins, bin_cnt, bin_cap, n, cnt, bucket, cap, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int chunk, int validate,
+                              const int* cnt, const uint32_t* bucket, int cap, uint32_t* pr, Scalars* sc,
+                              hipStream_t s) {
+  const size_t lds = (size_t)((chunk + 7) & ~7) * sizeof(uint16_t);
+  const bool vec = (n % 4 == 0) && (chunk % 8 == 0 || chunk == n);
+  const PrefixBuckets pb{cnt, bucket, cap, pr};
+  if (vec)
+    hipLaunchKernelGGL((k_invert_rows<4, 1, 1>), dim3(n), dim3(kInvBlock), lds, s, pref_w, rank, n, chunk,
+                       validate, sc, pb);
+  else
+    hipLaunchKernelGGL((k_invert_rows<1, 0, 1>), dim3(n), dim3(kInvBlock), lds, s, pref_w, rank, n, chunk,
+                       validate, sc, pb);
   return hipGetLastError();
 }
 
 hipError_t set_invert_lds_limit(int bytes) {
-  const void* fns[] = {reinterpret_cast<const void*>(&k_invert_rows<4, 1>),
-                       reinterpret_cast<const void*>(&k_invert_rows<4, 0>),
-                       reinterpret_cast<const void*>(&k_invert_rows<1, 0>)};
+  const void* fns[] = {reinterpret_cast<const void*>(&k_invert_rows<4, 1, 0>),
+                       reinterpret_cast<const void*>(&k_invert_rows<4, 0, 0>),
+                       reinterpret_cast<const void*>(&k_invert_rows<1, 0, 0>),
+                       reinterpret_cast<const void*>(&k_invert_rows<4, 1, 1>),
+                       reinterpret_cast<const void*>(&k_invert_rows<1, 0, 1>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
@@ -843,21 +1019,14 @@ hipError_t set_invert_lds_limit(int bytes) {
 }
 
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s) {
+                            int gated, Scalars* sc, hipStream_t s) {
   const int men_blocks = (n + kGatherMen - 1) / kGatherMen;
   const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
   const long long tiles = (long long)men_blocks * col_tiles;
   if (tiles > (1ll << 30)) return hipErrorInvalidValue;
   const int per_xcd = (int)((tiles + 7) / 8);
   hipLaunchKernelGGL(k_gather_pr, dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
-                     (int)tiles, per_xcd, sc);
-  return hipGetLastError();
-}
-
-hipError_t launch_gather_row(const int32_t* pref_m, const uint16_t* rank_w, int n, int m, int k0,
-                             int count, uint32_t* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_row, dim3((count + 255) / 256), dim3(256), 0, s, pref_m, rank_w, n, m,
-                     k0, count, out);
+                     (int)tiles, per_xcd, gated, sc);
   return hipGetLastError();
 }
 

@@ -20,6 +20,9 @@ struct ProposeArgs {
   int poll;                    // steps between reads of the shared `active` counter
   const int2* starts;          // resume mode: (man, position) pairs to start from;
   int nstarts;                 //   null = every man from position 0
+  int rank_sparse;             // 1: rank_w exists only if sc->prefix_overflow (sparse
+                               //   prefix pass); a chain reaching position K is then
+                               //   parked at K for the host, which builds rank_w
 };
 
 hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s);
@@ -42,10 +45,20 @@ hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipS
 hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int chunk, int validate,
                               Scalars* sc, hipStream_t s, int nontemporal);
 hipError_t set_invert_lds_limit(int bytes);
+// gated = 1: runs only if a sparse prefix pass overflowed (sc->prefix_overflow).
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s);
-hipError_t launch_gather_row(const int32_t* pref_m, const uint16_t* rank_w, int n, int m, int k0,
-                             int count, uint32_t* out, hipStream_t s);
+                            int gated, Scalars* sc, hipStream_t s);
+// Sparse prefix pass: buckets of (m, k < K) per woman (bins of 256 women with
+// bin_cap entries each, then per-woman buckets of cap entries and their counts),
+// then the PR prefix from the women's rows without the n x n rank matrix (or the
+// full rank matrix if a bin or bucket overflowed).
+int bin_count(int n);
+hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
+                                int bin_cap, int* cnt, uint32_t* bucket, int cap, Scalars* sc, int cus,
+                                hipStream_t s);
+hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int chunk, int validate,
+                              const int* cnt, const uint32_t* bucket, int cap, uint32_t* pr, Scalars* sc,
+                              hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);
 hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,

@@ -64,7 +64,7 @@ enum : int {
 // Small device-resident control block of one solve.
 struct Scalars {
   int active;        // proposal waves that still hold work (a chain or unstarted men)
-  int pad0;
+  int prefix_overflow;  // sparse prefix pass: a woman had more suitors than her bucket holds
   int parked_count;  // chains parked for the host tail
   int error;         // kErr* bits
   int dup_first;     // precheck: 1 if two men share a first choice

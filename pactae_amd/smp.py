@@ -197,6 +197,16 @@ class Schedule(enum.IntEnum):
     MW_CAS = _abi.SMP_SCHEDULE_MW_CAS
 
 
+class InitMode(enum.IntEnum):
+    """enum smp_init_mode: how init builds the men's PR prefix.  FULL is the
+    reference's init (n x n rank matrix, then the gather, kernel.cu:672-709);
+    SPARSE looks up only the prefix pairs in the women's rows and builds the rank
+    matrix only when a chain needs it.  Results are identical in every mode."""
+    AUTO = _abi.SMP_INIT_AUTO
+    FULL = _abi.SMP_INIT_FULL
+    SPARSE = _abi.SMP_INIT_SPARSE
+
+
 class SmpEngineBamboo:
     """MI355X BambooSMP engine (SmpEngineBamboo, smp_engine_bamboo.h:13-119).
 

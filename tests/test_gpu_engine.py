@@ -24,6 +24,17 @@ TAIL_CONFIGS = [
 ]
 
 
+# Init modes: 1 = full rank matrix then gather; 2 = sparse prefix pass (rank
+# matrix only if a chain walks past the prefix or a bucket overflows).
+INIT_CONFIGS = [
+    dict(init_mode=1),
+    dict(init_mode=2),
+    dict(init_mode=2, prefix_cols=8, tail_mode=1),  # most chains walk past the prefix
+    dict(init_mode=2, prefix_cols=8, tail_mode=2, tail_threshold=16),
+    dict(init_mode=2, prefix_cols=100000),          # K = n: every man is in every bucket
+]
+
+
 def solve(smp, n, pm, pw, **opts):
     obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
     with smp.SmpEngineBamboo(obj, n, **opts) as e:
@@ -55,6 +66,69 @@ def test_parity_mid_sizes(gpu, oracle, kind, cfg):
         got, prof = solve(smp, n, pm, pw, **cfg)
         assert np.array_equal(got, want), (kind, n, cfg)
         assert prof["proposals"] == t["proposals"]
+
+
+@pytest.mark.parametrize("cfg", INIT_CONFIGS)
+@pytest.mark.parametrize("kind", range(8))
+def test_init_modes(gpu, oracle, kind, cfg):
+    """Both init modes, including chains that walk past a sparse prefix (the rank
+    matrix is then built before they resume), give the oracle's matching."""
+    smp = gpu
+    for n in (1, 2, 3, 5, 63, 64, 65, 257, 1000):
+        if kind == 7 and n < 4:
+            continue
+        pm, pw = smp.generate(kind, n, seed=5000 + n)
+        want, t = oracle.gs_seq(n, pm, pw)
+        got, prof = solve(smp, n, pm, pw, **cfg)
+        assert np.array_equal(got, want), (kind, n, cfg)
+        assert prof["proposals"] == t["proposals"]
+        if prof["is_perfect"] or prof["master_list"]:
+            continue
+        if cfg["init_mode"] == 1:
+            assert prof["sparse_init"] == 0 and prof["rank_matrix"] == 1
+        else:
+            assert prof["sparse_init"] == 1
+
+
+@pytest.mark.parametrize("n", [7, 64, 333, 1024, 4001])
+def test_sparse_prefix_matrices(gpu, n):
+    """Kernel-level parity of the sparse prefix pass: the PR prefix it writes
+    against a numpy gather, and the rank matrix it builds on demand against
+    numpy argsort."""
+    smp = gpu
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=77 + n)
+    M, W = pm.reshape(n, n), pw.reshape(n, n)
+    rank_w = np.argsort(W, axis=1)
+    for cols in (1, 64, n):
+        obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+        with smp.SmpEngineBamboo(obj, n, prefix_cols=cols, init_mode=2, precheck=0, tail_mode=1) as e:
+            e.find_stable_matching()
+            prof = e.profile()
+            K = min(cols, n)
+            # Sparse buffers cover prefixes up to 256 columns (any width for
+            # n <= 2048); a wider prefix takes the full init.
+            assert prof["sparse_init"] == (1 if K <= 256 or n <= 2048 else 0)
+            pr = e.pr_matrix()
+            assert pr.shape == (n, K, 2)
+            assert np.array_equal(pr[:, :, 0], M[:, :K])
+            assert np.array_equal(pr[:, :, 1], rank_w[M[:, :K], np.arange(n)[:, None]])
+            assert np.array_equal(e.rank_matrix(), rank_w)
+
+
+def test_sparse_overflow_falls_back(gpu, oracle):
+    """All men share one list and the precheck is off: each of the first K women
+    of that list is named by all n men, far more than a bucket holds, so the
+    sparse pass writes the full rank matrix and the gated gather builds the
+    prefix.  Result: the serial-dictatorship matching with n(n+1)/2 proposals."""
+    smp = gpu
+    n = 1000
+    pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=31)
+    want = oracle.serial_dictatorship(n, pm[:n], pw)
+    for cfg in (dict(tail_mode=1), dict(tail_mode=2, tail_threshold=16)):
+        got, prof = solve(smp, n, pm, pw, init_mode=2, prefix_cols=16, precheck=0, **cfg)
+        assert prof["sparse_init"] == 1 and prof["rank_matrix"] == 1
+        assert np.array_equal(got, want), cfg
+        assert prof["proposals"] == n * (n + 1) // 2
 
 
 @pytest.mark.parametrize("kind", range(8))
@@ -102,7 +176,8 @@ def test_config2_uniform_20000(gpu, oracle):
     pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=0x5EED0002)
     want, t = oracle.gs_seq(n, pm, pw)
     for cfg in (dict(tail_mode=1), dict(tail_mode=2, tail_threshold=16), dict(tail_mode=0),
-                dict(tail_mode=2, tail_threshold=256)):
+                dict(tail_mode=2, tail_threshold=256), dict(init_mode=1),
+                dict(init_mode=2, prefix_cols=32)):
         got, prof = solve(smp, n, pm, pw, **cfg)
         assert np.array_equal(got, want), cfg
         assert prof["proposals"] == t["proposals"]
