@@ -86,7 +86,7 @@ def single_instance(smp, args):
     tune = None
     if not args.no_tune and not opts:
         t0 = time.perf_counter()
-        tune = eng.tune(3)
+        tune = eng.tune(5)
         log(f"tuned hand-off threshold / prefix on this instance in {time.perf_counter() - t0:.2f}s: {tune}")
         for _ in range(args.warmup):
             eng.exec()

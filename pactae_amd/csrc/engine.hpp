@@ -287,7 +287,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (hipMalloc(&d_bucket, (size_t)n * bucket_cap(n, K_sparse_alloc) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_bins, (size_t)pactae::bin_count(n) * bin_cap(K_sparse_alloc) * sizeof(unsigned long long)) !=
             hipSuccess ||
-        hipMalloc(&d_bin_cnt, 256 * sizeof(int)) != hipSuccess) {
+        hipMalloc(&d_bin_cnt, 256 * pactae::kBinCounterStride * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("allocation of the prefix buckets failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
@@ -356,52 +356,80 @@ struct smp_engine : public pactae::AbsSmpEngine {
   }
 
   // Measured choice of the hand-off threshold T and the prefix width K: every
-  // candidate pair is run `trials` times on this instance and the lowest median
-  // wall time wins.  AUTO mode then uses it.  Results are identical for every
-  // pair (the matching is unique); only the time differs.
+  // candidate pair is run `trials` times on this instance and ranked by its
+  // median wall time; the four best, and the untuned defaults, are then run
+  // 3 * trials times more and the lowest median of those wins (one noisy median
+  // among ~33 candidates must not decide).  AUTO mode then uses it.  Results
+  // are identical for every pair (the matching is unique); only the time differs.
   int tune(int trials, smp_tune_result* out) {
-    if (trials <= 0) trials = 3;
+    if (trials <= 0) trials = 5;
     const int Ks[] = {64, 128, 256};
     const int Ts[] = {0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 1024};
     const smp_opts saved = opts;
-    double best = 1e300, gpu_only = 1e300;
-    int bK = prefix_for(n), bT = 0, cands = 0;
-    std::vector<double> w(trials);
+    struct Cand {
+      int k, t;
+      double med;
+    };
+    std::vector<Cand> cands;
+    double gpu_only = 1e300;
+    // Median of `runs` solves with prefix k and threshold t.
+    auto measure = [&](int k, int t, int runs, double* med) -> int {
+      opts.prefix_cols = k;
+      opts.tail_mode = t == 0 ? SMP_TAIL_GPU : SMP_TAIL_HOST;
+      opts.tail_threshold = t;
+      configure(n);
+      std::vector<double> w(runs);
+      for (int i = 0; i < runs; ++i) {
+        auto t0 = Clock::now();
+        if (int st = exec()) return st;
+        w[i] = ms_since(t0);
+      }
+      std::sort(w.begin(), w.end());
+      *med = w[runs / 2];
+      return SMP_OK;
+    };
+    auto fail = [&](int st) {
+      opts = saved;
+      configure(n);
+      return st;
+    };
     for (int k : Ks) {
       if (k > K_alloc) continue;
       for (int t : Ts) {
         if (t > n) continue;
-        opts.prefix_cols = k;
-        opts.tail_mode = t == 0 ? SMP_TAIL_GPU : SMP_TAIL_HOST;
-        opts.tail_threshold = t;
-        configure(n);
-        for (int i = 0; i < trials; ++i) {
-          auto t0 = Clock::now();
-          if (int st = exec()) {
-            opts = saved;
-            configure(n);
-            return st;
-          }
-          w[i] = ms_since(t0);
-        }
-        std::sort(w.begin(), w.end());
-        const double med = w[trials / 2];
-        ++cands;
+        double med = 0;
+        if (int st = measure(k, t, trials, &med)) return fail(st);
+        cands.push_back({k, t, med});
         if (t == 0) gpu_only = std::min(gpu_only, med);
-        if (med < best) best = med, bK = k, bT = t;
       }
     }
+    std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.med < y.med; });
+    std::vector<Cand> finals(cands.begin(), cands.begin() + std::min<size_t>(4, cands.size()));
+    {
+      opts = saved;
+      opts.tail_mode = SMP_TAIL_AUTO;
+      opts.prefix_cols = opts.tail_threshold = 0;
+      const int def_k = prefix_for(n), def_t = std::min(n, 32);
+      bool have = false;
+      for (const Cand& c : finals) have = have || (c.k == def_k && c.t == def_t);
+      if (!have) finals.push_back({def_k, def_t, 0});
+    }
+    Cand best{prefix_for(n), 0, 1e300};
+    for (Cand& c : finals) {
+      if (int st = measure(c.k, c.t, 3 * trials, &c.med)) return fail(st);
+      if (c.med < best.med) best = c;
+    }
     opts = saved;
-    tuned_K = bK;
-    tuned_T = bT;
+    tuned_K = best.k;
+    tuned_T = best.t;
     configure(n);
     if (out) {
       *out = smp_tune_result{};
-      out->prefix_cols = bK;
-      out->tail_threshold = bT;
-      out->best_ms = best;
+      out->prefix_cols = best.k;
+      out->tail_threshold = best.t;
+      out->best_ms = best.med;
       out->gpu_only_ms = gpu_only;
-      out->candidates = cands;
+      out->candidates = (int)cands.size();
     }
     return SMP_OK;
   }

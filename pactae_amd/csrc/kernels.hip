@@ -300,27 +300,31 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
 // The order inside a bucket varies from run to run, which changes nothing (each
 // entry names its own PR slot).  A bin or bucket that runs out of room sets
 // sc->prefix_overflow, and the pass falls back to the full rank matrix.
-constexpr int kBinWomen = 256;     // women per bin: bin = w >> 8 (at most 256 bins)
-constexpr int kBinEntries = 2048;  // prefix entries per k_bin_prefix workgroup (8 per thread)
+constexpr int kBinWomen = 256;      // women per bin: bin = w >> 8 (at most 256 bins)
+constexpr int kBinThreads = 1024;   // k_bin_prefix workgroup
+constexpr int kBinEntries = 8192;   // prefix entries per k_bin_prefix workgroup (8 per thread)
+// Bin counters sit one 128-byte line apart: every k_bin_prefix workgroup adds to
+// each of them once, and same-line atomics would serialize.
+constexpr int kBinPad = kBinCounterStride;
 
-__global__ __launch_bounds__(256) void k_bin_prefix(const int32_t* __restrict__ pref_m, int n, int K, int nbins,
+__global__ __launch_bounds__(kBinThreads) void k_bin_prefix(const int32_t* __restrict__ pref_m, int n, int K, int nbins,
                                                     int* __restrict__ bin_cnt, unsigned long long* __restrict__ bins,
                                                     int bin_cap, Scalars* sc) {
   __shared__ int hist[kBinWomen];
   __shared__ int base[kBinWomen];
-  constexpr int kPer = kBinEntries / 256;
+  constexpr int kPer = kBinEntries / kBinThreads;
   const int tid = threadIdx.x;
   const unsigned long long total = (unsigned long long)n * (unsigned long long)K;  // <= 65535^2 < 2^32
   int bad = 0, over = 0;
   for (unsigned long long e0 = (unsigned long long)blockIdx.x * kBinEntries; e0 < total;
        e0 += (unsigned long long)gridDim.x * kBinEntries) {
-    hist[tid] = 0;
+    if (tid < kBinWomen) hist[tid] = 0;
     __syncthreads();
     uint32_t wv[kPer], mk[kPer];
     int loc[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-      const unsigned long long e64 = e0 + (unsigned long long)(j * 256 + tid);
+      const unsigned long long e64 = e0 + (unsigned long long)(j * kBinThreads + tid);
       wv[j] = 0xFFFFFFFFu;
       mk[j] = 0;
       if (e64 < total) {
@@ -338,7 +342,7 @@ __global__ __launch_bounds__(256) void k_bin_prefix(const int32_t* __restrict__ 
 #pragma unroll
     for (int j = 0; j < kPer; ++j) loc[j] = wv[j] != 0xFFFFFFFFu ? atomicAdd(&hist[wv[j] >> 8], 1) : 0;
     __syncthreads();
-    if (tid < nbins && hist[tid] > 0) base[tid] = atomicAdd(&bin_cnt[tid], hist[tid]);
+    if (tid < nbins && hist[tid] > 0) base[tid] = atomicAdd(&bin_cnt[tid * kBinPad], hist[tid]);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
@@ -362,7 +366,7 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const unsigned long long* __r
   const int tid = threadIdx.x, b = blockIdx.x;
   if (tid < kBinWomen) c[tid] = 0;
   __syncthreads();
-  const int total = min(bin_cnt[b], bin_cap);
+  const int total = min(bin_cnt[b * kBinPad], bin_cap);
   const unsigned long long* src = bins + (size_t)b * bin_cap;
   int over = 0;
   constexpr int kLoads = 8;  // entries loaded per thread before any is placed
@@ -981,11 +985,12 @@ hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cn
                                 hipStream_t s) {
   const int nbins = bin_count(n);
   if (nbins > kBinWomen) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, bin_cnt, (size_t)nbins, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, bin_cnt, (size_t)nbins * kBinPad, 0);
   const unsigned long long total = (unsigned long long)n * (unsigned long long)K;
   const long long chunks = (long long)((total + kBinEntries - 1) / kBinEntries);
   const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));
-  hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(256), 0, s, pref_m, n, K, nbins, bin_cnt, bins, bin_cap, sc);
+  hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(kBinThreads), 0, s, pref_m, n, K, nbins, bin_cnt, bins, bin_cap,
+                     sc);
   hipLaunchKernelGGL(k_bin_sort, dim3(nbins), dim3(1024), 0, s, bins, bin_cnt, bin_cap, n, cnt, bucket, cap, sc);
   return hipGetLastError();
 }

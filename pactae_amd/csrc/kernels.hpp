@@ -49,10 +49,12 @@ hipError_t set_invert_lds_limit(int bytes);
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
                             int gated, Scalars* sc, hipStream_t s);
 // Sparse prefix pass: buckets of (m, k < K) per woman (bins of 256 women with
-// bin_cap entries each, then per-woman buckets of cap entries and their counts),
+// bin_cap entries each and counters kBinCounterStride ints apart in bin_cnt,
+// then per-woman buckets of cap entries and their counts),
 // then the PR prefix from the women's rows without the n x n rank matrix (or the
 // full rank matrix if a bin or bucket overflowed).
 int bin_count(int n);
+constexpr int kBinCounterStride = 32;  // bin_cnt holds 256 * kBinCounterStride ints
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
                                 int bin_cap, int* cnt, uint32_t* bucket, int cap, Scalars* sc, int cus,
                                 hipStream_t s);
