@@ -139,6 +139,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   unsigned long long* d_bins = nullptr;  // sparse prefix pass: bins of 256 women
   int* d_bin_cnt = nullptr;      // entries per bin
   int K_sparse_alloc = 0;        // widest prefix the sparse buffers hold
+  bool sparse_overflowed = false;  // a sparse pass on these lists overflowed a bin or bucket
   bool sparse = false;           // this run's init took the sparse prefix pass
   bool rank_built = false;       // d_rank_w holds this run's women's rank matrix
 
@@ -304,9 +305,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // Entries per bin of 256 women: 256 k on average for uniform lists (exactly
   // that at most when k = n), plus 12.5% and 1024 of slack.
   static int bin_cap(int k) { return 256 * k + 32 * k + 1024; }
+  // AUTO also stops taking the sparse pass on lists where it has overflowed once
+  // (men whose prefixes name the same few women, e.g. correlated lists): its
+  // buckets would only be built to be thrown away.
   bool use_sparse() const {
     if (opts.init_mode == SMP_INIT_FULL || K > K_sparse_alloc) return false;
-    return opts.init_mode == SMP_INIT_SPARSE || 4ll * K <= n;
+    if (opts.init_mode == SMP_INIT_SPARSE) return true;
+    return 4ll * K <= n && !sparse_overflowed;
   }
   // The women's rank matrix, if this run has not built it (sparse prefix pass).
   int ensure_rank_matrix() {
@@ -441,6 +446,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (ni <= 0 || ni > n_alloc) return SMP_ERR_INVALID;
     d_pref_m = pm;
     d_pref_w = pw;
+    sparse_overflowed = false;  // other lists
     configure(ni);
     if (int st = exec()) return st;
     if (d_out) {
@@ -714,6 +720,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (sparse && h_sc->prefix_overflow != 0) {  // the sparse pass fell back to the full matrix
       rank_built = true;
       times.rank_matrix = 1;
+      sparse_overflowed = true;
     }
     const int handoff = h_sc->parked_count;
     times.handoff_free = handoff;

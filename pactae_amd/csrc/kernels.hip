@@ -12,9 +12,8 @@
 //                                     detection, women-proposing walk, matching
 //   k_mw_cas / k_post_cas             MW-Par-GPU-CAS baseline schedule
 //                                     src/smp/smp_engine_mw_3_f9.cu:37-97
-//   k_bucket_prefix + k_invert_rows<.., SP=1>
-//                                     sparse prefix pass: F9RankMatrixInitKernel and
-//                                     F9PRMatrixInitKernel restricted to the first K list
+//   k_bin_prefix, k_bin_sort,         sparse prefix pass: F9RankMatrixInitKernel and
+//   k_invert_rows<.., SP=1>           F9PRMatrixInitKernel restricted to the first K list
 //                                     positions, without writing the n x n rank matrix
 //
 // All arithmetic is integer.  Every loop is bounded by n (list length) or by the

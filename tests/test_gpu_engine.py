@@ -131,6 +131,26 @@ def test_sparse_overflow_falls_back(gpu, oracle):
         assert prof["proposals"] == n * (n + 1) // 2
 
 
+def test_sparse_overflow_is_remembered(gpu, oracle):
+    """Correlated lists: every man's prefix names the same few women, so the
+    sparse pass overflows.  In auto mode the engine then keeps the full init for
+    these lists; forced sparse mode keeps trying.  Every run is bit-exact."""
+    smp = gpu
+    n = 3000
+    pm, pw = smp.generate(smp.Workload.CORRELATED, n, seed=8)
+    want, t = oracle.gs_seq(n, pm, pw)
+    obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+    for mode, second_sparse in ((0, 0), (2, 1)):
+        with smp.SmpEngineBamboo(obj, n, init_mode=mode) as e:
+            assert np.array_equal(e.find_stable_matching(), want)
+            p1 = e.profile()
+            assert p1["sparse_init"] == 1 and p1["rank_matrix"] == 1
+            assert np.array_equal(e.find_stable_matching(), want)
+            p2 = e.profile()
+            assert p2["sparse_init"] == second_sparse and p2["rank_matrix"] == 1
+            assert p2["proposals"] == t["proposals"]
+
+
 @pytest.mark.parametrize("kind", range(8))
 def test_mw_cas_baseline_schedule(gpu, oracle, kind):
     """The reference's GPU baseline (one thread per man, 32-bit CAS) through the
