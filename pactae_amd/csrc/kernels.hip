@@ -103,8 +103,11 @@ __global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t
 // matrix is then never written (2n^2 bytes saved) and the separate gather is not
 // needed.  If some bucket overflowed (sc->prefix_overflow), every row is written
 // out in full as with SP = 0 and k_gather_pr (gated) builds the prefix instead.
-constexpr int kInvBlock = 1024;  // measured: 1024 > 512 ≈ 256 threads per row at n = 20,000
+// Measured at n = 20,000: 1024 > 512 ≈ 256 threads per row for the full
+// inversion; for the sparse pass 1024 x 8 loads ≈ 768 x 7 < 512 x 10 < 640 x 8.
+constexpr int kInvBlock = 1024;
 constexpr int kBlk = 8;
+
 
 struct PrefixBuckets {
   const int* cnt;          // suitors per woman (may exceed cap: overflow)
@@ -115,8 +118,8 @@ struct PrefixBuckets {
 
 // Two 1024-thread workgroups per CU (at most 64 VGPRs): with one, half the row
 // loads the CU could keep in flight are missing.
-template <int VEC, int NT, int SP>
-__global__ __launch_bounds__(kInvBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_invert_rows(const int32_t* __restrict__ pref,
+template <int VEC, int NT, int SP, int BS = kInvBlock, int KB = kBlk>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_invert_rows(const int32_t* __restrict__ pref,
                                                            uint16_t* __restrict__ rank, int n,
                                                            int chunk, int validate, Scalars* sc,
                                                            PrefixBuckets pb) {
@@ -134,52 +137,54 @@ __global__ __launch_bounds__(kInvBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
     for (int c0 = 0; c0 < n; c0 += chunk) {
       const int clen = min(chunk, n - c0);
       if (validate)
-        for (int c = tid; c < (clen + 1) / 2; c += kInvBlock) lds32[c] = 0xFFFFFFFFu;
+        for (int c = tid; c < (clen + 1) / 2; c += BS) lds32[c] = 0xFFFFFFFFu;
       __syncthreads();
       if constexpr (VEC == 4) {
         const int4* src4 = reinterpret_cast<const int4*>(src);
         const int nq = n >> 2;
-        for (int q0 = tid; q0 < nq; q0 += kInvBlock * kBlk) {
-          int4 v[kBlk];
+        for (int q0 = tid; q0 < nq; q0 += BS * KB) {
+          int4 v[KB];
+          // Loads past the row's end are clamped to its last vector (and not
+          // scattered), so every load is unconditional and all KB are issued
+          // before the first wait.
 #pragma unroll
-          for (int j = 0; j < kBlk; ++j) {
-            const int q = q0 + j * kInvBlock;
-            if (q < nq) {
-              if constexpr (NT) {
-                v[j].x = __builtin_nontemporal_load(&src4[q].x);
-                v[j].y = __builtin_nontemporal_load(&src4[q].y);
-                v[j].z = __builtin_nontemporal_load(&src4[q].z);
-                v[j].w = __builtin_nontemporal_load(&src4[q].w);
-              } else {
-                v[j] = src4[q];
-              }
+          for (int j = 0; j < KB; ++j) {
+            const int q = min(q0 + j * BS, nq - 1);
+            if constexpr (NT) {
+              v[j].x = __builtin_nontemporal_load(&src4[q].x);
+              v[j].y = __builtin_nontemporal_load(&src4[q].y);
+              v[j].z = __builtin_nontemporal_load(&src4[q].z);
+              v[j].w = __builtin_nontemporal_load(&src4[q].w);
+            } else {
+              v[j] = src4[q];
             }
           }
 #pragma unroll
-          for (int j = 0; j < kBlk; ++j) {
-            const int q = q0 + j * kInvBlock;
-            if (q >= nq) break;
-            const int r = q << 2;
-            const int e[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+          for (int j = 0; j < KB; ++j) {
+            const int q = q0 + j * BS;
+            if (q < nq) {
+              const int r = q << 2;
+              const int e[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const unsigned u = (unsigned)(e[t] - c0);
-              if (u < (unsigned)clen) lds16[u] = (uint16_t)(r + t);
-              else bad |= (unsigned)e[t] >= (unsigned)n;
+              for (int t = 0; t < 4; ++t) {
+                const unsigned u = (unsigned)(e[t] - c0);
+                if (u < (unsigned)clen) lds16[u] = (uint16_t)(r + t);
+                else bad |= (unsigned)e[t] >= (unsigned)n;
+              }
             }
           }
         }
       } else {
-        for (int r0 = tid; r0 < n; r0 += kInvBlock * kBlk) {
-          int v[kBlk];
+        for (int r0 = tid; r0 < n; r0 += BS * KB) {
+          int v[KB];
 #pragma unroll
-          for (int j = 0; j < kBlk; ++j) {
-            const int r = r0 + j * kInvBlock;
+          for (int j = 0; j < KB; ++j) {
+            const int r = r0 + j * BS;
             if (r < n) v[j] = src[r];
           }
 #pragma unroll
-          for (int j = 0; j < kBlk; ++j) {
-            const int r = r0 + j * kInvBlock;
+          for (int j = 0; j < KB; ++j) {
+            const int r = r0 + j * BS;
             if (r >= n) break;
             const unsigned u = (unsigned)(v[j] - c0);
             if (u < (unsigned)clen) lds16[u] = (uint16_t)r;
@@ -192,14 +197,14 @@ __global__ __launch_bounds__(kInvBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
         // Prefix entries of this woman's suitors whose man falls in this chunk.
         const int cntw = min(pb.cnt[row], pb.cap);
         const uint32_t* bk = pb.bucket + (size_t)row * pb.cap;
-        for (int j = tid; j < cntw; j += kInvBlock) {
+        for (int j = tid; j < cntw; j += BS) {
           const uint32_t e = bk[j];
           const unsigned u = (e & 0xFFFFu) - (unsigned)c0;
           if (u < (unsigned)clen)
             pb.pr[pr_index((int)(e & 0xFFFFu), (int)(e >> 16), n)] = pr_pack((uint32_t)row, lds16[u]);
         }
         if (validate) {  // a hole in the LDS row marks a row that is not a permutation
-          for (int c = tid; c < (clen + 1) / 2; c += kInvBlock) {
+          for (int c = tid; c < (clen + 1) / 2; c += BS) {
             const uint32_t x = lds32[c];
             if ((x & 0xFFFFu) == 0xFFFFu || (2 * c + 1 < clen && (x >> 16) == 0xFFFFu)) bad |= 2;
           }
@@ -214,7 +219,7 @@ __global__ __launch_bounds__(kInvBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
         if ((n & 7) == 0) {
           const v4u* l4 = reinterpret_cast<const v4u*>(lds16);
           v4u* dst8 = reinterpret_cast<v4u*>(dst + c0);
-          for (int q = tid; q < (clen >> 3); q += kInvBlock) {
+          for (int q = tid; q < (clen >> 3); q += BS) {
             const v4u p = l4[q];
             if (validate && (has_hole(p.x) || has_hole(p.y) || has_hole(p.z) || has_hole(p.w))) bad |= 2;
             if constexpr (NT) __builtin_nontemporal_store(p, dst8 + q);
@@ -223,7 +228,7 @@ __global__ __launch_bounds__(kInvBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
         } else {
           const v2u* l2 = reinterpret_cast<const v2u*>(lds16);
           v2u* dst4 = reinterpret_cast<v2u*>(dst + c0);
-          for (int q = tid; q < (clen >> 2); q += kInvBlock) {
+          for (int q = tid; q < (clen >> 2); q += BS) {
             const v2u p = l2[q];
             if (validate && (has_hole(p.x) || has_hole(p.y))) bad |= 2;
             if constexpr (NT) __builtin_nontemporal_store(p, dst4 + q);
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(kInvBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
           }
         }
       } else {
-        for (int c = tid; c < clen; c += kInvBlock) {
+        for (int c = tid; c < clen; c += BS) {
           const uint16_t v = lds16[c];
           if (validate && v == 0xFFFFu) bad |= 2;
           dst[c0 + c] = v;
