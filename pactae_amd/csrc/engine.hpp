@@ -138,6 +138,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   uint32_t* d_bucket = nullptr;  // sparse prefix pass: per-woman buckets of (m, k)
   unsigned long long* d_bins = nullptr;  // sparse prefix pass: bins of 256 women
   int* d_bin_cnt = nullptr;      // entries per bin
+  int* d_bucket_cnt = nullptr;   // entries per woman's bucket
+  bool bins_pending = false;     // the bucket kernels of this run are queued (after the precheck)
   int K_sparse_alloc = 0;        // widest prefix the sparse buffers hold
   bool sparse_overflowed = false;  // a sparse pass on these lists overflowed a bin or bucket
   bool sparse = false;           // this run's init took the sparse prefix pass
@@ -158,7 +160,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int blocks_copied = 0, blocks_ready = 0;
   int requeues = 0;  // copy-stream replacements made by check_copy_overlap
   bool state_copied = false;  // control block, partner words and parked list are on the host
-  hipEvent_t ev[8] = {};
+  // ev[0..5]: init/core kernel timing and the tail's state copy; ev[6], ev[7]:
+  // around the bucket kernels; ev[8]: the precheck's flags have reached the host.
+  hipEvent_t ev[9] = {};
 
   smp_times times{};
   volatile uint64_t touch_sink = 0;
@@ -202,6 +206,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     d_bins = nullptr;
     (void)hipFree(d_bin_cnt);
     d_bin_cnt = nullptr;
+    (void)hipFree(d_bucket_cnt);
+    d_bucket_cnt = nullptr;
     (void)hipFree(d_wstats);
     d_wstats = nullptr;
     (void)hipFree(d_starts);
@@ -288,7 +294,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (hipMalloc(&d_bucket, (size_t)n * bucket_cap(n, K_sparse_alloc) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_bins, (size_t)pactae::bin_count(n) * bin_cap(K_sparse_alloc) * sizeof(unsigned long long)) !=
             hipSuccess ||
-        hipMalloc(&d_bin_cnt, 256 * pactae::kBinCounterStride * sizeof(int)) != hipSuccess) {
+        hipMalloc(&d_bin_cnt, 256 * pactae::kBinCounterStride * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_bucket_cnt, (size_t)n * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("allocation of the prefix buckets failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
@@ -312,6 +319,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (opts.init_mode == SMP_INIT_FULL || K > K_sparse_alloc) return false;
     if (opts.init_mode == SMP_INIT_SPARSE) return true;
     return 4ll * K <= n && !sparse_overflowed;
+  }
+  // Queues the sparse pass's bucket kernels on the engine's stream, timed by
+  // ev[6] and ev[7].
+  int launch_buckets(int gated) {
+    SMP_HIP_TRY(hipEventRecord(ev[6], stream));
+    SMP_HIP_TRY(pactae::launch_bucket_prefix(d_pref_m, n, K, d_bin_cnt, d_bins, bin_cap(K), d_bucket_cnt, d_bucket,
+                                             bucket_cap(n, K), gated, d_sc, dev.cus, stream));
+    SMP_HIP_TRY(hipEventRecord(ev[7], stream));
+    bins_pending = true;
+    return SMP_OK;
   }
   // The women's rank matrix, if this run has not built it (sparse prefix pass).
   int ensure_rank_matrix() {
@@ -497,6 +514,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     matrices_valid = false;
     sparse = false;
     rank_built = false;
+    bins_pending = false;
     times.schedule = opts.schedule;
     SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
     if (!opts.precheck || opts.schedule == SMP_SCHEDULE_MW_CAS) {
@@ -505,13 +523,21 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     SMP_HIP_TRY(pactae::launch_fill_i32(d_seen, n, 0, stream));
     SMP_HIP_TRY(pactae::launch_first_choices(d_pref_m, n, d_seen, d_sc, stream));
-    if (int st = read_scalars()) return st;
+    SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
+    SMP_HIP_TRY(hipEventRecord(ev[8], stream));
+    if (opts.schedule == SMP_SCHEDULE_BAMBOO && use_sparse()) {
+      // The sparse pass's bucket kernels go right behind the precheck, gated on
+      // its flags on the device, so they run while the host reads the flags.
+      if (int st = launch_buckets(/*gated=*/1)) return st;
+    }
+    SMP_HIP_TRY(wait_event(ev[8]));
     if (int st = check_device_error("precheck")) return st;
     *perfect = h_sc->dup_first == 0;
     times.is_perfect = *perfect ? 1 : 0;
     // Every man names the same first choice: is every list the same permutation?
     master = false;
     if (!*perfect && n > 1 && h_sc->first_differs == 0) {
+      bins_pending = false;  // the gated bucket kernels found the path closed and did nothing
       SMP_HIP_TRY(pactae::launch_rows_equal(d_pref_m, n, d_sc, dev.cus, stream));
       if (int st = read_scalars()) return st;
       master = h_sc->not_master == 0;
@@ -553,17 +579,18 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.sparse_init = sparse ? 1 : 0;
     SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
     if (sparse) {
-      // Buckets of (man, position) per woman over the first K positions (counts
-      // in d_seen, free after the precheck), then one pass over the women's rows
-      // that writes the PR prefix directly.  If a bucket overflowed, that pass
-      // writes the full rank matrix instead and the gated gather builds the
-      // prefix from it.
+      // Buckets of (man, position) per woman over the first K positions (queued
+      // by the precheck unless it was off or saw all first choices equal), then
+      // one pass over the women's rows that writes the PR prefix directly.  If a
+      // bucket overflowed, that pass writes the full rank matrix instead and the
+      // gated gather builds the prefix from it.
       const int cap = bucket_cap(n, K);
-      SMP_HIP_TRY(hipEventRecord(ev[6], stream));
-      SMP_HIP_TRY(pactae::launch_bucket_prefix(d_pref_m, n, K, d_bin_cnt, d_bins, bin_cap(K), d_seen, d_bucket, cap,
-                                               d_sc, dev.cus, stream));
+      if (!bins_pending) {
+        if (int st = launch_buckets(/*gated=*/0)) return st;
+      }
+      bins_pending = false;
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
-      SMP_HIP_TRY(pactae::launch_rank_prefix(d_pref_w, d_rank_w, n, chunk, opts.validate, d_seen, d_bucket, cap,
+      SMP_HIP_TRY(pactae::launch_rank_prefix(d_pref_w, d_rank_w, n, chunk, opts.validate, d_bucket_cnt, d_bucket, cap,
                                              d_pr, d_sc, stream));
       SMP_HIP_TRY(hipEventRecord(ev[1], stream));
       SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, /*gated=*/1, d_sc, stream));
@@ -1003,7 +1030,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (!times.is_perfect) {
       float ms = 0;
       if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) times.rank_kernel_ms = ms;
-      if (sparse && hipEventElapsedTime(&ms, ev[6], ev[0]) == hipSuccess) times.bucket_kernel_ms = ms;
+      if (sparse && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess) times.bucket_kernel_ms = ms;
       if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) times.gather_kernel_ms = ms;
       if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess) times.propose_kernel_ms = ms;
     }

@@ -311,9 +311,18 @@ constexpr int kBinEntries = 8192;   // prefix entries per k_bin_prefix workgroup
 // each of them once, and same-line atomics would serialize.
 constexpr int kBinPad = kBinCounterStride;
 
+// gated: queued right behind the precheck, before the host has read it; the
+// kernels do nothing unless the precheck left the general path open (two men
+// share a first choice, and not all first choices are the same).
+__device__ inline bool general_path_open(const Scalars* sc) {
+  return __hip_atomic_load(&sc->dup_first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 &&
+         __hip_atomic_load(&sc->first_differs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 __global__ __launch_bounds__(kBinThreads) void k_bin_prefix(const int32_t* __restrict__ pref_m, int n, int K, int nbins,
                                                     int* __restrict__ bin_cnt, unsigned long long* __restrict__ bins,
-                                                    int bin_cap, Scalars* sc) {
+                                                    int bin_cap, int gated, Scalars* sc) {
+  if (gated && !general_path_open(sc)) return;
   __shared__ int hist[kBinWomen];
   __shared__ int base[kBinWomen];
   constexpr int kPer = kBinEntries / kBinThreads;
@@ -365,7 +374,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_prefix(const int32_t* __res
 __global__ __launch_bounds__(1024) void k_bin_sort(const unsigned long long* __restrict__ bins,
                                                    const int* __restrict__ bin_cnt, int bin_cap, int n,
                                                    int* __restrict__ cnt, uint32_t* __restrict__ bucket, int cap,
-                                                   Scalars* sc) {
+                                                   int gated, Scalars* sc) {
+  if (gated && !general_path_open(sc)) return;
   __shared__ int c[kBinWomen];
   const int tid = threadIdx.x, b = blockIdx.x;
   if (tid < kBinWomen) c[tid] = 0;
@@ -985,7 +995,7 @@ hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int ch
 int bin_count(int n) { return (n + kBinWomen - 1) / kBinWomen; }
 
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
-                                int bin_cap, int* cnt, uint32_t* bucket, int cap, Scalars* sc, int cus,
+                                int bin_cap, int* cnt, uint32_t* bucket, int cap, int gated, Scalars* sc, int cus,
                                 hipStream_t s) {
   const int nbins = bin_count(n);
   if (nbins > kBinWomen) return hipErrorInvalidValue;
@@ -994,8 +1004,9 @@ hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cn
   const long long chunks = (long long)((total + kBinEntries - 1) / kBinEntries);
   const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));
   hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(kBinThreads), 0, s, pref_m, n, K, nbins, bin_cnt, bins, bin_cap,
+                     gated, sc);
+  hipLaunchKernelGGL(k_bin_sort, dim3(nbins), dim3(1024), 0, s, bins, bin_cnt, bin_cap, n, cnt, bucket, cap, gated,
                      sc);
-  hipLaunchKernelGGL(k_bin_sort, dim3(nbins), dim3(1024), 0, s, bins, bin_cnt, bin_cap, n, cnt, bucket, cap, sc);
   return hipGetLastError();
 }
 

@@ -55,8 +55,10 @@ hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint3
 // full rank matrix if a bin or bucket overflowed).
 int bin_count(int n);
 constexpr int kBinCounterStride = 32;  // bin_cnt holds 256 * kBinCounterStride ints
+// gated = 1: the kernels do nothing unless the precheck (k_first_choices, queued
+// before them) found two men sharing a first choice and not all first choices equal.
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
-                                int bin_cap, int* cnt, uint32_t* bucket, int cap, Scalars* sc, int cus,
+                                int bin_cap, int* cnt, uint32_t* bucket, int cap, int gated, Scalars* sc, int cus,
                                 hipStream_t s);
 hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int chunk, int validate,
                               const int* cnt, const uint32_t* bucket, int cap, uint32_t* pr, Scalars* sc,
