@@ -142,8 +142,7 @@ int smp_result_device(smp_engine* e, const int32_t** d_match) {
     pactae::set_error("smp_result_device before a successful smp_exec");
     return SMP_ERR_STATE;
   }
-  *d_match = e->d_match;
-  return SMP_OK;
+  return e->device_match(d_match);
 }
 
 int smp_profile(const smp_engine* e, smp_times* t) {

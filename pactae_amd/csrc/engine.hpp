@@ -150,6 +150,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
   unsigned long long* h_partner = nullptr;
   Scalars* h_sc = nullptr;
   int2* h_parked = nullptr;
+  // The same three buffers as seen from the device (k_export_state writes them).
+  unsigned long long* h_partner_dev = nullptr;
+  Scalars* h_sc_dev = nullptr;
+  int2* h_parked_dev = nullptr;
+  bool match_on_host_only = false;  // the host tail built the matching; d_match not yet mirrored
   int32_t* h_match = nullptr;  // pinned result (man -> woman)
   int2* d_starts = nullptr;     // chains handed back by the host tail
   bool host_finished = false;   // the host tail completed every chain
@@ -277,7 +282,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
-    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h_pr_dev), h_pr, 0) != hipSuccess) {
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h_pr_dev), h_pr, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_partner_dev), h_partner, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_sc_dev), h_sc, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_parked_dev), h_parked, 0) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("pinned PR prefix buffer is not visible to the device");
       return SMP_ERR_HIP;
@@ -319,6 +327,17 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (opts.init_mode == SMP_INIT_FULL || K > K_sparse_alloc) return false;
     if (opts.init_mode == SMP_INIT_SPARSE) return true;
     return 4ll * K <= n && !sparse_overflowed;
+  }
+  // The device result buffer, mirrored from the host's matching if the host tail
+  // built it.
+  int device_match(const int32_t** out) {
+    if (match_on_host_only) {
+      SMP_HIP_TRY(hipMemcpyAsync(d_match, h_match, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+      SMP_HIP_TRY(wait_stream(stream));
+      match_on_host_only = false;
+    }
+    *out = d_match;
+    return SMP_OK;
   }
   // Queues the sparse pass's bucket kernels on the engine's stream, timed by
   // ev[6] and ev[7].
@@ -467,8 +486,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
     configure(ni);
     if (int st = exec()) return st;
     if (d_out) {
-      SMP_HIP_TRY(hipMemcpyAsync(d_out, d_match, (size_t)ni * sizeof(int32_t),
-                                 hipMemcpyDeviceToDevice, stream));
+      const int32_t* dm = nullptr;
+      if (int st = device_match(&dm)) return st;
+      SMP_HIP_TRY(hipMemcpyAsync(d_out, dm, (size_t)ni * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
       SMP_HIP_TRY(wait_stream(stream));
     }
     return SMP_OK;
@@ -673,17 +693,15 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipEventRecord(ev[3], stream));
     SMP_HIP_TRY(pactae::launch_propose(propose_args(T, nullptr, 0), grid, stream));
     SMP_HIP_TRY(hipEventRecord(ev[4], stream));
-    SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
     state_copied = false;
+    if (T == 0) SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
     if (T > 0 && opts.prefix_copy >= 1) {
       // Everything the host tail needs follows the kernel on the engine's own
       // stream: first the small state (control block, partner words, parked
-      // chains), then the PR prefix block by block, each with an event.
-      SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
-      SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, stream));
-      SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost,
-                                 stream));
+      // chains, written to host memory by k_export_state), then the PR prefix
+      // block by block, each with an event.
+      SMP_HIP_TRY(pactae::launch_export_state(d_sc, d_wstats, grid * 4, d_partner, d_parked, n, h_sc_dev,
+                                              h_partner_dev, h_parked_dev, stream));
       SMP_HIP_TRY(hipEventRecord(ev[5], stream));
       const size_t blk = (size_t)n * pactae::kPrBlock;
       const int nb = (K + pactae::kPrBlock - 1) / pactae::kPrBlock;
@@ -713,12 +731,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
       }
     } else if (T > 0) {
       // The state the host tail reads (control block, partner words, parked
-      // chains) follows the kernel on its own stream, so that one wait covers all
-      // three copies.
-      SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
-      SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, stream));
-      SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost, stream));
+      // chains) is written to host memory by k_export_state right behind the
+      // kernel on its own stream, so that one wait covers all of it.
+      SMP_HIP_TRY(pactae::launch_export_state(d_sc, d_wstats, grid * 4, d_partner, d_parked, n, h_sc_dev,
+                                              h_partner_dev, h_parked_dev, stream));
       SMP_HIP_TRY(hipEventRecord(ev[5], stream));
       // While the kernel runs: pull the PR prefix blocks into the host caches as
       // they land (a DMA-written buffer starts out in DRAM only).  Block 0 always;
@@ -910,12 +926,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   // -- post (smp_engine_bamboo.cu:289-306) ----------------------------------
   int PostProc(bool perfect) override {
+    match_on_host_only = false;
     if (host_finished) {
-      // The host holds the matching; mirror it to the device result buffer (on
-      // the copy stream: it needs nothing still queued on the engine's stream).
-      SMP_HIP_TRY(hipMemcpyAsync(d_match, h_match, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
-                                 copy_stream));
-      SMP_HIP_TRY(wait_stream(copy_stream));
+      // The host holds the matching (the result the reference returns); the
+      // device copy is made only if a caller asks for it (device_match()).
+      match_on_host_only = true;
       times.proposals = host_proposals_total;
       times.atomics = (int64_t)h_sc->atomics;
       times.steps = (int64_t)h_sc->steps;

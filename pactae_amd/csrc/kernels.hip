@@ -585,6 +585,41 @@ __global__ __launch_bounds__(256) void k_fold_wstats(const unsigned long long* _
   }
 }
 
+// Hand-off export after the proposal kernel: folds the per-wave counters into
+// the control block (block 0), then writes the control block, the partner words
+// and the parked chains straight into pinned host memory (device-visible
+// pointers of hipHostMalloc buffers).  One kernel on the engine's queue instead
+// of three copies, each of which would wait for a switch between the compute
+// queue and a copy engine.  The host reads the buffers after the kernel's event.
+__global__ __launch_bounds__(256) void k_export_state(Scalars* sc, const unsigned long long* __restrict__ wstats,
+                                                      int waves, const unsigned long long* __restrict__ partner,
+                                                      const int2* __restrict__ parked, int n, Scalars* h_sc,
+                                                      unsigned long long* h_partner, int2* h_parked) {
+  if (blockIdx.x == 0) {
+    __shared__ unsigned long long red[2][4];
+    unsigned long long at = 0, st = 0;
+    for (int g = threadIdx.x; g < waves; g += blockDim.x) {
+      at += wstats[2 * g];
+      st += wstats[2 * g + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      at += __shfl_down(at, off);
+      st += __shfl_down(st, off);
+    }
+    if ((threadIdx.x & 63) == 0) red[0][threadIdx.x >> 6] = at, red[1][threadIdx.x >> 6] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sc->atomics += red[0][0] + red[0][1] + red[0][2] + red[0][3];
+      sc->steps += red[1][0] + red[1][1] + red[1][2] + red[1][3];
+      *h_sc = *sc;
+    }
+  }
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+  for (int i = tid; i < n; i += nth) h_partner[i] = partner[i];
+  const int pc = min(sc->parked_count, n);
+  for (int i = tid; i < pc; i += nth) h_parked[i] = parked[i];
+}
+
 // ---------------------------------------------------------------------------
 // Post: match[man] = woman from the partner words; also sums (position + 1)
 // over all women, which equals the total number of proposals of any
@@ -1047,6 +1082,15 @@ hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint3
   const int per_xcd = (int)((tiles + 7) / 8);
   hipLaunchKernelGGL(k_gather_pr, dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
                      (int)tiles, per_xcd, gated, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, int waves,
+                               const unsigned long long* partner, const int2* parked, int n, Scalars* h_sc,
+                               unsigned long long* h_partner, int2* h_parked, hipStream_t s) {
+  const int grid = std::max(1, std::min(64, (n + 255) / 256));
+  hipLaunchKernelGGL(k_export_state, dim3(grid), dim3(256), 0, s, sc, wstats, waves, partner, parked, n, h_sc,
+                     h_partner, h_parked);
   return hipGetLastError();
 }
 

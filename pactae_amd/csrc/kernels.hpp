@@ -65,6 +65,11 @@ hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int 
                               hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);
+// Folds the proposal waves' counters, then writes the control block, the partner
+// words and the parked chains into host memory (device-visible pinned pointers).
+hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, int waves,
+                               const unsigned long long* partner, const int2* parked, int n, Scalars* h_sc,
+                               unsigned long long* h_partner, int2* h_parked, hipStream_t s);
 hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
                        hipStream_t s);
 
