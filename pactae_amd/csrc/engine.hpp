@@ -536,16 +536,24 @@ struct smp_engine : public pactae::AbsSmpEngine {
     rank_built = false;
     bins_pending = false;
     times.schedule = opts.schedule;
-    SMP_HIP_TRY(pactae::launch_reset_scalars(d_sc, std::min(n, propose_grid() * 4), stream));
-    if (!opts.precheck || opts.schedule == SMP_SCHEDULE_MW_CAS) {
+    // One launch resets the control block, clears the precheck's counts and the
+    // sparse pass's bin counters, and marks every woman unclaimed (the
+    // proposal kernel's start state; the master-list path refills the words
+    // for its men).
+    const bool run_precheck = opts.precheck && opts.schedule != SMP_SCHEDULE_MW_CAS;
+    const bool may_sparse = opts.schedule == SMP_SCHEDULE_BAMBOO && use_sparse();
+    SMP_HIP_TRY(pactae::launch_reset_scalars(
+        d_sc, std::min(n, propose_grid() * 4), stream, run_precheck ? d_seen : nullptr, (size_t)n,
+        may_sparse ? d_bin_cnt : nullptr, (size_t)pactae::bin_count(n) * pactae::kBinCounterStride,
+        opts.schedule == SMP_SCHEDULE_BAMBOO ? d_partner : nullptr, (size_t)n));
+    if (!run_precheck) {
       *perfect = false;
       return SMP_OK;
     }
-    SMP_HIP_TRY(pactae::launch_fill_i32(d_seen, n, 0, stream));
     SMP_HIP_TRY(pactae::launch_first_choices(d_pref_m, n, d_seen, d_sc, stream));
     SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
     SMP_HIP_TRY(hipEventRecord(ev[8], stream));
-    if (opts.schedule == SMP_SCHEDULE_BAMBOO && use_sparse()) {
+    if (may_sparse) {
       // The sparse pass's bucket kernels go right behind the precheck, gated on
       // its flags on the device, so they run while the host reads the flags.
       if (int st = launch_buckets(/*gated=*/1)) return st;
@@ -597,7 +605,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     matrices_valid = true;
     sparse = use_sparse();
     times.sparse_init = sparse ? 1 : 0;
-    SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
+    // (The women's partner words were set to unclaimed by the precheck's reset.)
     if (sparse) {
       // Buckets of (man, position) per woman over the first K positions (queued
       // by the precheck unless it was off or saw all first choices equal), then

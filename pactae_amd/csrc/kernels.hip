@@ -40,7 +40,17 @@ __global__ void k_fill_i32(int* p, size_t count, int v) {
     p[i] = v;
 }
 
-__global__ void k_reset_scalars(Scalars* sc, int active) {
+// Resets the control block, zeroes up to two int arrays (the precheck's
+// first-choice counts, the sparse pass's bin counters) and sets the women's
+// partner words to unclaimed, all in one launch.
+__global__ void k_reset_scalars(Scalars* sc, int active, int* z0, size_t n0, int* z1, size_t n1,
+                                unsigned long long* partner, size_t np) {
+  const size_t nth = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n0 + n1 + np; i += nth) {
+    if (i < n0) z0[i] = 0;
+    else if (i < n0 + n1) z1[i - n0] = 0;
+    else partner[i - n0 - n1] = kEmpty;
+  }
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     sc->active = active;
     sc->parked_count = 0;
@@ -945,8 +955,13 @@ hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s) {
-  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, active);
+hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s, int* z0, size_t n0, int* z1, size_t n1,
+                                unsigned long long* partner, size_t np) {
+  if (!z0) n0 = 0;
+  if (!z1) n1 = 0;
+  if (!partner) np = 0;
+  const int grid = (int)std::max<size_t>(1, std::min<size_t>(128, (n0 + n1 + np + 255) / 256));
+  hipLaunchKernelGGL(k_reset_scalars, dim3(grid), dim3(256), 0, s, sc, active, z0, n0, z1, n1, partner, np);
   return hipGetLastError();
 }
 
@@ -1034,7 +1049,6 @@ hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cn
                                 hipStream_t s) {
   const int nbins = bin_count(n);
   if (nbins > kBinWomen) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, bin_cnt, (size_t)nbins * kBinPad, 0);
   const unsigned long long total = (unsigned long long)n * (unsigned long long)K;
   const long long chunks = (long long)((total + kBinEntries - 1) / kBinEntries);
   const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));

@@ -28,7 +28,10 @@ struct ProposeArgs {
 hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s);
-hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s);
+// Also zeroes z0[0..n0) and z1[0..n1) and sets partner[0..np) to kEmpty (null = none).
+hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s, int* z0 = nullptr, size_t n0 = 0,
+                                int* z1 = nullptr, size_t n1 = 0, unsigned long long* partner = nullptr,
+                                size_t np = 0);
 hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars* sc, hipStream_t s);
 hipError_t launch_rows_equal(const int32_t* pref_m, int n, Scalars* sc, int cus, hipStream_t s);
 hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars* sc, hipStream_t s);
@@ -50,6 +53,7 @@ hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint3
                             int gated, Scalars* sc, hipStream_t s);
 // Sparse prefix pass: buckets of (m, k < K) per woman (bins of 256 women with
 // bin_cap entries each and counters kBinCounterStride ints apart in bin_cnt,
+// which must be zero: launch_reset_scalars clears them),
 // then per-woman buckets of cap entries and their counts),
 // then the PR prefix from the women's rows without the n x n rank matrix (or the
 // full rank matrix if a bin or bucket overflowed).
