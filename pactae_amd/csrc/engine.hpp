@@ -384,12 +384,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
   void configure(int ni) {
     n = ni;
     K = prefix_for(ni);
-    // Defaults from the measured sweeps at n = 2,000 and 20,000 (uniform): the
-    // host takes over at 32 free men; smp_tune replaces this with a measurement.
+    // Default from the measured sweep at n = 20,000 (uniform): the host takes
+    // over at 64 free men; smp_tune replaces this with a measurement.
     if (opts.tail_mode == SMP_TAIL_GPU)
       T = 0;
     else
-      T = opts.tail_threshold > 0 ? std::min(opts.tail_threshold, ni) : std::min(ni, 32);
+      T = opts.tail_threshold > 0 ? std::min(opts.tail_threshold, ni) : std::min(ni, 64);
     if (ni == n_alloc && opts.tail_mode == SMP_TAIL_AUTO && tuned_T >= 0) {
       if (opts.prefix_cols <= 0) K = tuned_K;
       if (opts.tail_threshold <= 0) T = tuned_T;
@@ -405,7 +405,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int tune(int trials, smp_tune_result* out) {
     if (trials <= 0) trials = 5;
     const int Ks[] = {64, 128, 256};
-    const int Ts[] = {0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 1024};
+    const int Ts[] = {0, 1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 256, 1024};
     const smp_opts saved = opts;
     struct Cand {
       int k, t;
@@ -450,7 +450,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       opts = saved;
       opts.tail_mode = SMP_TAIL_AUTO;
       opts.prefix_cols = opts.tail_threshold = 0;
-      const int def_k = prefix_for(n), def_t = std::min(n, 32);
+      const int def_k = prefix_for(n), def_t = std::min(n, 64);
       bool have = false;
       for (const Cand& c : finals) have = have || (c.k == def_k && c.t == def_t);
       if (!have) finals.push_back({def_k, def_t, 0});
