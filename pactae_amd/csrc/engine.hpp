@@ -146,7 +146,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
   bool rank_built = false;       // d_rank_w holds this run's women's rank matrix
 
   uint32_t* h_pr = nullptr;  // pinned copy of the PR prefix for the host tail
-  uint32_t* h_pr_dev = nullptr;  // the same buffer as seen from the device
   unsigned long long* h_partner = nullptr;
   Scalars* h_sc = nullptr;
   int2* h_parked = nullptr;
@@ -282,12 +281,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
-    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h_pr_dev), h_pr, 0) != hipSuccess ||
-        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_partner_dev), h_partner, 0) != hipSuccess ||
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h_partner_dev), h_partner, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&h_sc_dev), h_sc, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&h_parked_dev), h_parked, 0) != hipSuccess) {
       (void)hipGetLastError();
-      pactae::set_error("pinned PR prefix buffer is not visible to the device");
+      pactae::set_error("pinned hand-off buffers are not visible to the device");
       return SMP_ERR_HIP;
     }
     if (hipHostMalloc(&h_match, (size_t)n * sizeof(int32_t), pactae::kHostAllocFlags) != hipSuccess ||
