@@ -33,15 +33,19 @@ def test_struct_layouts_match_header(tmp_path):
     from pactae_amd import _abi
     src = tmp_path / "sz.c"
     src.write_text('#include "smp_hip.h"\n#include <stdio.h>\n#include <stddef.h>\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(smp_opts), sizeof(smp_times),'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(smp_opts), sizeof(smp_times),'
                    ' sizeof(smp_batch_opts), sizeof(smp_batch_times), offsetof(smp_times, proposals),'
-                   ' offsetof(smp_times, is_perfect)); return 0;}\n')
+                   ' offsetof(smp_times, is_perfect), offsetof(smp_opts, init_mode),'
+                   ' offsetof(smp_times, bucket_kernel_ms), offsetof(smp_times, sparse_init),'
+                   ' offsetof(smp_times, rank_matrix)); return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     want = [ctypes.sizeof(_abi.SmpOpts), ctypes.sizeof(_abi.SmpTimes), ctypes.sizeof(_abi.SmpBatchOpts),
             ctypes.sizeof(_abi.SmpBatchTimes), _abi.SmpTimes.proposals.offset,
-            _abi.SmpTimes.is_perfect.offset]
+            _abi.SmpTimes.is_perfect.offset, _abi.SmpOpts.init_mode.offset,
+            _abi.SmpTimes.bucket_kernel_ms.offset, _abi.SmpTimes.sparse_init.offset,
+            _abi.SmpTimes.rank_matrix.offset]
     assert [int(x) for x in out] == want
 
 
@@ -53,6 +57,20 @@ def test_errors_are_reported_not_thrown(smp):
                                                               np.zeros(0, np.int32), 0), 0)
     assert ei.value.status == 1
     assert "n must be" in str(ei.value)
+
+
+def test_unknown_options_are_rejected(smp):
+    # Option values are checked before any device is touched.
+    import numpy as np
+    n = 4
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=1)
+    obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+    for bad in (dict(init_mode=3), dict(init_mode=-1), dict(schedule=2)):
+        with pytest.raises(smp.SmpError) as ei:
+            smp.SmpEngineBamboo(obj, n, **bad)
+        assert ei.value.status == 1, bad
+        assert "init_mode or schedule" in str(ei.value)
+    assert [m.value for m in smp.InitMode] == [0, 1, 2]
 
 
 def test_version(smp):
