@@ -564,7 +564,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
     master = false;
     if (!*perfect && n > 1 && h_sc->first_differs == 0) {
       bins_pending = false;  // the gated bucket kernels found the path closed and did nothing
-      SMP_HIP_TRY(pactae::launch_rows_equal(d_pref_m, n, d_sc, dev.cus, stream));
+      // The men's lists compared with man 0's and, ahead of the decision, the
+      // women's permutation check the master-list path needs, in one launch.
+      SMP_HIP_TRY(pactae::launch_master_scan(d_pref_m, d_pref_w, n, opts.validate, d_sc, dev.cus, stream));
       if (int st = read_scalars()) return st;
       master = h_sc->not_master == 0;
     }
@@ -594,7 +596,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
       SMP_HIP_TRY(pactae::launch_master_inverse(d_pref_m, n, d_seen, d_sc, stream));
-      if (opts.validate) SMP_HIP_TRY(pactae::launch_check_rows(d_pref_w, n, d_sc, dev.cus, stream));
+      // (The women's rows were checked by the precheck's master-list scan.)
       SMP_HIP_TRY(hipEventRecord(ev[1], stream));
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
       return SMP_OK;

@@ -69,16 +69,27 @@ __global__ void k_reset_scalars(Scalars* sc, int active, int* z0, size_t n0, int
 // Precheck: do all men have distinct first choices?  seen[] must be zero.
 // The flags are combined per wave (ballot) before one atomic each: on uniform
 // input thousands of men repeat a first choice, and one same-address atomic per
-// man would serialize the kernel.
+// man would serialize the kernel.  The lanes that name the same woman as lane 0
+// count themselves with one atomic (the rest one each): when most men share a
+// first choice (shared or correlated lists), 20,000 same-address atomics took
+// 0.2-0.4 ms.
 __global__ void k_first_choices(const int32_t* __restrict__ pref_m, int n, int* seen, Scalars* sc) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   bool range = false, dup = false, diff = false;
+  const int w = m < n ? pref_m[(size_t)m * n] : -1;
+  const bool valid = m < n && (unsigned)w < (unsigned)n;
+  const int w0 = __shfl(w, 0);
+  const unsigned long long grp = __ballot(valid && w == w0);  // lane 0 is in it when valid
+  int grp_old = 0;
+  if (lane == 0 && grp) grp_old = atomicAdd(&seen[w0], __popcll(grp));
+  grp_old = __shfl(grp_old, 0);
   if (m < n) {
-    const int w = pref_m[(size_t)m * n];
-    if ((unsigned)w >= (unsigned)n) {
+    if (!valid) {
       range = true;
     } else {
-      dup = atomicAdd(&seen[w], 1) != 0;
+      const bool in_grp = (grp >> lane) & 1;
+      dup = in_grp ? (grp_old != 0 || __popcll(grp) >= 2) : atomicAdd(&seen[w], 1) != 0;
       diff = w != pref_m[0];
     }
   }
@@ -663,14 +674,47 @@ __global__ __launch_bounds__(256) void k_post(const unsigned long long* __restri
 // n*H_n proposals instead of n(n+1)/2.  A man ranks woman w at Linv[w], the same
 // for every man, so a proposal needs no rank gather.
 
-// One wave per man m >= 1 compares his list with man 0's, 512 entries a step, and
-// stops at the first difference (or when another wave has found one).
-__global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ pref_m, int n, Scalars* sc) {
+// One wave per man m >= 1 compares his list with man 0's and stops at the first
+// difference (or when another wave has found one).  With n % 4 == 0 a step reads
+// 2,048 entries as 16-byte streaming loads (8 per lane in flight); man 0's row is
+// read through the caches, where every wave finds it.
+// Workgroups blk = 0..nblk-1 of a launch (256 threads each).
+__device__ void rows_equal_blocks(const int32_t* __restrict__ pref_m, int n, Scalars* sc, int blk, int nblk) {
   const int lane = threadIdx.x & 63;
-  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
-  for (int m = 1 + (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); m < n; m += waves) {
+  const int waves = nblk * (int)(blockDim.x >> 6);
+  const bool vec = (n & 3) == 0;
+  for (int m = 1 + (int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)); m < n; m += waves) {
     if (__hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const int32_t* row = pref_m + (size_t)m * n;
+    if (vec) {
+      const int4* r4 = reinterpret_cast<const int4*>(row);
+      const int4* l4 = reinterpret_cast<const int4*>(pref_m);
+      const int nq = n >> 2;
+      for (int q0 = 0; q0 < nq; q0 += 512) {
+        int4 a[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int q = min(q0 + 64 * j + lane, nq - 1);
+          a[j].x = __builtin_nontemporal_load(&r4[q].x);
+          a[j].y = __builtin_nontemporal_load(&r4[q].y);
+          a[j].z = __builtin_nontemporal_load(&r4[q].z);
+          a[j].w = __builtin_nontemporal_load(&r4[q].w);
+        }
+        bool diff = false;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int q = min(q0 + 64 * j + lane, nq - 1);
+          const int4 b = l4[q];
+          diff |= a[j].x != b.x || a[j].y != b.y || a[j].z != b.z || a[j].w != b.w;
+        }
+        if (__ballot(diff)) {
+          if (lane == 0 && !__hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicOr(&sc->not_master, 1);
+          return;
+        }
+      }
+      continue;
+    }
     for (int k0 = 0; k0 < n; k0 += 512) {
       bool diff = false;
 #pragma unroll
@@ -687,6 +731,10 @@ __global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ 
   }
 }
 
+__global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ pref_m, int n, Scalars* sc) {
+  rows_equal_blocks(pref_m, n, sc, (int)blockIdx.x, (int)gridDim.x);
+}
+
 // linv[L[k]] = k over the shared list L (linv pre-filled with -1).
 __global__ void k_master_inverse(const int32_t* __restrict__ L, int n, int32_t* linv, Scalars* sc) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -699,12 +747,18 @@ __global__ void k_master_inverse(const int32_t* __restrict__ L, int n, int32_t* 
   }
 }
 
-// Permutation check of every row (one workgroup per row, an n-bit LDS set).
-__global__ __launch_bounds__(256) void k_check_rows(const int32_t* __restrict__ pref, int n, Scalars* sc) {
-  extern __shared__ uint32_t bits[];
+// Permutation check of every row (one workgroup per row, an n-bit LDS set), by
+// workgroups blk = 0..nblk-1 of a launch.  With stop_unless_master the check
+// stops once another workgroup has found that the men's lists differ (it then ran
+// ahead of a decision that went the other way; the general path checks the rows
+// itself).
+__device__ void check_rows_blocks(const int32_t* __restrict__ pref, int n, Scalars* sc, int blk, int nblk,
+                                  uint32_t* bits, bool stop_unless_master) {
   const int words = (n + 31) >> 5;
   int bad = 0;
-  for (int row = blockIdx.x; row < n; row += gridDim.x) {
+  for (int row = blk; row < n; row += nblk) {
+    if (stop_unless_master && __hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      break;
     for (int i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0;
     __syncthreads();
     const int32_t* src = pref + (size_t)row * n;
@@ -732,6 +786,26 @@ __global__ __launch_bounds__(256) void k_check_rows(const int32_t* __restrict__ 
   if (bad) atomicOr(&sc->error, bad);
 }
 
+__global__ __launch_bounds__(256) void k_check_rows(const int32_t* __restrict__ pref, int n, Scalars* sc) {
+  extern __shared__ uint32_t bits[];
+  check_rows_blocks(pref, n, sc, (int)blockIdx.x, (int)gridDim.x, bits, false);
+}
+
+// Master-list detection and the women's permutation check in one launch: the
+// first g_rows workgroups compare the men's lists with man 0's, the others check
+// the women's rows (when check != 0), so the two 4n^2-byte reads overlap instead of
+// running one after the other.
+__global__ __launch_bounds__(256) void k_master_scan(const int32_t* __restrict__ pref_m,
+                                                     const int32_t* __restrict__ pref_w, int n, int check,
+                                                     int g_rows, Scalars* sc) {
+  extern __shared__ uint32_t bits[];
+  if ((int)blockIdx.x < g_rows) {
+    rows_equal_blocks(pref_m, n, sc, (int)blockIdx.x, g_rows);
+    return;
+  }
+  if (check) check_rows_blocks(pref_w, n, sc, (int)blockIdx.x - g_rows, (int)gridDim.x - g_rows, bits, true);
+}
+
 // Women-proposing walk: one wave per chain (woman), 64 list positions a step.
 // Woman w's proposal to the man at position k of her list is the packed word
 // (Linv[w], w, k); the man keeps the smallest (atomicMin on his partner word), so
@@ -745,7 +819,9 @@ __global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restric
   const int waves = (int)(gridDim.x * (blockDim.x >> 6));
   int next_init = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   int w = -1, k = 0;
-  uint32_t rw = 0;
+  // Speculation: while a claim is in flight, the next segment of the woman it
+  // would displace (by the hint) is loaded too; (spec_w, spec_k) says whose.
+  int spec_w = -1, spec_k = -1, spec_m = 0;
   unsigned long long n_atomics = 0, n_steps = 0;
   if (__hip_atomic_load(&sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) next_init = n;
   for (;;) {
@@ -759,20 +835,22 @@ __global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restric
       if (lane == 0) atomicOr(&sc->error, kErrListEnd);
       break;
     }
-    rw = (uint32_t)linv[w];
+    const uint32_t rw = (uint32_t)linv[w];
     const int kk = k + lane;
     bool valid = kk < n;
     int m = 0;
     if (valid) {
-      m = pref_w[(size_t)w * n + kk];
+      m = (spec_w == w && spec_k == k) ? spec_m : pref_w[(size_t)w * n + kk];
       if ((unsigned)m >= (unsigned)n) {
         valid = false;
         atomicOr(&sc->error, kErrEntryRange);
       }
     }
+    spec_w = -1;
     const unsigned long long mine = pw_pack(rw, (uint32_t)w, (uint32_t)kk);
-    const unsigned long long hint =
-        valid ? __hip_atomic_load(&partner_m[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+    // Plain (cached) reads: a stale hint is only ever too large (words only
+    // decrease), which can cost a failed atomic but never hides a candidate.
+    const unsigned long long hint = valid ? partner_m[m] : 0ull;
     unsigned long long cand = __ballot(valid && mine < hint);
     ++n_steps;
     bool moved = false;
@@ -780,6 +858,13 @@ __global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restric
       const int j = __ffsll((long long)cand) - 1;
       unsigned long long old = 0;
       if (lane == j) old = atomicMin(&partner_m[m], mine);
+      const unsigned long long hj = __shfl(hint, j);
+      int sw = -1, sk = -1;
+      if (hj != kEmpty) {
+        sw = (int)pw_man(hj);
+        sk = (int)pw_pos(hj) + 1;
+        if ((unsigned)sw < (unsigned)n && sk + lane < n) spec_m = pref_w[(size_t)sw * n + sk + lane];
+      }
       old = __shfl(old, j);
       ++n_atomics;
       if (old > __shfl(mine, j)) {
@@ -788,6 +873,7 @@ __global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restric
         } else {
           w = (int)pw_man(old);
           k = (int)pw_pos(old) + 1;
+          if (w == sw && k == sk) spec_w = sw, spec_k = sk;
         }
         moved = true;
         break;
@@ -968,6 +1054,15 @@ hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s, int* z0,
 hipError_t launch_rows_equal(const int32_t* pref_m, int n, Scalars* sc, int cus, hipStream_t s) {
   const int grid = std::max(1, std::min((n + 3) / 4, cus * 8));
   hipLaunchKernelGGL(k_rows_equal, dim3(grid), dim3(256), 0, s, pref_m, n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_master_scan(const int32_t* pref_m, const int32_t* pref_w, int n, int check, Scalars* sc, int cus,
+                              hipStream_t s) {
+  const int g_rows = std::max(1, std::min((n + 3) / 4, cus * 4));
+  const int g_check = check ? std::max(1, std::min(n, cus * 4)) : 0;
+  const size_t lds = (size_t)((n + 31) / 32) * 4;
+  hipLaunchKernelGGL(k_master_scan, dim3(g_rows + g_check), dim3(256), lds, s, pref_m, pref_w, n, check, g_rows, sc);
   return hipGetLastError();
 }
 
