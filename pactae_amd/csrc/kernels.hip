@@ -8,7 +8,7 @@
 //   k_gather_pr                       F9PRMatrixInitKernel   src/smp/kernel.cu:688-709
 //   k_propose                         La3F9CoreKernel        src/smp/kernel.cu:633-670
 //   k_post                            PostProc               src/smp/smp_engine_bamboo.cu:299-303
-//   k_rows_equal .. k_post_master     master-list path (every man holds the same list):
+//   k_master_scan .. k_post_master    master-list path (every man holds the same list):
 //                                     detection, women-proposing walk, matching
 //   k_mw_cas / k_post_cas             MW-Par-GPU-CAS baseline schedule
 //                                     src/smp/smp_engine_mw_3_f9.cu:37-97
@@ -731,10 +731,6 @@ __device__ void rows_equal_blocks(const int32_t* __restrict__ pref_m, int n, Sca
   }
 }
 
-__global__ __launch_bounds__(256) void k_rows_equal(const int32_t* __restrict__ pref_m, int n, Scalars* sc) {
-  rows_equal_blocks(pref_m, n, sc, (int)blockIdx.x, (int)gridDim.x);
-}
-
 // linv[L[k]] = k over the shared list L (linv pre-filled with -1).
 __global__ void k_master_inverse(const int32_t* __restrict__ L, int n, int32_t* linv, Scalars* sc) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -784,11 +780,6 @@ __device__ void check_rows_blocks(const int32_t* __restrict__ pref, int n, Scala
     __syncthreads();
   }
   if (bad) atomicOr(&sc->error, bad);
-}
-
-__global__ __launch_bounds__(256) void k_check_rows(const int32_t* __restrict__ pref, int n, Scalars* sc) {
-  extern __shared__ uint32_t bits[];
-  check_rows_blocks(pref, n, sc, (int)blockIdx.x, (int)gridDim.x, bits, false);
 }
 
 // Master-list detection and the women's permutation check in one launch: the
@@ -1051,12 +1042,6 @@ hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s, int* z0,
   return hipGetLastError();
 }
 
-hipError_t launch_rows_equal(const int32_t* pref_m, int n, Scalars* sc, int cus, hipStream_t s) {
-  const int grid = std::max(1, std::min((n + 3) / 4, cus * 8));
-  hipLaunchKernelGGL(k_rows_equal, dim3(grid), dim3(256), 0, s, pref_m, n, sc);
-  return hipGetLastError();
-}
-
 hipError_t launch_master_scan(const int32_t* pref_m, const int32_t* pref_w, int n, int check, Scalars* sc, int cus,
                               hipStream_t s) {
   const int g_rows = std::max(1, std::min((n + 3) / 4, cus * 4));
@@ -1070,13 +1055,6 @@ hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars
   hipError_t e = launch_fill_i32(linv, n, -1, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_master_inverse, dim3((n + 255) / 256), dim3(256), 0, s, L, n, linv, sc);
-  return hipGetLastError();
-}
-
-hipError_t launch_check_rows(const int32_t* pref, int n, Scalars* sc, int cus, hipStream_t s) {
-  const size_t lds = (size_t)((n + 31) / 32) * 4;
-  const int grid = std::max(1, std::min(n, cus * 8));
-  hipLaunchKernelGGL(k_check_rows, dim3(grid), dim3(256), lds, s, pref, n, sc);
   return hipGetLastError();
 }
 
