@@ -128,6 +128,7 @@ __global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t
 // inversion; for the sparse pass 1024 x 8 loads ≈ 768 x 7 < 512 x 10 < 640 x 8.
 constexpr int kInvBlock = 1024;
 constexpr int kBlk = 8;
+constexpr int kBlkShort = 5;  // sparse pass on rows of at most 20,480 entries
 
 
 struct PrefixBuckets {
@@ -149,11 +150,27 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
   const int tid = threadIdx.x;
   int bad = 0;
   // Sparse pass: look up the bucket entries unless a bucket overflowed.
-  const bool lookup =
-      SP && __hip_atomic_load(&sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
-  for (int row = blockIdx.x; row < n; row += gridDim.x) {
+  // (The flag is read here and tested after the scatter, so the row's loads do
+  // not wait for it.)
+  int overflow = SP ? __hip_atomic_load(&sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 1;
+  {  // one row per workgroup: the grid is n workgroups
+    const int row = blockIdx.x;
     const int32_t* src = pref + (size_t)row * n;
     uint16_t* dst = rank + (size_t)row * n;
+    // The sparse pass's bucket loads (the woman's count and the first BS
+    // entries of her bucket) are issued ahead of the row's loads, so their
+    // latency hides behind the row instead of following the scatter.  Entries
+    // at or past the count are read (the bucket holds cap entries) but never
+    // used, and so are all of them if a bucket overflowed: the loads do not wait
+    // for the overflow flag.  The eight-load variant has no registers to spare
+    // for this (it would spill) and reads them at the lookup.
+    constexpr bool kEarly = SP && VEC == 4 && KB <= kBlkShort;
+    int cntw = 0;
+    uint32_t e_first = 0;
+    if (kEarly) {
+      cntw = pb.cnt[row];
+      if (tid < pb.cap) e_first = pb.bucket[(size_t)row * pb.cap + tid];
+    }
 
     for (int c0 = 0; c0 < n; c0 += chunk) {
       const int clen = min(chunk, n - c0);
@@ -170,19 +187,27 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
           // before the first wait.
 #pragma unroll
           for (int j = 0; j < KB; ++j) {
-            const int q = min(q0 + j * BS, nq - 1);
+            // A 32-bit byte offset from the row's (uniform) base: one VGPR per
+            // address instead of two (a row is at most 256 KB).
+            const uint32_t off = (uint32_t)min(q0 + j * BS, nq - 1) << 4;
+            const int4* a = reinterpret_cast<const int4*>(reinterpret_cast<const char*>(src4) + off);
             if constexpr (NT) {
-              v[j].x = __builtin_nontemporal_load(&src4[q].x);
-              v[j].y = __builtin_nontemporal_load(&src4[q].y);
-              v[j].z = __builtin_nontemporal_load(&src4[q].z);
-              v[j].w = __builtin_nontemporal_load(&src4[q].w);
+              v[j].x = __builtin_nontemporal_load(&a->x);
+              v[j].y = __builtin_nontemporal_load(&a->y);
+              v[j].z = __builtin_nontemporal_load(&a->z);
+              v[j].w = __builtin_nontemporal_load(&a->w);
             } else {
-              v[j] = src4[q];
+              v[j] = *a;
             }
           }
+          // With kEarly the clamped vectors are scattered too (they rewrite the
+          // row's last four slots with the same values), so the scatter has no
+          // branch and every load has been waited for at the end of a round;
+          // otherwise the loop header waits for all outstanding loads, the
+          // bucket loads included, before issuing the row's.
 #pragma unroll
           for (int j = 0; j < KB; ++j) {
-            const int q = q0 + j * BS;
+            const int q = kEarly ? min(q0 + j * BS, nq - 1) : q0 + j * BS;
             if (q < nq) {
               const int r = q << 2;
               const int e[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
@@ -214,12 +239,23 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         }
       }
       __syncthreads();
+      // Opaque here, or the compiler tests the flag ahead of the row's loads.
+      if constexpr (SP) asm volatile("" : "+v"(overflow));
+      const bool lookup = SP && overflow == 0;
       if (lookup) {
         // Prefix entries of this woman's suitors whose man falls in this chunk.
-        const int cntw = min(pb.cnt[row], pb.cap);
+        if constexpr (kEarly) {
+          // Opaque from here on: otherwise the compiler hoists the first
+          // entry's address arithmetic above the row's loads, which then wait
+          // for the bucket loads instead of overlapping them.
+          asm volatile("" : "+v"(e_first), "+v"(cntw));
+        } else {
+          cntw = pb.cnt[row];
+        }
+        cntw = min(cntw, pb.cap);
         const uint32_t* bk = pb.bucket + (size_t)row * pb.cap;
         for (int j = tid; j < cntw; j += BS) {
-          const uint32_t e = bk[j];
+          const uint32_t e = kEarly && j == tid ? e_first : bk[j];
           const unsigned u = (e & 0xFFFFu) - (unsigned)c0;
           if (u < (unsigned)clen)
             pb.pr[pr_index((int)(e & 0xFFFFu), (int)(e >> 16), n)] = pr_pack((uint32_t)row, lds16[u]);
@@ -1138,7 +1174,12 @@ hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int 
   const size_t lds = (size_t)((chunk + 7) & ~7) * sizeof(uint16_t);
   const bool vec = (n % 4 == 0) && (chunk % 8 == 0 || chunk == n);
   const PrefixBuckets pb{cnt, bucket, cap, pr};
-  if (vec)
+  // Five loads per thread cover rows up to 20,480 entries in one round, with
+  // registers to spare for the bucket entry loaded ahead of the row.
+  if (vec && n / 4 <= kInvBlock * kBlkShort)
+    hipLaunchKernelGGL((k_invert_rows<4, 1, 1, kInvBlock, kBlkShort>), dim3(n), dim3(kInvBlock), lds, s, pref_w,
+                       rank, n, chunk, validate, sc, pb);
+  else if (vec)
     hipLaunchKernelGGL((k_invert_rows<4, 1, 1>), dim3(n), dim3(kInvBlock), lds, s, pref_w, rank, n, chunk,
                        validate, sc, pb);
   else
@@ -1152,6 +1193,7 @@ hipError_t set_invert_lds_limit(int bytes) {
                        reinterpret_cast<const void*>(&k_invert_rows<4, 0, 0>),
                        reinterpret_cast<const void*>(&k_invert_rows<1, 0, 0>),
                        reinterpret_cast<const void*>(&k_invert_rows<4, 1, 1>),
+                       reinterpret_cast<const void*>(&k_invert_rows<4, 1, 1, kInvBlock, kBlkShort>),
                        reinterpret_cast<const void*>(&k_invert_rows<1, 0, 1>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
