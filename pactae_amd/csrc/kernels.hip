@@ -128,7 +128,7 @@ __global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t
 // inversion; for the sparse pass 1024 x 8 loads ≈ 768 x 7 < 512 x 10 < 640 x 8.
 constexpr int kInvBlock = 1024;
 constexpr int kBlk = 8;
-constexpr int kBlkShort = 5;  // sparse pass on rows of at most 20,480 entries
+constexpr int kBlkShort = 5;  // k_rank_prefix_row: rows of at most 20,480 entries
 
 
 struct PrefixBuckets {
@@ -157,20 +157,6 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const int row = blockIdx.x;
     const int32_t* src = pref + (size_t)row * n;
     uint16_t* dst = rank + (size_t)row * n;
-    // The sparse pass's bucket loads (the woman's count and the first BS
-    // entries of her bucket) are issued ahead of the row's loads, so their
-    // latency hides behind the row instead of following the scatter.  Entries
-    // at or past the count are read (the bucket holds cap entries) but never
-    // used, and so are all of them if a bucket overflowed: the loads do not wait
-    // for the overflow flag.  The eight-load variant has no registers to spare
-    // for this (it would spill) and reads them at the lookup.
-    constexpr bool kEarly = SP && VEC == 4 && KB <= kBlkShort;
-    int cntw = 0;
-    uint32_t e_first = 0;
-    if (kEarly) {
-      cntw = pb.cnt[row];
-      if (tid < pb.cap) e_first = pb.bucket[(size_t)row * pb.cap + tid];
-    }
 
     for (int c0 = 0; c0 < n; c0 += chunk) {
       const int clen = min(chunk, n - c0);
@@ -200,14 +186,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
               v[j] = *a;
             }
           }
-          // With kEarly the clamped vectors are scattered too (they rewrite the
-          // row's last four slots with the same values), so the scatter has no
-          // branch and every load has been waited for at the end of a round;
-          // otherwise the loop header waits for all outstanding loads, the
-          // bucket loads included, before issuing the row's.
 #pragma unroll
           for (int j = 0; j < KB; ++j) {
-            const int q = kEarly ? min(q0 + j * BS, nq - 1) : q0 + j * BS;
+            const int q = q0 + j * BS;
             if (q < nq) {
               const int r = q << 2;
               const int e[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
@@ -244,18 +225,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
       const bool lookup = SP && overflow == 0;
       if (lookup) {
         // Prefix entries of this woman's suitors whose man falls in this chunk.
-        if constexpr (kEarly) {
-          // Opaque from here on: otherwise the compiler hoists the first
-          // entry's address arithmetic above the row's loads, which then wait
-          // for the bucket loads instead of overlapping them.
-          asm volatile("" : "+v"(e_first), "+v"(cntw));
-        } else {
-          cntw = pb.cnt[row];
-        }
-        cntw = min(cntw, pb.cap);
+        const int cntw = min(pb.cnt[row], pb.cap);
         const uint32_t* bk = pb.bucket + (size_t)row * pb.cap;
         for (int j = tid; j < cntw; j += BS) {
-          const uint32_t e = kEarly && j == tid ? e_first : bk[j];
+          const uint32_t e = bk[j];
           const unsigned u = (e & 0xFFFFu) - (unsigned)c0;
           if (u < (unsigned)clen)
             pb.pr[pr_index((int)(e & 0xFFFFu), (int)(e >> 16), n)] = pr_pack((uint32_t)row, lds16[u]);
@@ -300,6 +273,107 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         }
       }
       __syncthreads();
+    }
+  }
+  if (bad & 1) atomicOr(&sc->error, kErrEntryRange);
+  if (bad & 2) atomicOr(&sc->error, kErrNotPerm);
+}
+
+// ---------------------------------------------------------------------------
+// Sparse prefix pass for rows that one round of loads covers and the LDS holds
+// whole (n <= BS * KB * 4; 20,480 with 1024 threads x 5 loads): the work of
+// k_invert_rows<4, 1, 1> for one row per workgroup, as straight-line code.
+// Every load is issued first (the row's, then the woman's bucket count and
+// first BS entries), so they are in flight while the LDS row is cleared and
+// during the barrier; the overflow flag read at the start is tested only after
+// the scatter.  Bucket entries at or past the count, or all of them after an
+// overflow, are read but not used.
+template <int BS, int KB>
+__device__ __forceinline__ void load_row_vec4(const int32_t* __restrict__ src, int nq, int tid, int4 (&v)[KB]) {
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    // 32-bit byte offsets from the row's uniform base (a row is at most 256 KB);
+    // past the row's end the last vector is loaded again.
+    const uint32_t off = (uint32_t)min(tid + j * BS, nq - 1) << 4;
+    const int4* a = reinterpret_cast<const int4*>(reinterpret_cast<const char*>(src) + off);
+    v[j].x = __builtin_nontemporal_load(&a->x);
+    v[j].y = __builtin_nontemporal_load(&a->y);
+    v[j].z = __builtin_nontemporal_load(&a->z);
+    v[j].w = __builtin_nontemporal_load(&a->w);
+  }
+}
+
+template <int BS, int KB>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_rank_prefix_row(
+    const int32_t* __restrict__ pref, uint16_t* __restrict__ rank, int n, int validate, Scalars* sc,
+    PrefixBuckets pb) {
+  extern __shared__ uint16_t lds16[];
+  uint32_t* lds32 = reinterpret_cast<uint32_t*>(lds16);
+  const int tid = threadIdx.x;
+  const int row = blockIdx.x;
+  const int nq = n >> 2;
+  int bad = 0;
+  int overflow = __hip_atomic_load(&sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int4 v[KB];
+  load_row_vec4<BS, KB>(pref + (size_t)row * n, nq, tid, v);
+  int cntw = pb.cnt[row];
+  uint32_t e_first = tid < pb.cap ? pb.bucket[(size_t)row * pb.cap + tid] : 0u;
+  if (validate) {
+    for (int c = tid; c < (n + 1) / 2; c += BS) lds32[c] = 0xFFFFFFFFu;
+    __syncthreads();
+  }
+  // The clamped vectors rewrite the row's last four slots with the same values,
+  // so the scatter has no branch.
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    const int q = min(tid + j * BS, nq - 1);
+    const int e[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if ((unsigned)e[t] < (unsigned)n) lds16[e[t]] = (uint16_t)((q << 2) + t);
+      else bad |= 1;
+    }
+  }
+  __syncthreads();
+  // Opaque from here: the compiler may not compute with these values (and so
+  // wait for their loads) ahead of this point.
+  asm volatile("" : "+v"(overflow), "+v"(cntw), "+v"(e_first));
+  if (overflow == 0) {
+    cntw = min(cntw, pb.cap);
+    const uint32_t* bk = pb.bucket + (size_t)row * pb.cap;
+    for (int j = tid; j < cntw; j += BS) {
+      const uint32_t e = j == tid ? e_first : bk[j];
+      const int m = (int)(e & 0xFFFFu);
+      if (m < n) pb.pr[pr_index(m, (int)(e >> 16), n)] = pr_pack((uint32_t)row, lds16[m]);
+    }
+    if (validate) {  // a hole in the LDS row marks a row that is not a permutation
+      for (int c = tid; c < (n + 1) / 2; c += BS) {
+        const uint32_t x = lds32[c];
+        if ((x & 0xFFFFu) == 0xFFFFu || (2 * c + 1 < n && (x >> 16) == 0xFFFFu)) bad |= 2;
+      }
+    }
+  } else {
+    // A bucket overflowed: the whole rank row, as the full inversion writes it.
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    auto has_hole = [](unsigned x) { return (x & 0xFFFFu) == 0xFFFFu || (x >> 16) == 0xFFFFu; };
+    uint16_t* dst = rank + (size_t)row * n;
+    if ((n & 7) == 0) {
+      const v4u* l4 = reinterpret_cast<const v4u*>(lds16);
+      v4u* dst8 = reinterpret_cast<v4u*>(dst);
+      for (int q = tid; q < (n >> 3); q += BS) {
+        const v4u p = l4[q];
+        if (validate && (has_hole(p.x) || has_hole(p.y) || has_hole(p.z) || has_hole(p.w))) bad |= 2;
+        __builtin_nontemporal_store(p, dst8 + q);
+      }
+    } else {
+      const v2u* l2 = reinterpret_cast<const v2u*>(lds16);
+      v2u* dst4 = reinterpret_cast<v2u*>(dst);
+      for (int q = tid; q < (n >> 2); q += BS) {
+        const v2u p = l2[q];
+        if (validate && (has_hole(p.x) || has_hole(p.y))) bad |= 2;
+        __builtin_nontemporal_store(p, dst4 + q);
+      }
     }
   }
   if (bad & 1) atomicOr(&sc->error, kErrEntryRange);
@@ -1174,11 +1248,10 @@ hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int 
   const size_t lds = (size_t)((chunk + 7) & ~7) * sizeof(uint16_t);
   const bool vec = (n % 4 == 0) && (chunk % 8 == 0 || chunk == n);
   const PrefixBuckets pb{cnt, bucket, cap, pr};
-  // Five loads per thread cover rows up to 20,480 entries in one round, with
-  // registers to spare for the bucket entry loaded ahead of the row.
-  if (vec && n / 4 <= kInvBlock * kBlkShort)
-    hipLaunchKernelGGL((k_invert_rows<4, 1, 1, kInvBlock, kBlkShort>), dim3(n), dim3(kInvBlock), lds, s, pref_w,
-                       rank, n, chunk, validate, sc, pb);
+  // Five loads per thread cover rows up to 20,480 entries in one round.
+  if (vec && chunk == n && n / 4 <= kInvBlock * kBlkShort)
+    hipLaunchKernelGGL((k_rank_prefix_row<kInvBlock, kBlkShort>), dim3(n), dim3(kInvBlock), lds, s, pref_w, rank,
+                       n, validate, sc, pb);
   else if (vec)
     hipLaunchKernelGGL((k_invert_rows<4, 1, 1>), dim3(n), dim3(kInvBlock), lds, s, pref_w, rank, n, chunk,
                        validate, sc, pb);
@@ -1193,7 +1266,7 @@ hipError_t set_invert_lds_limit(int bytes) {
                        reinterpret_cast<const void*>(&k_invert_rows<4, 0, 0>),
                        reinterpret_cast<const void*>(&k_invert_rows<1, 0, 0>),
                        reinterpret_cast<const void*>(&k_invert_rows<4, 1, 1>),
-                       reinterpret_cast<const void*>(&k_invert_rows<4, 1, 1, kInvBlock, kBlkShort>),
+                       reinterpret_cast<const void*>(&k_rank_prefix_row<kInvBlock, kBlkShort>),
                        reinterpret_cast<const void*>(&k_invert_rows<1, 0, 1>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
