@@ -138,7 +138,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   uint32_t* d_bucket = nullptr;  // sparse prefix pass: per-woman buckets of (m, k)
   unsigned long long* d_bins = nullptr;  // sparse prefix pass: bins of 256 women
   int* d_bin_cnt = nullptr;      // entries per bin
-  int* d_bucket_cnt = nullptr;   // entries per woman's bucket
+  int* d_bucket_cnt = nullptr;   // entries per woman's bucket (inside the d_bin_cnt allocation)
   bool bins_pending = false;     // the bucket kernels of this run are queued (after the precheck)
   int K_sparse_alloc = 0;        // widest prefix the sparse buffers hold
   bool sparse_overflowed = false;  // a sparse pass on these lists overflowed a bin or bucket
@@ -210,8 +210,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     d_bins = nullptr;
     (void)hipFree(d_bin_cnt);
     d_bin_cnt = nullptr;
-    (void)hipFree(d_bucket_cnt);
-    d_bucket_cnt = nullptr;
+    d_bucket_cnt = nullptr;  // inside d_bin_cnt
     (void)hipFree(d_wstats);
     d_wstats = nullptr;
     (void)hipFree(d_starts);
@@ -300,14 +299,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (hipMalloc(&d_bucket, (size_t)n * bucket_cap(n, K_sparse_alloc) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_bins, (size_t)pactae::bin_count(n) * bin_cap(K_sparse_alloc) * sizeof(unsigned long long)) !=
             hipSuccess ||
-        hipMalloc(&d_bin_cnt, 256 * pactae::kBinCounterStride * sizeof(int)) != hipSuccess ||
-        hipMalloc(&d_bucket_cnt, (size_t)n * sizeof(int)) != hipSuccess) {
+        hipMalloc(&d_bin_cnt, counter_words() * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("allocation of the prefix buckets failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
     }
+    d_bucket_cnt = d_bin_cnt + 256 * pactae::kBinCounterStride;
     return SMP_OK;
   }
+  // Bin counters, then the per-woman bucket counts: one range the reset clears.
+  size_t counter_words() const { return 256 * (size_t)pactae::kBinCounterStride + (size_t)n_alloc; }
 
   // Sparse prefix pass: bucket capacity per woman for prefix width k.  A woman's
   // suitor count over the first k positions of n uniform lists is about
@@ -542,7 +543,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     const bool may_sparse = opts.schedule == SMP_SCHEDULE_BAMBOO && use_sparse();
     SMP_HIP_TRY(pactae::launch_reset_scalars(
         d_sc, std::min(n, propose_grid() * 4), stream, run_precheck ? d_seen : nullptr, (size_t)n,
-        may_sparse ? d_bin_cnt : nullptr, (size_t)pactae::bin_count(n) * pactae::kBinCounterStride,
+        may_sparse ? d_bin_cnt : nullptr, counter_words(),
         opts.schedule == SMP_SCHEDULE_BAMBOO ? d_partner : nullptr, (size_t)n));
     if (!run_precheck) {
       *perfect = false;

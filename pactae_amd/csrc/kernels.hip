@@ -430,14 +430,18 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
 //                 (coalesced along the men's rows), counts them per bin of 256
 //                 women in LDS, reserves room in each bin with one global
 //                 atomicAdd per bin, and writes (w << 32 | k << 16 | m) there;
-//   k_bin_sort    one workgroup per bin places its entries into the women's
-//                 buckets with LDS atomics and writes the per-woman counts.
+//   k_bin_sort    kSortSplit workgroups per bin, each taking spans of 8192 of
+//                 its entries: LDS atomics count them per woman, one global
+//                 atomicAdd per woman reserves their slots in her bucket (the
+//                 per-woman counts, zeroed by the reset launch), then they are
+//                 placed.
 // The order inside a bucket varies from run to run, which changes nothing (each
 // entry names its own PR slot).  A bin or bucket that runs out of room sets
 // sc->prefix_overflow, and the pass falls back to the full rank matrix.
 constexpr int kBinWomen = 256;      // women per bin: bin = w >> 8 (at most 256 bins)
 constexpr int kBinThreads = 1024;   // k_bin_prefix workgroup
 constexpr int kBinEntries = 8192;   // prefix entries per k_bin_prefix workgroup (8 per thread)
+constexpr int kSortSplit = 4;       // k_bin_sort workgroups per bin
 // Bin counters sit one 128-byte line apart: every k_bin_prefix workgroup adds to
 // each of them once, and same-line atomics would serialize.
 constexpr int kBinPad = kBinCounterStride;
@@ -503,36 +507,46 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_prefix(const int32_t* __res
 }
 
 __global__ __launch_bounds__(1024) void k_bin_sort(const unsigned long long* __restrict__ bins,
-                                                   const int* __restrict__ bin_cnt, int bin_cap, int n,
+                                                   const int* __restrict__ bin_cnt, int bin_cap,
                                                    int* __restrict__ cnt, uint32_t* __restrict__ bucket, int cap,
                                                    int gated, Scalars* sc) {
   if (gated && !general_path_open(sc)) return;
   __shared__ int c[kBinWomen];
-  const int tid = threadIdx.x, b = blockIdx.x;
-  if (tid < kBinWomen) c[tid] = 0;
-  __syncthreads();
+  __shared__ int base[kBinWomen];
+  const int tid = threadIdx.x;
+  const int b = (int)blockIdx.x / kSortSplit, part = (int)blockIdx.x % kSortSplit;
   const int total = min(bin_cnt[b * kBinPad], bin_cap);
   const unsigned long long* src = bins + (size_t)b * bin_cap;
   int over = 0;
   constexpr int kLoads = 8;  // entries loaded per thread before any is placed
-  for (int i0 = tid; i0 < total; i0 += (int)blockDim.x * kLoads) {
+  constexpr int kSpan = 1024 * kLoads;
+  for (int i0 = part * kSpan; i0 < total; i0 += kSortSplit * kSpan) {
+    if (tid < kBinWomen) c[tid] = 0;
+    __syncthreads();
     unsigned long long v[kLoads];
+    int loc[kLoads];
 #pragma unroll
     for (int j = 0; j < kLoads; ++j) {
-      const int i = i0 + j * (int)blockDim.x;
+      const int i = i0 + j * 1024 + tid;
       v[j] = i < total ? src[i] : ~0ull;
     }
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j)
+      loc[j] = v[j] != ~0ull ? atomicAdd(&c[(uint32_t)(v[j] >> 32) & (kBinWomen - 1)], 1) : 0;
+    __syncthreads();
+    // One global reservation per woman of the bin for this span's entries.
+    if (tid < kBinWomen && c[tid] > 0) base[tid] = atomicAdd(&cnt[b * kBinWomen + tid], c[tid]);
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kLoads; ++j) {
       if (v[j] == ~0ull) continue;
       const uint32_t w = (uint32_t)(v[j] >> 32);
-      const int slot = atomicAdd(&c[w & (kBinWomen - 1)], 1);
+      const int slot = base[w & (kBinWomen - 1)] + loc[j];
       if (slot < cap) bucket[(size_t)w * cap + slot] = (uint32_t)v[j];
       else over = 1;
     }
+    __syncthreads();  // c and base are reused by the next span
   }
-  __syncthreads();
-  if (tid < kBinWomen && b * kBinWomen + tid < n) cnt[b * kBinWomen + tid] = c[tid];
   if (__ballot(over) && (tid & 63) == 0) atomicOr(&sc->prefix_overflow, 1);
 }
 
@@ -1237,8 +1251,8 @@ hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cn
   const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));
   hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(kBinThreads), 0, s, pref_m, n, K, nbins, bin_cnt, bins, bin_cap,
                      gated, sc);
-  hipLaunchKernelGGL(k_bin_sort, dim3(nbins), dim3(1024), 0, s, bins, bin_cnt, bin_cap, n, cnt, bucket, cap, gated,
-                     sc);
+  hipLaunchKernelGGL(k_bin_sort, dim3(nbins * kSortSplit), dim3(1024), 0, s, bins, bin_cnt, bin_cap, cnt, bucket, cap,
+                     gated, sc);
   return hipGetLastError();
 }
 

@@ -56,10 +56,10 @@ hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint3
                             int gated, Scalars* sc, hipStream_t s);
 // Sparse prefix pass: buckets of (m, k < K) per woman (bins of 256 women with
 // bin_cap entries each and counters kBinCounterStride ints apart in bin_cnt,
-// which must be zero: launch_reset_scalars clears them),
-// then per-woman buckets of cap entries and their counts),
-// then the PR prefix from the women's rows without the n x n rank matrix (or the
-// full rank matrix if a bin or bucket overflowed).
+// then per-woman buckets of cap entries and their counts in cnt; bin_cnt and
+// cnt must be zero: the engine keeps them in one allocation that
+// launch_reset_scalars clears), then the PR prefix from the women's rows without
+// the n x n rank matrix (or the full rank matrix if a bin or bucket overflowed).
 int bin_count(int n);
 constexpr int kBinCounterStride = 32;  // bin_cnt holds 256 * kBinCounterStride ints
 // gated = 1: the kernels do nothing unless the precheck (k_first_choices, queued
