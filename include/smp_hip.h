@@ -144,15 +144,24 @@ typedef struct smp_times {
    * AbsSmpEngine reports; post (result assembly and the copy of the matching to
    * the host) is untimed in the reference and is timed here.  total is all four. */
   double precheck_ms, init_ms, core_ms, post_ms, total_ms;
-  /* Device time of the main kernels (HIP events on the engine's stream). */
+  /* Device time of the main kernels (HIP events on the engine's stream; the
+   * engine records as few as it can, since each event between two kernels
+   * delays the second by several microseconds):
+   *   rank_kernel_ms     the rank pass: the full inversion, or the sparse prefix
+   *                      pass (from its launch to the end of init);
+   *   gather_kernel_ms   the PR prefix gather (full init, or after a bucket
+   *                      overflow of the sparse pass);
+   *   propose_kernel_ms  the proposal kernel and what follows it on the stream
+   *                      until its state is in host memory (the state export,
+   *                      or the counter fold when the GPU finishes alone). */
   double rank_kernel_ms, gather_kernel_ms, propose_kernel_ms;
   /* Host time of the hand-off (copies) and of the host tail loop. */
   double handoff_ms, host_tail_ms;
   double copy_wait_ms;     /* host wait for the PR-prefix copy after the kernel launch */
   double warm_ms;          /* host cache warm-up of that copy                          */
   double resume_ms;        /* proposal launch for chains the host handed back          */
-  double bucket_kernel_ms; /* sparse prefix pass: the bucket kernel (HIP events); the
-                              rank pass after it is in rank_kernel_ms               */
+  double bucket_kernel_ms; /* sparse prefix pass: the bucket kernels (HIP events,
+                              from the precheck's flag copy to the rank pass)     */
   int64_t proposals;       /* sum over men of (final list position + 1)           */
   int64_t atomics;         /* atomicMin attempts made by the proposal kernel(s)   */
   int64_t host_proposals;  /* proposals made by the host tail                     */

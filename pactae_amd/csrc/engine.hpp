@@ -164,9 +164,28 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int blocks_copied = 0, blocks_ready = 0;
   int requeues = 0;  // copy-stream replacements made by check_copy_overlap
   bool state_copied = false;  // control block, partner words and parked list are on the host
-  // ev[0..5]: init/core kernel timing and the tail's state copy; ev[6], ev[7]:
-  // around the bucket kernels; ev[8]: the precheck's flags have reached the host.
+  // Events on the engine's stream.  Each one recorded between two kernels costs
+  // a gap of several microseconds on the queue, so the general path records
+  // only ev[8] (the precheck's flags have reached the host), ev[0] (init's main
+  // pass starts), ev[2] (init done; the prefix copies wait for it) and ev[5]
+  // (the proposal phase is done and its state is in host memory); ev[1], ev[4]
+  // and ev[6] serve the full init, the baseline and master-list paths and
+  // buckets queued without the precheck.  The spans below name the phases they
+  // bound in this run (null: not timed).
   hipEvent_t ev[9] = {};
+  struct Span {
+    hipEvent_t* a = nullptr;
+    hipEvent_t* b = nullptr;
+  };
+  Span span_bucket, span_rank, span_gather, span_propose;
+  hipEvent_t* bucket_ev = nullptr;
+  // Elapsed time of a span into *out (when both events completed); clears it.
+  void take_span(Span& sp, double* out) {
+    float ms = 0;
+    if (sp.a && sp.b && hipEventElapsedTime(&ms, *sp.a, *sp.b) == hipSuccess) *out = ms;
+    (void)hipGetLastError();
+    sp = Span{};
+  }
 
   smp_times times{};
   volatile uint64_t touch_sink = 0;
@@ -338,13 +357,17 @@ struct smp_engine : public pactae::AbsSmpEngine {
     *out = d_match;
     return SMP_OK;
   }
-  // Queues the sparse pass's bucket kernels on the engine's stream, timed by
-  // ev[6] and ev[7].
+  // Queues the sparse pass's bucket kernels on the engine's stream.  Queued
+  // behind the precheck, they are timed from its event (ev[8]) to the init's
+  // (ev[0]); otherwise from ev[6].
   int launch_buckets(int gated) {
-    SMP_HIP_TRY(hipEventRecord(ev[6], stream));
+    bucket_ev = &ev[8];
+    if (!gated) {
+      SMP_HIP_TRY(hipEventRecord(ev[6], stream));
+      bucket_ev = &ev[6];
+    }
     SMP_HIP_TRY(pactae::launch_bucket_prefix(d_pref_m, n, K, d_bin_cnt, d_bins, bin_cap(K), d_bucket_cnt, d_bucket,
                                              bucket_cap(n, K), gated, d_sc, dev.cus, stream));
-    SMP_HIP_TRY(hipEventRecord(ev[7], stream));
     bins_pending = true;
     return SMP_OK;
   }
@@ -528,6 +551,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // -- precheck (smp_engine_bamboo.h:25-33) ---------------------------------
   int IsPerfect(bool* perfect) override {
     times = smp_times{};
+    span_bucket = span_rank = span_gather = span_propose = Span{};
     have_result = false;
     master = false;
     matrices_valid = false;
@@ -585,21 +609,20 @@ struct smp_engine : public pactae::AbsSmpEngine {
                                              /*nontemporal=*/1));
       rank_built = true;
       times.rank_matrix = 1;
-      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
+      span_rank = {&ev[0], &ev[2]};
       return SMP_OK;
     }
     if (master) {
       // Master list: the inverse of the shared list (in d_seen, free after the
-      // precheck) and, when validating, a permutation check of the women's rows;
-      // neither the rank matrix nor the PR prefix is needed.
+      // precheck); neither the rank matrix nor the PR prefix is needed.  (The
+      // women's rows were checked by the precheck's master-list scan.)
       blocks_copied = 0;
       SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
       SMP_HIP_TRY(pactae::launch_master_inverse(d_pref_m, n, d_seen, d_sc, stream));
-      // (The women's rows were checked by the precheck's master-list scan.)
-      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
+      span_rank = {&ev[0], &ev[2]};
       return SMP_OK;
     }
     const int chunk = invert_chunk();
@@ -611,18 +634,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
       // Buckets of (man, position) per woman over the first K positions (queued
       // by the precheck unless it was off or saw all first choices equal), then
       // one pass over the women's rows that writes the PR prefix directly.  If a
-      // bucket overflowed, that pass writes the full rank matrix instead and the
-      // gated gather builds the prefix from it.
+      // bucket overflowed, that pass writes the full rank matrix instead; the
+      // proposal kernel then does nothing, and CoreProc gathers the prefix from
+      // the matrix and runs the proposal phase again.
       const int cap = bucket_cap(n, K);
       if (!bins_pending) {
         if (int st = launch_buckets(/*gated=*/0)) return st;
       }
       bins_pending = false;
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
+      span_bucket = {bucket_ev, &ev[0]};
       SMP_HIP_TRY(pactae::launch_rank_prefix(d_pref_w, d_rank_w, n, chunk, opts.validate, d_bucket_cnt, d_bucket, cap,
                                              d_pr, d_sc, stream));
-      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, /*gated=*/1, d_sc, stream));
+      span_rank = {&ev[0], &ev[2]};
     } else {
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
       SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, chunk, opts.validate, d_sc, stream,
@@ -630,16 +654,25 @@ struct smp_engine : public pactae::AbsSmpEngine {
       rank_built = true;
       times.rank_matrix = 1;
       SMP_HIP_TRY(hipEventRecord(ev[1], stream));
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, /*gated=*/0, d_sc, stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
+      span_rank = {&ev[0], &ev[1]};
+      span_gather = {&ev[1], &ev[2]};
     }
+    // No wait here: the proposal kernel does nothing when init flagged malformed
+    // lists, and the error is reported after it.
+    return end_init();
+  }
+
+  // The end of init (ev[2]) and, for the host tail, the PR prefix copied to the
+  // host beside the proposal kernel: one 32-position block per copy (block 0,
+  // where nearly all late proposals fall, first; the others alternate between
+  // two streams), each with an event so the host can cache a block as soon as it
+  // lands.
+  int end_init() {
     SMP_HIP_TRY(hipEventRecord(ev[2], stream));
     blocks_copied = 0;
     blocks_ready = 0;
     if (T > 0 && opts.prefix_copy == 0) {
-      // PR prefix to the host, overlapped with the proposal kernel, one
-      // 32-position block per copy (block 0, where nearly all late proposals fall,
-      // first; the others alternate between two streams), each with an event so
-      // the host can cache a block as soon as it lands.
       const size_t blk = (size_t)n * pactae::kPrBlock;
       const int nb = (K + pactae::kPrBlock - 1) / pactae::kPrBlock;
       SMP_HIP_TRY(hipStreamWaitEvent(copy_stream, ev[2], 0));
@@ -652,8 +685,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
         blocks_copied = b + 1;
       }
     }
-    // No wait here: the proposal kernel does nothing when init flagged malformed
-    // lists, and the error is reported after it.
     return SMP_OK;
   }
 
@@ -682,28 +713,61 @@ struct smp_engine : public pactae::AbsSmpEngine {
     host_finished = false;
     const int grid = propose_grid();
     if (opts.schedule == SMP_SCHEDULE_MW_CAS) {
-      SMP_HIP_TRY(hipEventRecord(ev[3], stream));
       SMP_HIP_TRY(pactae::launch_mw_cas(d_pref_m, d_pref_w, d_rank_w, d_seen, d_cas_next, n, d_sc, stream));
       SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+      span_propose = {&ev[2], &ev[4]};
       if (int st = read_scalars()) return st;
       if (int st = check_device_error("MW-CAS walk")) return st;
       times.handoff_free = 0;
       return SMP_OK;
     }
     if (master) {
-      SMP_HIP_TRY(hipEventRecord(ev[3], stream));
       SMP_HIP_TRY(pactae::launch_propose_master(d_pref_w, d_seen, d_partner, n, d_sc, grid, stream));
       SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+      span_propose = {&ev[2], &ev[4]};
       if (int st = read_scalars()) return st;
       if (int st = check_device_error("women-proposing walk")) return st;
       times.handoff_free = 0;
       return SMP_OK;
     }
-    SMP_HIP_TRY(hipEventRecord(ev[3], stream));
+    for (int run = 0;; ++run) {
+      if (int st = propose_phase(grid)) return st;
+      if (run == 0 && sparse && !rank_built && h_sc->prefix_overflow != 0) {
+        // A bucket overflowed: the sparse pass wrote the rank matrix instead of
+        // the prefix and the proposal kernel did nothing.  Gather the prefix
+        // from the matrix and run the phase again.
+        rank_built = true;
+        times.rank_matrix = 1;
+        sparse_overflowed = true;
+        take_span(span_rank, &times.rank_kernel_ms);
+        SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+        SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
+        span_gather = {&ev[1], &ev[2]};
+        if (int st = end_init()) return st;
+        continue;
+      }
+      break;
+    }
+    const int handoff = h_sc->parked_count;
+    times.handoff_free = handoff;
+    if (handoff > 0) {
+      if (int st = host_tail()) return st;
+    }
+    return SMP_OK;
+  }
+
+  // The persistent proposal kernel from the start state (or resume list) and
+  // what follows it on the engine's stream, up to the point where the host has
+  // the control block (and, with a hand-off, the partner words and parked
+  // chains).  ev[5] marks the end of the phase on the device.
+  int propose_phase(int grid) {
     SMP_HIP_TRY(pactae::launch_propose(propose_args(T, nullptr, 0), grid, stream));
-    SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+    span_propose = {&ev[2], &ev[5]};
     state_copied = false;
-    if (T == 0) SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
+    if (T == 0) {
+      SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[5], stream));
+    }
     if (T > 0 && opts.prefix_copy >= 1) {
       // Everything the host tail needs follows the kernel on the engine's own
       // stream: first the small state (control block, partner words, parked
@@ -750,7 +814,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       // later blocks only while the kernel is still running.
       bool kernel_done = false;
       for (int b = 0; b < blocks_copied && !kernel_done; ++b) {
-        if (b > 0 && hipEventQuery(ev[4]) == hipSuccess) break;
+        if (b > 0 && hipEventQuery(ev[5]) == hipSuccess) break;
         auto t0 = Clock::now();
         SMP_HIP_TRY(wait_event(ev_blk[b]));
         times.copy_wait_ms += ms_since(t0);
@@ -769,16 +833,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
       if (int st = read_scalars()) return st;
       if (int st = check_device_error("proposal kernel")) return st;
     }
-    if (sparse && h_sc->prefix_overflow != 0) {  // the sparse pass fell back to the full matrix
-      rank_built = true;
-      times.rank_matrix = 1;
-      sparse_overflowed = true;
-    }
-    const int handoff = h_sc->parked_count;
-    times.handoff_free = handoff;
-    if (handoff > 0) {
-      if (int st = host_tail()) return st;
-    }
     return SMP_OK;
   }
 
@@ -791,7 +845,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     const size_t words = blk / 2, chunk = 32768;
     uint64_t acc = 0;
     for (size_t c = 0; c < words; c += chunk) {
-      if (stop_when_done && hipEventQuery(ev[4]) == hipSuccess) {
+      if (stop_when_done && hipEventQuery(ev[5]) == hipSuccess) {
         touch_sink = acc;
         return false;
       }
@@ -978,7 +1032,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     float copy_ms = 0, kern_ms = 0;
     if (hipEventQuery(ev_blk[0]) != hipSuccess) return;
     if (hipEventElapsedTime(&copy_ms, ev[2], ev_blk[0]) != hipSuccess) return;
-    if (hipEventElapsedTime(&kern_ms, ev[2], ev[4]) != hipSuccess) return;
+    if (hipEventElapsedTime(&kern_ms, ev[2], ev[5]) != hipSuccess) return;
     if (kern_ms > 0.12f && copy_ms >= kern_ms && requeues < 8) {
       (void)hipStreamSynchronize(copy_stream);
       (void)hipStreamSynchronize(copy_stream2);
@@ -1015,7 +1069,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     t->rank_ms = ms_since(t0);
     if (init_pr) {
       t0 = Clock::now();
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, /*gated=*/0, d_sc, stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
       SMP_HIP_TRY(wait_stream(stream));
       t->pr_ms = ms_since(t0);
@@ -1051,13 +1105,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.prefix_cols = K;
     times.tail_threshold = T;
     if (!times.is_perfect && T > 0 && blocks_copied > 0 && opts.prefix_copy == 0) check_copy_overlap();
-    if (!times.is_perfect) {
-      float ms = 0;
-      if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) times.rank_kernel_ms = ms;
-      if (sparse && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess) times.bucket_kernel_ms = ms;
-      if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) times.gather_kernel_ms = ms;
-      if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess) times.propose_kernel_ms = ms;
-    }
+    take_span(span_bucket, &times.bucket_kernel_ms);
+    take_span(span_rank, &times.rank_kernel_ms);
+    take_span(span_gather, &times.gather_kernel_ms);
+    take_span(span_propose, &times.propose_kernel_ms);
     have_result = true;
     return SMP_OK;
   }

@@ -123,7 +123,8 @@ __global__ void k_first_match(const int32_t* __restrict__ pref_m, int n, int32_t
 // pr[m, k] = (w, rank) straight into the column-blocked prefix.  The n x n rank
 // matrix is then never written (2n^2 bytes saved) and the separate gather is not
 // needed.  If some bucket overflowed (sc->prefix_overflow), every row is written
-// out in full as with SP = 0 and k_gather_pr (gated) builds the prefix instead.
+// out in full as with SP = 0, and the engine builds the prefix with k_gather_pr
+// once the host has seen the flag.
 // Measured at n = 20,000: 1024 > 512 ≈ 256 threads per row for the full
 // inversion; for the sparse pass 1024 x 8 loads ≈ 768 x 7 < 512 x 10 < 640 x 8.
 constexpr int kInvBlock = 1024;
@@ -393,10 +394,7 @@ constexpr int kGatherCols = 256;
 __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ pref_m,
                                                    const uint16_t* __restrict__ rank_w,
                                                    uint32_t* __restrict__ pr, int n, int K,
-                                                   int tiles, int per_xcd, int gated, Scalars* sc) {
-  // gated: the fallback after a sparse pass, needed only if a bucket overflowed.
-  if (gated && __hip_atomic_load(&sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-    return;
+                                                   int tiles, int per_xcd, Scalars* sc) {
   const int b = blockIdx.x;
   const int t = (b & 7) * per_xcd + (b >> 3);
   if (t >= tiles) return;
@@ -587,10 +585,15 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   if (__hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
     next_init = total;  // init found malformed lists: do nothing
   }
-  // After a sparse prefix pass only positions < K have ranks (unless it fell back
-  // to the full rank matrix).
-  const bool have_rank =
-      !a.rank_sparse || __hip_atomic_load(&a.sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+  // After a sparse prefix pass only positions < K have ranks.  If a bucket
+  // overflowed, the pass wrote the rank matrix instead of the prefix, which the
+  // gather has not built yet: the kernel then does nothing at all (not even the
+  // `active` count), and the host gathers and launches it again.
+  if (a.rank_sparse && __hip_atomic_load(&a.sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    if (lane == 0) a.wstats[2 * gwave] = a.wstats[2 * gwave + 1] = 0;
+    return;
+  }
+  const bool have_rank = !a.rank_sparse;
   for (;;) {
     if (m < 0) {
       if (next_init >= total) {
@@ -1290,14 +1293,14 @@ hipError_t set_invert_lds_limit(int bytes) {
 }
 
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            int gated, Scalars* sc, hipStream_t s) {
+                            Scalars* sc, hipStream_t s) {
   const int men_blocks = (n + kGatherMen - 1) / kGatherMen;
   const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
   const long long tiles = (long long)men_blocks * col_tiles;
   if (tiles > (1ll << 30)) return hipErrorInvalidValue;
   const int per_xcd = (int)((tiles + 7) / 8);
   hipLaunchKernelGGL(k_gather_pr, dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
-                     (int)tiles, per_xcd, gated, sc);
+                     (int)tiles, per_xcd, sc);
   return hipGetLastError();
 }
 

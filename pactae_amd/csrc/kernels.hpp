@@ -20,9 +20,10 @@ struct ProposeArgs {
   int poll;                    // steps between reads of the shared `active` counter
   const int2* starts;          // resume mode: (man, position) pairs to start from;
   int nstarts;                 //   null = every man from position 0
-  int rank_sparse;             // 1: rank_w exists only if sc->prefix_overflow (sparse
-                               //   prefix pass); a chain reaching position K is then
-                               //   parked at K for the host, which builds rank_w
+  int rank_sparse;             // 1: after a sparse prefix pass, rank_w does not exist;
+                               //   a chain reaching position K is parked at K for the
+                               //   host, which builds rank_w.  If the pass overflowed
+                               //   (sc->prefix_overflow), the kernel does nothing.
 };
 
 hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s);
@@ -51,9 +52,8 @@ hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipS
 hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int chunk, int validate,
                               Scalars* sc, hipStream_t s, int nontemporal);
 hipError_t set_invert_lds_limit(int bytes);
-// gated = 1: runs only if a sparse prefix pass overflowed (sc->prefix_overflow).
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            int gated, Scalars* sc, hipStream_t s);
+                            Scalars* sc, hipStream_t s);
 // Sparse prefix pass: buckets of (m, k < K) per woman (bins of 256 women with
 // bin_cap entries each and counters kBinCounterStride ints apart in bin_cnt,
 // then per-woman buckets of cap entries and their counts in cnt; bin_cnt and

@@ -118,8 +118,10 @@ def test_sparse_prefix_matrices(gpu, n):
 def test_sparse_overflow_falls_back(gpu, oracle):
     """All men share one list and the precheck is off: each of the first K women
     of that list is named by all n men, far more than a bucket holds, so the
-    sparse pass writes the full rank matrix and the gated gather builds the
-    prefix.  Result: the serial-dictatorship matching with n(n+1)/2 proposals."""
+    sparse pass writes the full rank matrix, the proposal kernel stands down, and
+    the engine gathers the prefix and runs the proposal phase again (with the
+    GPU finishing alone, and with a host tail).  Result: the serial-dictatorship
+    matching with n(n+1)/2 proposals."""
     smp = gpu
     n = 1000
     pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=31)
