@@ -110,8 +110,12 @@ typedef struct smp_opts {
                              costs failed atomics, never correctness)             */
   int32_t poll_interval;  /* proposal-kernel steps between reads of the shared
                              active-wave counter that triggers the hand-off; 0 = 16 */
-  int32_t host_spin;      /* 1 = the host polls for device completion, 0 = it blocks
-                             in the runtime (default; polling was measured slower) */
+  int32_t host_spin;      /* 1 = the host polls the runtime for device completion,
+                             0 = it blocks in the runtime (default; polling was
+                             measured slower).  The two hand-offs on the critical
+                             path (after the precheck and after the proposal
+                             kernel) are always polled, through flag words the
+                             device writes into pinned host memory            */
   int32_t prefix_copy;    /* how the PR prefix reaches the host for the tail:
                              0 = copied beside the proposal kernel on copy streams
                              (default, measured fastest), 1 = copied after it on

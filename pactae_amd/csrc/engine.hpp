@@ -153,6 +153,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
   unsigned long long* h_partner_dev = nullptr;
   Scalars* h_sc_dev = nullptr;
   int2* h_parked_dev = nullptr;
+  // Pinned flag words the device writes last when it hands data to the host:
+  // [kFlagPrecheck] by k_publish_scalars, [kFlagExport] by k_export_state.  The
+  // host waits for the sequence number it queued (wait_flag) instead of for an
+  // event: a flag is seen about as soon as it is written, and unlike a copy
+  // plus an event it leaves no gap on the queue.
+  static constexpr int kFlagPrecheck = 0, kFlagExport = 1;
+  unsigned* h_flags = nullptr;
+  unsigned* h_flags_dev = nullptr;
+  unsigned flag_seq = 0;      // last sequence number queued (either flag)
+  unsigned export_seq = 0;    // the one the current export writes
   bool match_on_host_only = false;  // the host tail built the matching; d_match not yet mirrored
   int32_t* h_match = nullptr;  // pinned result (man -> woman)
   int2* d_starts = nullptr;     // chains handed back by the host tail
@@ -166,12 +176,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
   bool state_copied = false;  // control block, partner words and parked list are on the host
   // Events on the engine's stream.  Each one recorded between two kernels costs
   // a gap of several microseconds on the queue, so the general path records
-  // only ev[8] (the precheck's flags have reached the host), ev[0] (init's main
-  // pass starts), ev[2] (init done; the prefix copies wait for it) and ev[5]
-  // (the proposal phase is done and its state is in host memory); ev[1], ev[4]
-  // and ev[6] serve the full init, the baseline and master-list paths and
-  // buckets queued without the precheck.  The spans below name the phases they
-  // bound in this run (null: not timed).
+  // only ev[0] (init's main pass starts), ev[2] (init done; the prefix copies
+  // wait for it) and ev[5] (behind the state export, for timing only: the host
+  // waits for the export's flag word); ev[1], ev[4] and ev[6] serve the full
+  // init, the baseline and master-list paths and buckets queued without the
+  // precheck.  The spans below name the phases they bound in this run (null:
+  // not timed).
   hipEvent_t ev[9] = {};
   struct Span {
     hipEvent_t* a = nullptr;
@@ -240,6 +250,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipHostFree(h_partner);
     (void)hipHostFree(h_sc);
     (void)hipHostFree(h_parked);
+    (void)hipHostFree(h_flags);
+    h_flags = h_flags_dev = nullptr;
     d_rank_w = nullptr;
     d_pr = nullptr;
     d_partner = nullptr;
@@ -290,18 +302,21 @@ struct smp_engine : public pactae::AbsSmpEngine {
         hipHostMalloc(&h_partner, (size_t)n * sizeof(unsigned long long), pactae::kHostAllocFlags) !=
             hipSuccess ||
         hipHostMalloc(&h_sc, sizeof(Scalars), pactae::kHostAllocFlags) != hipSuccess ||
-        hipHostMalloc(&h_parked, (size_t)n * sizeof(int2), pactae::kHostAllocFlags) != hipSuccess) {
+        hipHostMalloc(&h_parked, (size_t)n * sizeof(int2), pactae::kHostAllocFlags) != hipSuccess ||
+        hipHostMalloc(&h_flags, 64, pactae::kHostAllocFlags) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("pinned host allocation failed for n=" + std::to_string(n));
       return SMP_ERR_NOMEM;
     }
+    for (int i = 0; i < 16; ++i) __atomic_store_n(&h_flags[i], 0u, __ATOMIC_RELAXED);
     if (hipError_t e = pactae::set_invert_lds_limit(((invert_chunk() + 7) & ~7) * 2); e != hipSuccess) {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
     if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h_partner_dev), h_partner, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&h_sc_dev), h_sc, 0) != hipSuccess ||
-        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_parked_dev), h_parked, 0) != hipSuccess) {
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_parked_dev), h_parked, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_flags_dev), h_flags, 0) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("pinned hand-off buffers are not visible to the device");
       return SMP_ERR_HIP;
@@ -358,16 +373,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return SMP_OK;
   }
   // Queues the sparse pass's bucket kernels on the engine's stream.  Queued
-  // behind the precheck, they are timed from its event (ev[8]) to the init's
-  // (ev[0]); otherwise from ev[6].
-  int launch_buckets(int gated) {
-    bucket_ev = &ev[8];
-    if (!gated) {
-      SMP_HIP_TRY(hipEventRecord(ev[6], stream));
-      bucket_ev = &ev[6];
+  // behind the precheck (gated) they are not timed (an event there would delay
+  // them); otherwise from ev[6] to the init's ev[0].
+  // stages as in launch_bucket_prefix (1: bin the prefix, 2: sort the bins).
+  int launch_buckets(int gated, int stages = 3) {
+    if (stages & 1) {
+      bucket_ev = nullptr;
+      if (!gated) {
+        SMP_HIP_TRY(hipEventRecord(ev[6], stream));
+        bucket_ev = &ev[6];
+      }
     }
     SMP_HIP_TRY(pactae::launch_bucket_prefix(d_pref_m, n, K, d_bin_cnt, d_bins, bin_cap(K), d_bucket_cnt, d_bucket,
-                                             bucket_cap(n, K), gated, d_sc, dev.cus, stream));
+                                             bucket_cap(n, K), gated, d_sc, dev.cus, stream, stages));
     bins_pending = true;
     return SMP_OK;
   }
@@ -396,6 +414,21 @@ struct smp_engine : public pactae::AbsSmpEngine {
     for (;;) {
       hipError_t q = hipEventQuery(e);
       if (q != hipErrorNotReady) return q;
+    }
+  }
+  bool flag_seen(int f, unsigned seq) const { return __atomic_load_n(&h_flags[f], __ATOMIC_ACQUIRE) == seq; }
+  // Polls flag f for seq (a read of host memory).  Now and then it asks the
+  // runtime about the stream, so that a failed launch or a device error ends
+  // the wait; a stream that drained without the flag is an error too.
+  hipError_t wait_flag(int f, unsigned seq) {
+    for (unsigned i = 1;; ++i) {
+      if (flag_seen(f, seq)) return hipSuccess;
+      if ((i & 4095) == 0) {
+        const hipError_t q = hipStreamQuery(stream);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return q;
+        return flag_seen(f, seq) ? hipSuccess : hipErrorUnknown;
+      }
     }
   }
 
@@ -574,14 +607,20 @@ struct smp_engine : public pactae::AbsSmpEngine {
       return SMP_OK;
     }
     SMP_HIP_TRY(pactae::launch_first_choices(d_pref_m, n, d_seen, d_sc, stream));
-    SMP_HIP_TRY(hipMemcpyAsync(h_sc, d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
-    SMP_HIP_TRY(hipEventRecord(ev[8], stream));
+    // The sparse pass's bucket kernels go right behind the precheck, gated on its
+    // flags on the device, so they run while the host reads the flags.  The
+    // flags are published between the two: the host's launches of the publish
+    // and of the sort then overlap the binning instead of leaving the queue
+    // idle, and the sort still covers the host's reaction.
     if (may_sparse) {
-      // The sparse pass's bucket kernels go right behind the precheck, gated on
-      // its flags on the device, so they run while the host reads the flags.
-      if (int st = launch_buckets(/*gated=*/1)) return st;
+      if (int st = launch_buckets(/*gated=*/1, /*stages=*/1)) return st;
     }
-    SMP_HIP_TRY(wait_event(ev[8]));
+    const unsigned seq = ++flag_seq;
+    SMP_HIP_TRY(pactae::launch_publish_scalars(d_sc, h_sc_dev, h_flags_dev + kFlagPrecheck, seq, stream));
+    if (may_sparse) {
+      if (int st = launch_buckets(/*gated=*/1, /*stages=*/2)) return st;
+    }
+    SMP_HIP_TRY(wait_flag(kFlagPrecheck, seq));
     if (int st = check_device_error("precheck")) return st;
     *perfect = h_sc->dup_first == 0;
     times.is_perfect = *perfect ? 1 : 0;
@@ -773,8 +812,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
       // stream: first the small state (control block, partner words, parked
       // chains, written to host memory by k_export_state), then the PR prefix
       // block by block, each with an event.
+      export_seq = ++flag_seq;
       SMP_HIP_TRY(pactae::launch_export_state(d_sc, d_wstats, grid * 4, d_partner, d_parked, n, h_sc_dev,
-                                              h_partner_dev, h_parked_dev, stream));
+                                              h_partner_dev, h_parked_dev, h_flags_dev + kFlagExport,
+                                              export_seq, stream));
       SMP_HIP_TRY(hipEventRecord(ev[5], stream));
       const size_t blk = (size_t)n * pactae::kPrBlock;
       const int nb = (K + pactae::kPrBlock - 1) / pactae::kPrBlock;
@@ -789,7 +830,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
         blocks_copied = b + 1;
       }
       auto t0 = Clock::now();
-      SMP_HIP_TRY(wait_event(ev[5]));
+      SMP_HIP_TRY(wait_flag(kFlagExport, export_seq));
       times.copy_wait_ms = ms_since(t0);
       state_copied = true;
       if (int st = check_device_error("proposal kernel")) return st;
@@ -806,15 +847,17 @@ struct smp_engine : public pactae::AbsSmpEngine {
       // The state the host tail reads (control block, partner words, parked
       // chains) is written to host memory by k_export_state right behind the
       // kernel on its own stream, so that one wait covers all of it.
+      export_seq = ++flag_seq;
       SMP_HIP_TRY(pactae::launch_export_state(d_sc, d_wstats, grid * 4, d_partner, d_parked, n, h_sc_dev,
-                                              h_partner_dev, h_parked_dev, stream));
+                                              h_partner_dev, h_parked_dev, h_flags_dev + kFlagExport,
+                                              export_seq, stream));
       SMP_HIP_TRY(hipEventRecord(ev[5], stream));
       // While the kernel runs: pull the PR prefix blocks into the host caches as
       // they land (a DMA-written buffer starts out in DRAM only).  Block 0 always;
       // later blocks only while the kernel is still running.
       bool kernel_done = false;
       for (int b = 0; b < blocks_copied && !kernel_done; ++b) {
-        if (b > 0 && hipEventQuery(ev[5]) == hipSuccess) break;
+        if (b > 0 && flag_seen(kFlagExport, export_seq)) break;
         auto t0 = Clock::now();
         SMP_HIP_TRY(wait_event(ev_blk[b]));
         times.copy_wait_ms += ms_since(t0);
@@ -824,7 +867,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
         times.warm_ms += ms_since(t0);
       }
       auto t0 = Clock::now();
-      SMP_HIP_TRY(wait_event(ev[5]));
+      SMP_HIP_TRY(wait_flag(kFlagExport, export_seq));
       times.copy_wait_ms += ms_since(t0);
       state_copied = true;
       if (int st = check_device_error("proposal kernel")) return st;
@@ -845,7 +888,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     const size_t words = blk / 2, chunk = 32768;
     uint64_t acc = 0;
     for (size_t c = 0; c < words; c += chunk) {
-      if (stop_when_done && hipEventQuery(ev[5]) == hipSuccess) {
+      if (stop_when_done && flag_seen(kFlagExport, export_seq)) {
         touch_sink = acc;
         return false;
       }
@@ -1030,9 +1073,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // time), the copy streams are replaced, which moves them to other queues.
   void check_copy_overlap() {
     float copy_ms = 0, kern_ms = 0;
-    if (hipEventQuery(ev_blk[0]) != hipSuccess) return;
-    if (hipEventElapsedTime(&copy_ms, ev[2], ev_blk[0]) != hipSuccess) return;
-    if (hipEventElapsedTime(&kern_ms, ev[2], ev[5]) != hipSuccess) return;
+    if (hipEventQuery(ev_blk[0]) != hipSuccess || hipEventQuery(ev[5]) != hipSuccess ||
+        hipEventElapsedTime(&copy_ms, ev[2], ev_blk[0]) != hipSuccess ||
+        hipEventElapsedTime(&kern_ms, ev[2], ev[5]) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
     if (kern_ms > 0.12f && copy_ms >= kern_ms && requeues < 8) {
       (void)hipStreamSynchronize(copy_stream);
       (void)hipStreamSynchronize(copy_stream2);

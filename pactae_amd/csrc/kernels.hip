@@ -59,6 +59,7 @@ __global__ void k_reset_scalars(Scalars* sc, int active, int* z0, size_t n0, int
     sc->dup_first = 0;
     sc->first_differs = 0;
     sc->not_master = 0;
+    sc->export_blocks = 0;
     sc->atomics = 0;
     sc->steps = 0;
     sc->proposals = 0;
@@ -742,7 +743,8 @@ __global__ __launch_bounds__(256) void k_fold_wstats(const unsigned long long* _
 __global__ __launch_bounds__(256) void k_export_state(Scalars* sc, const unsigned long long* __restrict__ wstats,
                                                       int waves, const unsigned long long* __restrict__ partner,
                                                       const int2* __restrict__ parked, int n, Scalars* h_sc,
-                                                      unsigned long long* h_partner, int2* h_parked) {
+                                                      unsigned long long* h_partner, int2* h_parked,
+                                                      unsigned* h_flag, unsigned seq) {
   if (blockIdx.x == 0) {
     __shared__ unsigned long long red[2][4];
     unsigned long long at = 0, st = 0;
@@ -766,6 +768,26 @@ __global__ __launch_bounds__(256) void k_export_state(Scalars* sc, const unsigne
   for (int i = tid; i < n; i += nth) h_partner[i] = partner[i];
   const int pc = min(sc->parked_count, n);
   for (int i = tid; i < pc; i += nth) h_parked[i] = parked[i];
+  // Publish: each wave's host writes are released at system scope, then the
+  // last block to finish writes seq into the host's flag word, so a host that
+  // sees seq sees everything above.
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&sc->export_blocks, 1) == (int)gridDim.x - 1) {
+    sc->export_blocks = 0;
+    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Precheck publish: the control block into pinned host memory, then seq into
+// the host's flag word with a system-scope release.  Queued behind the
+// precheck, it stands in for a copy and an event, each of which would leave
+// the queue idle for several microseconds before the kernels behind it.
+__global__ void k_publish_scalars(const Scalars* __restrict__ sc, Scalars* h_sc, unsigned* h_flag, unsigned seq) {
+  if (threadIdx.x == 0) {
+    *h_sc = *sc;
+    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1246,16 +1268,19 @@ int bin_count(int n) { return (n + kBinWomen - 1) / kBinWomen; }
 
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
                                 int bin_cap, int* cnt, uint32_t* bucket, int cap, int gated, Scalars* sc, int cus,
-                                hipStream_t s) {
+                                hipStream_t s, int stages) {
   const int nbins = bin_count(n);
   if (nbins > kBinWomen) return hipErrorInvalidValue;
-  const unsigned long long total = (unsigned long long)n * (unsigned long long)K;
-  const long long chunks = (long long)((total + kBinEntries - 1) / kBinEntries);
-  const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));
-  hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(kBinThreads), 0, s, pref_m, n, K, nbins, bin_cnt, bins, bin_cap,
-                     gated, sc);
-  hipLaunchKernelGGL(k_bin_sort, dim3(nbins * kSortSplit), dim3(1024), 0, s, bins, bin_cnt, bin_cap, cnt, bucket, cap,
-                     gated, sc);
+  if (stages & 1) {
+    const unsigned long long total = (unsigned long long)n * (unsigned long long)K;
+    const long long chunks = (long long)((total + kBinEntries - 1) / kBinEntries);
+    const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));
+    hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(kBinThreads), 0, s, pref_m, n, K, nbins, bin_cnt, bins,
+                       bin_cap, gated, sc);
+  }
+  if (stages & 2)
+    hipLaunchKernelGGL(k_bin_sort, dim3(nbins * kSortSplit), dim3(1024), 0, s, bins, bin_cnt, bin_cap, cnt, bucket,
+                       cap, gated, sc);
   return hipGetLastError();
 }
 
@@ -1306,10 +1331,16 @@ hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint3
 
 hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, int waves,
                                const unsigned long long* partner, const int2* parked, int n, Scalars* h_sc,
-                               unsigned long long* h_partner, int2* h_parked, hipStream_t s) {
+                               unsigned long long* h_partner, int2* h_parked, unsigned* h_flag, unsigned seq,
+                               hipStream_t s) {
   const int grid = std::max(1, std::min(64, (n + 255) / 256));
   hipLaunchKernelGGL(k_export_state, dim3(grid), dim3(256), 0, s, sc, wstats, waves, partner, parked, n, h_sc,
-                     h_partner, h_parked);
+                     h_partner, h_parked, h_flag, seq);
+  return hipGetLastError();
+}
+
+hipError_t launch_publish_scalars(const Scalars* sc, Scalars* h_sc, unsigned* h_flag, unsigned seq, hipStream_t s) {
+  hipLaunchKernelGGL(k_publish_scalars, dim3(1), dim3(64), 0, s, sc, h_sc, h_flag, seq);
   return hipGetLastError();
 }
 

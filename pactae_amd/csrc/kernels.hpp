@@ -64,19 +64,24 @@ int bin_count(int n);
 constexpr int kBinCounterStride = 32;  // bin_cnt holds 256 * kBinCounterStride ints
 // gated = 1: the kernels do nothing unless the precheck (k_first_choices, queued
 // before them) found two men sharing a first choice and not all first choices equal.
+// stages: 1 = k_bin_prefix, 2 = k_bin_sort, 3 = both (in that order).
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
                                 int bin_cap, int* cnt, uint32_t* bucket, int cap, int gated, Scalars* sc, int cus,
-                                hipStream_t s);
+                                hipStream_t s, int stages = 3);
 hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int chunk, int validate,
                               const int* cnt, const uint32_t* bucket, int cap, uint32_t* pr, Scalars* sc,
                               hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);
 // Folds the proposal waves' counters, then writes the control block, the partner
-// words and the parked chains into host memory (device-visible pinned pointers).
+// words and the parked chains into host memory (device-visible pinned pointers)
+// and, once all of it is out, seq into *h_flag (pinned).
 hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, int waves,
                                const unsigned long long* partner, const int2* parked, int n, Scalars* h_sc,
-                               unsigned long long* h_partner, int2* h_parked, hipStream_t s);
+                               unsigned long long* h_partner, int2* h_parked, unsigned* h_flag, unsigned seq,
+                               hipStream_t s);
+// The control block into pinned host memory, then seq into *h_flag (pinned).
+hipError_t launch_publish_scalars(const Scalars* sc, Scalars* h_sc, unsigned* h_flag, unsigned seq, hipStream_t s);
 hipError_t launch_post(const unsigned long long* partner, int n, int32_t* match, Scalars* sc,
                        hipStream_t s);
 

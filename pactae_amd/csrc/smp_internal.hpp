@@ -70,7 +70,7 @@ struct Scalars {
   int dup_first;     // precheck: 1 if two men share a first choice
   int first_differs; // precheck: 1 if two men have different first choices
   int not_master;    // precheck: 1 if some man's list differs from man 0's
-  int pad1;
+  int export_blocks; // k_export_state: blocks finished (the last one resets it)
   unsigned long long atomics;    // atomicMin attempts
   unsigned long long steps;      // segment scans
   unsigned long long proposals;  // sum over women of (partner's list position + 1)
