@@ -1333,7 +1333,9 @@ hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, in
                                const unsigned long long* partner, const int2* parked, int n, Scalars* h_sc,
                                unsigned long long* h_partner, int2* h_parked, unsigned* h_flag, unsigned seq,
                                hipStream_t s) {
-  const int grid = std::max(1, std::min(64, (n + 255) / 256));
+  // Few blocks: each one's wave-level system-scope fences write back the L2, so
+  // their number, not the copy, sets the kernel's time.
+  const int grid = std::max(1, std::min(16, (n + 255) / 256));
   hipLaunchKernelGGL(k_export_state, dim3(grid), dim3(256), 0, s, sc, wstats, waves, partner, parked, n, h_sc,
                      h_partner, h_parked, h_flag, seq);
   return hipGetLastError();
