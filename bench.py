@@ -319,7 +319,8 @@ def main() -> int:
     if last["sparse_init"] and not last["rank_matrix"]:
         # Sparse prefix pass: 4 B read per women's list entry, plus per prefix entry
         # 4 B of bucket read and 4 B of PR prefix written.
-        rank_kernel = ("k_rank_prefix", "k_invert_rows<4,1,1> (sparse prefix pass over the women's rows)")
+        name = ("k_rank_prefix_row<1024,5>" if n % 4 == 0 and n <= 20480 else "k_invert_rows<4,1,1>")
+        rank_kernel = ("k_rank_prefix", f"{name} (sparse prefix pass over the women's rows)")
         rank_bytes = 4 * n * n + 8 * n * K
     else:
         # Full rank matrix: 4 B read from pref_w + 2 B written to the 16-bit rank matrix per entry.
