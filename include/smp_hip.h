@@ -17,6 +17,11 @@
  *                      src/smp/smp_engine_bamboo.cu:6-31 (device allocation + H2D of
  *                      both lists; SmpObj::CreateFromFlatLists include/smp/smp.h:25-29)
  *   smp_create_device  same constructor, for lists already resident in HBM (borrowed)
+ *   smp_create_ranked, smp_create_device_ranked
+ *                      same constructor with the women's inverse rank matrix as input,
+ *                      as the rank_w the reference builds in InitProc
+ *                      (F9RankMatrixInitKernel src/smp/kernel.cu:672-686) but derived by
+ *                      the caller; init keeps only the gather (kernel.cu:688-709)
  *   smp_exec           AbsSmpEngine::FindStableMatching include/smp/smp_engine_abs.h:10-36
  *                      (precheck IsPerfect smp_engine_bamboo.h:25-33, InitProc
  *                      smp_engine_bamboo.cu:33-106, CoreProc :108-287, PostProc :289-306)
@@ -206,6 +211,18 @@ int smp_create(const int32_t* pref_m, const int32_t* pref_w, int32_t n, const sm
 /* Borrows lists already in HBM; they must outlive the engine and stay unchanged. */
 int smp_create_device(const int32_t* d_pref_m, const int32_t* d_pref_w, int32_t n,
                       const smp_opts* opts, smp_engine** out);
+/* The men's lists and the women's inverse rank matrix derived by the caller
+ * (rank_w[w*n + m] = position of man m in woman w's list, int32, row-major n x n)
+ * instead of the women's lists.  Init is then the PR prefix gather alone.  Each
+ * rank is range-checked where it is read; the rank rows are not checked for
+ * being permutations (validate covers the women's lists only).  The women's
+ * lists are absent, so the master-list path is not taken (the general walk
+ * answers instead) and the MW-CAS schedule is rejected. */
+int smp_create_ranked(const int32_t* pref_m, const int32_t* rank_w, int32_t n, const smp_opts* opts,
+                      smp_engine** out);
+/* The same for arrays already in HBM (borrowed, as smp_create_device). */
+int smp_create_device_ranked(const int32_t* d_pref_m, const int32_t* d_rank_w, int32_t n,
+                             const smp_opts* opts, smp_engine** out);
 /* One full solve: precheck + init + core + post.  Synchronous: every device and
  * host-tail step has finished when it returns.  May be called repeatedly. */
 int smp_exec(smp_engine* e);

@@ -223,6 +223,8 @@ SIGNATURES = [
     ("smp_default_opts", None, [ctypes.POINTER(SmpOpts)]),
     ("smp_create", ctypes.c_int, [i32p, i32p, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
     ("smp_create_device", ctypes.c_int, [vp, vp, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
+    ("smp_create_ranked", ctypes.c_int, [i32p, i32p, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
+    ("smp_create_device_ranked", ctypes.c_int, [vp, vp, i32, ctypes.POINTER(SmpOpts), ctypes.POINTER(vp)]),
     ("smp_exec", ctypes.c_int, [vp]),
     ("smp_result", ctypes.c_int, [vp, i32p]),
     ("smp_result_device", ctypes.c_int, [vp, ctypes.POINTER(vp)]),

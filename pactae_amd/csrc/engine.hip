@@ -108,6 +108,63 @@ int smp_create_device(const int32_t* d_pref_m, const int32_t* d_pref_w, int32_t 
   return SMP_OK;
 }
 
+// Ranks given (the women's inverse rank matrix derived by the caller): no
+// women's lists, so no MW-CAS schedule (it maps ranks back to men through them).
+static int check_ranked(const smp_opts* opts) {
+  if (opts && opts->schedule == SMP_SCHEDULE_MW_CAS) {
+    pactae::set_error("opts: the MW-CAS schedule needs the women's lists, not their ranks");
+    return SMP_ERR_INVALID;
+  }
+  return SMP_OK;
+}
+
+int smp_create_ranked(const int32_t* pref_m, const int32_t* rank_w, int32_t n, const smp_opts* opts,
+                      smp_engine** out) {
+  if (!pref_m || !rank_w) {
+    pactae::set_error("null preference list or rank matrix");
+    return SMP_ERR_INVALID;
+  }
+  if (int st = check_ranked(opts)) return st;
+  smp_engine* e = nullptr;
+  if (int st = make_engine(n, opts, &e)) return st;
+  const size_t bytes = (size_t)n * n * sizeof(int32_t);
+  int32_t *dm = nullptr, *dr = nullptr;
+  if (hipMalloc(&dm, bytes) != hipSuccess || hipMalloc(&dr, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(dm);
+    pactae::set_error("device allocation of the preference lists failed");
+    delete e;
+    return SMP_ERR_NOMEM;
+  }
+  e->d_pref_m = dm;
+  e->d_rank_in = dr;
+  e->owns_lists = true;
+  if (hipMemcpy(dm, pref_m, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dr, rank_w, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    pactae::set_error("H2D copy of the preference lists failed");
+    delete e;
+    return SMP_ERR_HIP;
+  }
+  *out = e;
+  return SMP_OK;
+}
+
+int smp_create_device_ranked(const int32_t* d_pref_m, const int32_t* d_rank_w, int32_t n, const smp_opts* opts,
+                             smp_engine** out) {
+  if (!d_pref_m || !d_rank_w) {
+    pactae::set_error("null device preference list or rank matrix");
+    return SMP_ERR_INVALID;
+  }
+  if (int st = check_ranked(opts)) return st;
+  smp_engine* e = nullptr;
+  if (int st = make_engine(n, opts, &e)) return st;
+  e->d_pref_m = d_pref_m;
+  e->d_rank_in = d_rank_w;
+  e->owns_lists = false;
+  *out = e;
+  return SMP_OK;
+}
+
 int smp_exec(smp_engine* e) {
   if (!e) {
     pactae::set_error("null engine");
@@ -181,9 +238,14 @@ int smp_rank_matrix(smp_engine* e, int32_t* rank_out) {
     pactae::set_error("smp_rank_matrix: the last run built no rank matrix (first-choice or master-list path)");
     return SMP_ERR_STATE;
   }
+  const size_t nn = (size_t)e->n * e->n;
+  if (e->d_rank_in) {  // given as input
+    SMP_HIP_TRY(hipStreamSynchronize(e->stream));
+    SMP_HIP_TRY(hipMemcpy(rank_out, e->d_rank_in, nn * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return SMP_OK;
+  }
   if (int st = e->ensure_rank_matrix()) return st;  // a sparse prefix pass skips it
   SMP_HIP_TRY(hipStreamSynchronize(e->stream));
-  const size_t nn = (size_t)e->n * e->n;
   std::vector<uint16_t> r16(nn);
   SMP_HIP_TRY(hipMemcpy(r16.data(), e->d_rank_w, nn * sizeof(uint16_t), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < nn; ++i) rank_out[i] = (int32_t)r16[i];

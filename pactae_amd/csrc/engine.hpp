@@ -123,8 +123,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
   bool have_result = false;
 
   const int32_t* d_pref_m = nullptr;
-  const int32_t* d_pref_w = nullptr;
+  const int32_t* d_pref_w = nullptr;  // null when the ranks are given (d_rank_in)
   uint16_t* d_rank_w = nullptr;  // women's ranks, 16-bit (n <= SMP_MAX_N = 65535)
+  // A women's rank matrix given as input (int32, rank_in[w * n + m] = position of
+  // man m in woman w's list) instead of the women's lists: init is then the
+  // prefix gather alone.  Owned with the lists when owns_lists.
+  const int32_t* d_rank_in = nullptr;
   uint32_t* d_pr = nullptr;
   unsigned long long* d_partner = nullptr;
   int32_t* d_match = nullptr;
@@ -223,8 +227,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
     if (owns_lists) {
       (void)hipFree(const_cast<int32_t*>(d_pref_m));
       (void)hipFree(const_cast<int32_t*>(d_pref_w));
+      (void)hipFree(const_cast<int32_t*>(d_rank_in));
     }
-    d_pref_m = d_pref_w = nullptr;
+    d_pref_m = d_pref_w = d_rank_in = nullptr;
     (void)hipFree(d_rank_w);
     (void)hipFree(d_pr);
     (void)hipFree(d_partner);
@@ -357,7 +362,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // (men whose prefixes name the same few women, e.g. correlated lists): its
   // buckets would only be built to be thrown away.
   bool use_sparse() const {
-    if (opts.init_mode == SMP_INIT_FULL || K > K_sparse_alloc) return false;
+    if (d_rank_in || opts.init_mode == SMP_INIT_FULL || K > K_sparse_alloc) return false;
     if (opts.init_mode == SMP_INIT_SPARSE) return true;
     return 4ll * K <= n && !sparse_overflowed;
   }
@@ -391,7 +396,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   }
   // The women's rank matrix, if this run has not built it (sparse prefix pass).
   int ensure_rank_matrix() {
-    if (rank_built) return SMP_OK;
+    if (rank_built || d_rank_in) return SMP_OK;
     SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, invert_chunk(), opts.validate, d_sc, stream,
                                            /*nontemporal=*/1));
     rank_built = true;
@@ -534,7 +539,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // engine's buffers: the plain single-instance pipeline, used by the looped
   // baseline of the batched path.
   int run_view(const int32_t* pm, const int32_t* pw, int ni, int32_t* d_out) {
-    if (ni <= 0 || ni > n_alloc) return SMP_ERR_INVALID;
+    if (ni <= 0 || ni > n_alloc || d_rank_in) return SMP_ERR_INVALID;
     d_pref_m = pm;
     d_pref_w = pw;
     sparse_overflowed = false;  // other lists
@@ -626,7 +631,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.is_perfect = *perfect ? 1 : 0;
     // Every man names the same first choice: is every list the same permutation?
     master = false;
-    if (!*perfect && n > 1 && h_sc->first_differs == 0) {
+    // (With ranks given there are no women's lists to walk: the general path.)
+    if (!*perfect && n > 1 && h_sc->first_differs == 0 && !d_rank_in) {
       bins_pending = false;  // the gated bucket kernels found the path closed and did nothing
       // The men's lists compared with man 0's and, ahead of the decision, the
       // women's permutation check the master-list path needs, in one launch.
@@ -668,6 +674,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
     matrices_valid = true;
     sparse = use_sparse();
     times.sparse_init = sparse ? 1 : 0;
+    if (d_rank_in) {
+      // Ranks given: the prefix gather from them is the whole init.
+      SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_in, d_pr, n, K, d_sc, stream));
+      span_gather = {&ev[1], &ev[2]};
+      return end_init();
+    }
     // (The women's partner words were set to unclaimed by the precheck's reset.)
     if (sparse) {
       // Buckets of (man, position) per woman over the first K positions (queued
@@ -733,6 +746,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.pr = d_pr;
     a.pref_m = d_pref_m;
     a.rank_w = d_rank_w;
+    a.rank32 = d_rank_in;
     a.partner = d_partner;
     a.parked = d_parked;
     a.sc = d_sc;
@@ -744,7 +758,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.poll = opts.poll_interval > 0 ? opts.poll_interval : 16;
     a.starts = starts;
     a.nstarts = nstarts;
-    a.rank_sparse = rank_built ? 0 : 1;
+    a.rank_sparse = rank_built || d_rank_in ? 0 : 1;
     return a;
   }
 
@@ -1101,6 +1115,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // the last matrix built (AbsSmpInitEngine::Init, smp_init_abs.h:16-28).
   int init_only(bool init_pr, smp_init_times* t) {
     *t = smp_init_times{};
+    if (d_rank_in) {
+      pactae::set_error("smp_init_exec builds the rank matrix from the women's lists; this engine was given ranks");
+      return SMP_ERR_INVALID;
+    }
     matrices_valid = true;
     sparse = false;
     rank_built = true;

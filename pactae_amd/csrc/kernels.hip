@@ -392,8 +392,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 constexpr int kGatherMen = 32;
 constexpr int kGatherCols = 256;
 
+// RankT: uint16_t (the rank matrix built on the device) or int32_t (a rank
+// matrix given as input); a rank outside [0, n) is reported as a bad entry.
+template <typename RankT>
 __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ pref_m,
-                                                   const uint16_t* __restrict__ rank_w,
+                                                   const RankT* __restrict__ rank_w,
                                                    uint32_t* __restrict__ pr, int n, int K,
                                                    int tiles, int per_xcd, Scalars* sc) {
   const int b = blockIdx.x;
@@ -409,12 +412,13 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
     const int k = ct * kGatherCols + (i % chunks) * 64 + lane;
     if (m >= n || k >= K) continue;
     const int w = pref_m[(size_t)m * n + k];
-    uint32_t e;
+    uint32_t e = pr_pack(0, 0xFFFFu);
     if ((unsigned)w < (unsigned)n) {
-      e = pr_pack((uint32_t)w, (uint32_t)rank_w[(size_t)w * n + m]);
+      const uint32_t r = (uint32_t)rank_w[(size_t)w * n + m];
+      if (r < (uint32_t)n) e = pr_pack((uint32_t)w, r);
+      else bad = 1;
     } else {
       bad = 1;
-      e = pr_pack(0, 0xFFFFu);
     }
     pr[pr_index(m, k, n)] = e;
   }
@@ -637,8 +641,9 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         const int wi = a.pref_m[(size_t)m * n + kk];
         if ((unsigned)wi < (unsigned)n) {
           w = (uint32_t)wi;
-          r = (uint32_t)a.rank_w[(size_t)w * n + m];
-        } else {
+          r = a.rank32 ? (uint32_t)a.rank32[(size_t)w * n + m] : (uint32_t)a.rank_w[(size_t)w * n + m];
+        }
+        if ((unsigned)wi >= (unsigned)n || r >= (uint32_t)n) {
           valid = false;
           atomicOr(&a.sc->error, kErrEntryRange);
         }
@@ -1317,16 +1322,27 @@ hipError_t set_invert_lds_limit(int bytes) {
   return hipSuccess;
 }
 
-hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s) {
+template <typename RankT>
+static hipError_t launch_gather_pr_t(const int32_t* pref_m, const RankT* rank_w, uint32_t* pr, int n, int K,
+                                     Scalars* sc, hipStream_t s) {
   const int men_blocks = (n + kGatherMen - 1) / kGatherMen;
   const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
   const long long tiles = (long long)men_blocks * col_tiles;
   if (tiles > (1ll << 30)) return hipErrorInvalidValue;
   const int per_xcd = (int)((tiles + 7) / 8);
-  hipLaunchKernelGGL(k_gather_pr, dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
+  hipLaunchKernelGGL((k_gather_pr<RankT>), dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
                      (int)tiles, per_xcd, sc);
   return hipGetLastError();
+}
+
+hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
+                            Scalars* sc, hipStream_t s) {
+  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s);
+}
+
+hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
+                            Scalars* sc, hipStream_t s) {
+  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s);
 }
 
 hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, int waves,

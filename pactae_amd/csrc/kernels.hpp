@@ -11,6 +11,7 @@ struct ProposeArgs {
   const uint32_t* pr;
   const int32_t* pref_m;
   const uint16_t* rank_w;     // women's ranks, rank_w[w * n + m]
+  const int32_t* rank32;      // or, when not null, a rank matrix given as input
   unsigned long long* partner;
   int2* parked;
   Scalars* sc;
@@ -53,6 +54,9 @@ hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int ch
                               Scalars* sc, hipStream_t s, int nontemporal);
 hipError_t set_invert_lds_limit(int bytes);
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
+                            Scalars* sc, hipStream_t s);
+// The same from an int32 rank matrix given as input.
+hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
                             Scalars* sc, hipStream_t s);
 // Sparse prefix pass: buckets of (m, k < K) per woman (bins of 256 women with
 // bin_cap entries each and counters kBinCounterStride ints apart in bin_cnt,

@@ -212,7 +212,9 @@ class SmpEngineBamboo:
 
     ``SmpEngineBamboo(smp, n)`` copies the host lists into HBM like the reference
     constructor.  ``SmpEngineBamboo.from_device(d_m, d_w, n)`` borrows lists that
-    are already resident.
+    are already resident.  ``from_ranked`` and ``from_device_ranked`` take the
+    women's inverse rank matrix (``rank_w[w, m]`` = position of man m in woman w's
+    list) instead of the women's lists; init is then the prefix gather alone.
     """
 
     name = "BambooSMP-MI355X"
@@ -234,6 +236,27 @@ class SmpEngineBamboo:
         _check(_lib.smp_create_device(d_pref_m.ptr, d_pref_w.ptr, n, ctypes.byref(o),
                                       ctypes.byref(e._h)), "smp_create_device")
         e._keep = (d_pref_m, d_pref_w)
+        return e
+
+    @classmethod
+    def from_ranked(cls, pref_m: np.ndarray, rank_w: np.ndarray, n: int, **opts):
+        e = cls(None, n)
+        o = default_opts(**opts)
+        pm = np.ascontiguousarray(pref_m, dtype=np.int32).reshape(-1)
+        rw = np.ascontiguousarray(rank_w, dtype=np.int32).reshape(-1)
+        if pm.size != n * n or rw.size != n * n:
+            raise ValueError("pref_m and rank_w must hold n*n entries")
+        _check(_lib.smp_create_ranked(_i32p(pm), _i32p(rw), n, ctypes.byref(o), ctypes.byref(e._h)),
+               "smp_create_ranked")
+        return e
+
+    @classmethod
+    def from_device_ranked(cls, d_pref_m: DeviceArray, d_rank_w: DeviceArray, n: int, **opts):
+        e = cls(None, n)
+        o = default_opts(**opts)
+        _check(_lib.smp_create_device_ranked(d_pref_m.ptr, d_rank_w.ptr, n, ctypes.byref(o),
+                                             ctypes.byref(e._h)), "smp_create_device_ranked")
+        e._keep = (d_pref_m, d_rank_w)
         return e
 
     def exec(self) -> None:

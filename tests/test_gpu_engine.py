@@ -170,6 +170,80 @@ def test_mw_cas_baseline_schedule(gpu, oracle, kind):
         assert prof["proposals"] == t["proposals"], (kind, n)
 
 
+def host_ranks(pw, n):
+    """The women's inverse rank matrix derived on the host, the ranked engine's
+    input: rank[w, pref_w[w, r]] = r."""
+    rows = np.asarray(pw, dtype=np.int32).reshape(n, n)
+    rank = np.empty_like(rows)
+    np.put_along_axis(rank, rows, np.broadcast_to(np.arange(n, dtype=np.int32), (n, n)), axis=1)
+    return rank
+
+
+@pytest.mark.parametrize("cfg", [TAIL_CONFIGS[0], TAIL_CONFIGS[2], TAIL_CONFIGS[3], TAIL_CONFIGS[6]])
+@pytest.mark.parametrize("kind", range(8))
+def test_ranked_input(gpu, oracle, kind, cfg):
+    """The women's ranks given instead of their lists: the same matching and
+    proposal count as the oracle on every generator, with the GPU finishing
+    alone and with a host tail, and with chains walking past the prefix
+    (prefix_cols 1, where every proposal past position 0 reads the given ranks).
+    All-shared men's lists take the general walk (no women's lists to walk)."""
+    smp = gpu
+    for n in (1, 2, 5, 64, 65, 300, 1000):
+        if kind == 7 and n < 4:
+            continue
+        pm, pw = smp.generate(kind, n, seed=3000 + n)
+        want, t = oracle.gs_seq(n, pm, pw)
+        with smp.SmpEngineBamboo.from_ranked(pm, host_ranks(pw, n), n, **cfg) as e:
+            got = e.find_stable_matching()
+            prof = e.profile()
+        assert np.array_equal(got, want), (kind, n, cfg)
+        assert prof["proposals"] == t["proposals"], (kind, n, cfg)
+        assert prof["sparse_init"] == 0 and prof["master_list"] == 0
+
+
+def test_config1_ranks_given(gpu, oracle):
+    """Config 1 as stated: n = 2,000 uniform, the women's inverse rank matrix
+    derived on the host; lists and ranks resident in HBM."""
+    smp = gpu
+    n = 2000
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=0x5EED0001)
+    want, t = oracle.gs_seq(n, pm, pw)
+    rank = host_ranks(pw, n)
+    dm, dr = smp.DeviceArray(pm), smp.DeviceArray(rank)
+    for cfg in (dict(tail_mode=1), dict(tail_mode=2, tail_threshold=37), dict(tail_mode=0)):
+        with smp.SmpEngineBamboo.from_device_ranked(dm, dr, n, **cfg) as e:
+            for _ in range(2):
+                assert np.array_equal(e.find_stable_matching(), want), cfg
+            assert e.profile()["proposals"] == t["proposals"]
+            assert np.array_equal(e.rank_matrix().reshape(n, n), rank)
+    assert oracle.count_blocking_pairs(n, pm, pw, want) == 0
+
+
+def test_ranked_input_errors(gpu, oracle):
+    """A rank outside [0, n) is reported (where a proposal or the gather reads
+    it), not used; the MW-CAS schedule and the init-only run need the women's
+    lists and are refused."""
+    smp = gpu
+    n = 64
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=5)
+    rank = host_ranks(pw, n)
+    first = np.asarray(pm, dtype=np.int32).reshape(n, n)[:, 0]
+    bad = rank.copy()
+    bad[first[0], 0] = n  # read by man 0's first proposal, inside the prefix
+    with smp.SmpEngineBamboo.from_ranked(pm, bad, n, tail_mode=1, precheck=0) as e:
+        with pytest.raises(smp.SmpError) as ei:
+            e.exec()
+        assert "outside" in str(ei.value)
+    with pytest.raises(smp.SmpError) as ei:
+        smp.SmpEngineBamboo.from_ranked(pm, rank, n, schedule=smp.Schedule.MW_CAS)
+    assert ei.value.status == 1
+    want, _ = oracle.gs_seq(n, pm, pw)
+    with smp.SmpEngineBamboo.from_ranked(pm, rank, n) as e:
+        with pytest.raises(smp.SmpError):
+            e.init_only()
+        assert np.array_equal(e.find_stable_matching(), want)  # still usable after the refusal
+
+
 def test_config1_uniform_2000(gpu, oracle):
     smp = gpu
     n = 2000
