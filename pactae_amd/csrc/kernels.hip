@@ -587,6 +587,10 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   uint32_t spec_e = 0;
   unsigned long long n_atomics = 0, n_steps = 0;
   int since_poll = 0;
+  // Steps between polls: a.poll while many waves are busy, then every step once
+  // the count is near the threshold, so that the last waves see the hand-off
+  // within one step instead of up to a.poll steps (the kernel ends with them).
+  int poll_every = a.poll;
   if (__hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
     next_init = total;  // init found malformed lists: do nothing
   }
@@ -625,7 +629,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       break;
     }
     int act = 0;
-    const bool poll = T > 0 && ++since_poll >= a.poll;
+    const bool poll = T > 0 && ++since_poll >= poll_every;
     if (poll && lane == 0)
       act = __hip_atomic_load(&a.sc->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ++n_steps;
@@ -710,6 +714,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         }
         break;
       }
+      if (act <= 4 * T) poll_every = 1;
     }
   }
   if (lane == 0) {
