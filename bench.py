@@ -358,11 +358,12 @@ def main() -> int:
     rank_ms = statistics.mean(p["rank_kernel_ms"] for p in profs)
     K = last["prefix_cols"]
     if last["sparse_init"] and not last["rank_matrix"]:
-        # Sparse prefix pass: 4 B read per women's list entry, plus per prefix entry
-        # 4 B of bucket read and 4 B of PR prefix written.
+        # Sparse prefix pass: the row pass reads 4 B per women's list entry and,
+        # per prefix entry, 4 B of bucket and writes its 2-byte rank in bucket
+        # order; the scatter reads the 6 B back and writes the 4-byte PR entry.
         name = ("k_rank_prefix_row<1024,5>" if n % 4 == 0 and n <= 20480 else "k_invert_rows<4,1,1>")
-        rank_kernel = ("k_rank_prefix", f"{name} (sparse prefix pass over the women's rows)")
-        rank_bytes = 4 * n * n + 8 * n * K
+        rank_kernel = ("k_rank_prefix", f"{name} + k_scatter_prefix (sparse prefix pass over the women's rows)")
+        rank_bytes = 4 * n * n + 16 * n * K
     else:
         # Full rank matrix: 4 B read from pref_w + 2 B written to the 16-bit rank matrix per entry.
         rank_kernel = ("k_invert_rows", "k_invert_rows<4,1,0> (women's rank matrix)")

@@ -235,7 +235,7 @@ int smp_init_exec(smp_engine* e, int32_t init_pr, smp_init_times* t);
 /* Measure the hand-off threshold and PR prefix width on this instance: every
  * candidate (threshold in {0 (GPU finishes), 1, 2, 4, 8, 16, 24, 32, 48, 64, 96,
  * 128, 256, 1024}, prefix in
- * {64, 128, 256}) is solved `trials` times (0 = 5) and ranked by median wall
+ * {64, 96, 128, 256}) is solved `trials` times (0 = 5) and ranked by median wall
  * time; the four best and the untuned defaults are solved 3 * trials times more,
  * and the lowest of those medians is kept and used by later smp_exec calls in
  * SMP_TAIL_AUTO mode. */

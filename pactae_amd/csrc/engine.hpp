@@ -143,6 +143,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   unsigned long long* d_bins = nullptr;  // sparse prefix pass: bins of 256 women
   int* d_bin_cnt = nullptr;      // entries per bin
   int* d_bucket_cnt = nullptr;   // entries per woman's bucket (inside the d_bin_cnt allocation)
+  uint16_t* d_bucket_rank = nullptr;  // the row pass's ranks of the bucket entries, bucket order
   bool bins_pending = false;     // the bucket kernels of this run are queued (after the precheck)
   int K_sparse_alloc = 0;        // widest prefix the sparse buffers hold
   bool sparse_overflowed = false;  // a sparse pass on these lists overflowed a bin or bucket
@@ -245,6 +246,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipFree(d_bin_cnt);
     d_bin_cnt = nullptr;
     d_bucket_cnt = nullptr;  // inside d_bin_cnt
+    (void)hipFree(d_bucket_rank);
+    d_bucket_rank = nullptr;
     (void)hipFree(d_wstats);
     d_wstats = nullptr;
     (void)hipFree(d_starts);
@@ -336,6 +339,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     // n <= 2048, where n x n buckets are small).
     K_sparse_alloc = n <= 2048 ? K_alloc : std::min(K_alloc, 256);
     if (hipMalloc(&d_bucket, (size_t)n * bucket_cap(n, K_sparse_alloc) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_bucket_rank, (size_t)n * bucket_cap(n, K_sparse_alloc) * sizeof(uint16_t)) != hipSuccess ||
         hipMalloc(&d_bins, (size_t)pactae::bin_count(n) * bin_cap(K_sparse_alloc) * sizeof(unsigned long long)) !=
             hipSuccess ||
         hipMalloc(&d_bin_cnt, counter_words() * sizeof(int)) != hipSuccess) {
@@ -464,7 +468,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // are identical for every pair (the matching is unique); only the time differs.
   int tune(int trials, smp_tune_result* out) {
     if (trials <= 0) trials = 5;
-    const int Ks[] = {64, 128, 256};
+    const int Ks[] = {64, 96, 128, 256};
     const int Ts[] = {0, 1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 256, 1024};
     const smp_opts saved = opts;
     struct Cand {
@@ -697,7 +701,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
       span_bucket = {bucket_ev, &ev[0]};
       SMP_HIP_TRY(pactae::launch_rank_prefix(d_pref_w, d_rank_w, n, chunk, opts.validate, d_bucket_cnt, d_bucket, cap,
-                                             d_pr, d_sc, stream));
+                                             d_bucket_rank, d_pr, d_sc, stream));
       span_rank = {&ev[0], &ev[2]};
     } else {
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));

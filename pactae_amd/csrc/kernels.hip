@@ -137,7 +137,7 @@ struct PrefixBuckets {
   const int* cnt;          // suitors per woman (may exceed cap: overflow)
   const uint32_t* bucket;  // bucket[w * cap + j] = k << 16 | m
   int cap;
-  uint32_t* pr;            // column-blocked PR prefix
+  uint16_t* brank;         // brank[w * cap + j] = rank of bucket[w * cap + j]'s man
 };
 
 // Two 1024-thread workgroups per CU (at most 64 VGPRs): with one, half the row
@@ -232,8 +232,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         for (int j = tid; j < cntw; j += BS) {
           const uint32_t e = bk[j];
           const unsigned u = (e & 0xFFFFu) - (unsigned)c0;
-          if (u < (unsigned)clen)
-            pb.pr[pr_index((int)(e & 0xFFFFu), (int)(e >> 16), n)] = pr_pack((uint32_t)row, lds16[u]);
+          if (u < (unsigned)clen) pb.brank[(size_t)row * pb.cap + j] = lds16[u];
         }
         if (validate) {  // a hole in the LDS row marks a row that is not a permutation
           for (int c = tid; c < (clen + 1) / 2; c += BS) {
@@ -346,7 +345,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     for (int j = tid; j < cntw; j += BS) {
       const uint32_t e = j == tid ? e_first : bk[j];
       const int m = (int)(e & 0xFFFFu);
-      if (m < n) pb.pr[pr_index(m, (int)(e >> 16), n)] = pr_pack((uint32_t)row, lds16[m]);
+      if (m < n) pb.brank[(size_t)row * pb.cap + j] = lds16[m];
     }
     if (validate) {  // a hole in the LDS row marks a row that is not a permutation
       for (int c = tid; c < (n + 1) / 2; c += BS) {
@@ -380,6 +379,31 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
   }
   if (bad & 1) atomicOr(&sc->error, kErrEntryRange);
   if (bad & 2) atomicOr(&sc->error, kErrNotPerm);
+}
+
+// ---------------------------------------------------------------------------
+// Sparse prefix pass, last step: the ranks the row pass looked up, stored in
+// bucket order (coalesced, so that the row pass does not wait for scattered
+// writes), go to the men's PR prefix entries pr[m, k] = (w, rank).  One wave
+// per woman.  Nothing to do after a bucket overflow (the row pass wrote the
+// rank matrix instead).
+__global__ __launch_bounds__(256) void k_scatter_prefix(const int* __restrict__ cnt,
+                                                        const uint32_t* __restrict__ bucket,
+                                                        const uint16_t* __restrict__ brank, int cap, int n,
+                                                        uint32_t* __restrict__ pr, Scalars* sc) {
+  if (__hip_atomic_load(&sc->prefix_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (w >= n) return;
+  const int cntw = min(cnt[w], cap);
+  const uint32_t* bk = bucket + (size_t)w * cap;
+  const uint16_t* br = brank + (size_t)w * cap;
+  // Bound by the scattered 4-byte stores (2.56 M at n = 20,000, K = 128: ~34 us);
+  // loading a wave's slots ahead of the count changed nothing.
+  for (int j = (int)(threadIdx.x & 63); j < cntw; j += 64) {
+    const uint32_t e = bk[j];
+    const int m = (int)(e & 0xFFFFu);
+    if (m < n) pr[pr_index(m, (int)(e >> 16), n)] = pr_pack((uint32_t)w, br[j]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1295,11 +1319,11 @@ hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cn
 }
 
 hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int chunk, int validate,
-                              const int* cnt, const uint32_t* bucket, int cap, uint32_t* pr, Scalars* sc,
-                              hipStream_t s) {
+                              const int* cnt, const uint32_t* bucket, int cap, uint16_t* brank, uint32_t* pr,
+                              Scalars* sc, hipStream_t s) {
   const size_t lds = (size_t)((chunk + 7) & ~7) * sizeof(uint16_t);
   const bool vec = (n % 4 == 0) && (chunk % 8 == 0 || chunk == n);
-  const PrefixBuckets pb{cnt, bucket, cap, pr};
+  const PrefixBuckets pb{cnt, bucket, cap, brank};
   // Five loads per thread cover rows up to 20,480 entries in one round.
   if (vec && chunk == n && n / 4 <= kInvBlock * kBlkShort)
     hipLaunchKernelGGL((k_rank_prefix_row<kInvBlock, kBlkShort>), dim3(n), dim3(kInvBlock), lds, s, pref_w, rank,
@@ -1310,6 +1334,7 @@ hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int 
   else
     hipLaunchKernelGGL((k_invert_rows<1, 0, 1>), dim3(n), dim3(kInvBlock), lds, s, pref_w, rank, n, chunk,
                        validate, sc, pb);
+  hipLaunchKernelGGL(k_scatter_prefix, dim3((n + 3) / 4), dim3(256), 0, s, cnt, bucket, brank, cap, n, pr, sc);
   return hipGetLastError();
 }
 

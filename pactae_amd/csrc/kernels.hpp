@@ -72,9 +72,12 @@ constexpr int kBinCounterStride = 32;  // bin_cnt holds 256 * kBinCounterStride 
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
                                 int bin_cap, int* cnt, uint32_t* bucket, int cap, int gated, Scalars* sc, int cus,
                                 hipStream_t s, int stages = 3);
+// The row pass (ranks of each woman's bucket entries into brank, in bucket
+// order; or the full rank matrix after an overflow), then k_scatter_prefix
+// (brank into the PR prefix).
 hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int chunk, int validate,
-                              const int* cnt, const uint32_t* bucket, int cap, uint32_t* pr, Scalars* sc,
-                              hipStream_t s);
+                              const int* cnt, const uint32_t* bucket, int cap, uint16_t* brank, uint32_t* pr,
+                              Scalars* sc, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);
 // Folds the proposal waves' counters, then writes the control block, the partner
