@@ -46,15 +46,17 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def measured_traffic(kernel, n):
+def measured_traffic(kernel, n, K=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration), if
-    they were taken at this n; else None."""
+    they were taken at this n (and prefix width K, where it matters); else None."""
     path = os.path.join(ROOT, "profiles", f"counters_{kernel}_n{n}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         c = json.load(f)
+    if K is not None and c.get("prefix_cols") not in (None, K):
+        return None
     return int(c["fetch_size_kib_per_dispatch"] * 1024 * c["fetch_factor"]
                + c["write_size_kib_per_dispatch"] * 1024 * c["write_factor"])
 
@@ -402,7 +404,7 @@ def main() -> int:
         "init": {"sparse_prefix_pass": bool(last["sparse_init"]), "rank_matrix_built": bool(last["rank_matrix"])},
         "roofline": {"kernel": rank_kernel[1], "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(rank_kernel[0], n),
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(rank_kernel[0], n, K if rank_kernel[0] == "k_rank_prefix" else None),
                      "algorithmic_bytes": rank_bytes, "kernel_ms": round(rank_ms, 4)},
         "cpu_baseline": None,
         "gpu_baseline": gpu_base,
