@@ -384,8 +384,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // Queues the sparse pass's bucket kernels on the engine's stream.  Queued
   // behind the precheck (gated) they are not timed (an event there would delay
   // them); otherwise from ev[6] to the init's ev[0].
-  // stages as in launch_bucket_prefix (1: bin the prefix, 2: sort the bins).
-  int launch_buckets(int gated, int stages = 3) {
+  // stages as in launch_bucket_prefix (1: bin the prefix, 2: sort the bins);
+  // publish_seq != 0: the binning kernel also publishes the precheck's flags.
+  int launch_buckets(int gated, int stages = 3, unsigned publish_seq = 0) {
     if (stages & 1) {
       bucket_ev = nullptr;
       if (!gated) {
@@ -394,7 +395,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
       }
     }
     SMP_HIP_TRY(pactae::launch_bucket_prefix(d_pref_m, n, K, d_bin_cnt, d_bins, bin_cap(K), d_bucket_cnt, d_bucket,
-                                             bucket_cap(n, K), gated, d_sc, dev.cus, stream, stages));
+                                             bucket_cap(n, K), gated, d_sc, dev.cus, stream, stages, h_sc_dev,
+                                             publish_seq ? h_flags_dev + kFlagPrecheck : nullptr, publish_seq));
     bins_pending = true;
     return SMP_OK;
   }
@@ -617,17 +619,14 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     SMP_HIP_TRY(pactae::launch_first_choices(d_pref_m, n, d_seen, d_sc, stream));
     // The sparse pass's bucket kernels go right behind the precheck, gated on its
-    // flags on the device, so they run while the host reads the flags.  The
-    // flags are published between the two: the host's launches of the publish
-    // and of the sort then overlap the binning instead of leaving the queue
-    // idle, and the sort still covers the host's reaction.
-    if (may_sparse) {
-      if (int st = launch_buckets(/*gated=*/1, /*stages=*/1)) return st;
-    }
+    // flags on the device, so they run while the host reads the flags; the
+    // binning kernel publishes those flags as it starts (one launch fewer for
+    // the host to queue ahead of the kernels it feeds).
     const unsigned seq = ++flag_seq;
-    SMP_HIP_TRY(pactae::launch_publish_scalars(d_sc, h_sc_dev, h_flags_dev + kFlagPrecheck, seq, stream));
     if (may_sparse) {
-      if (int st = launch_buckets(/*gated=*/1, /*stages=*/2)) return st;
+      if (int st = launch_buckets(/*gated=*/1, /*stages=*/3, seq)) return st;
+    } else {
+      SMP_HIP_TRY(pactae::launch_publish_scalars(d_sc, h_sc_dev, h_flags_dev + kFlagPrecheck, seq, stream));
     }
     SMP_HIP_TRY(wait_flag(kFlagPrecheck, seq));
     if (int st = check_device_error("precheck")) return st;

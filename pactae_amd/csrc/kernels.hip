@@ -481,9 +481,16 @@ __device__ inline bool general_path_open(const Scalars* sc) {
          __hip_atomic_load(&sc->first_differs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+// With h_flag, workgroup 0 first publishes the control block (the precheck's
+// flags, final at this kernel's start) to the host, as k_publish_scalars does.
 __global__ __launch_bounds__(kBinThreads) void k_bin_prefix(const int32_t* __restrict__ pref_m, int n, int K, int nbins,
                                                     int* __restrict__ bin_cnt, unsigned long long* __restrict__ bins,
-                                                    int bin_cap, int gated, Scalars* sc) {
+                                                    int bin_cap, int gated, Scalars* sc, Scalars* h_sc,
+                                                    unsigned* h_flag, unsigned seq) {
+  if (h_flag && blockIdx.x == 0 && threadIdx.x == 0) {
+    *h_sc = *sc;
+    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
   if (gated && !general_path_open(sc)) return;
   __shared__ int hist[kBinWomen];
   __shared__ int base[kBinWomen];
@@ -1302,7 +1309,7 @@ int bin_count(int n) { return (n + kBinWomen - 1) / kBinWomen; }
 
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
                                 int bin_cap, int* cnt, uint32_t* bucket, int cap, int gated, Scalars* sc, int cus,
-                                hipStream_t s, int stages) {
+                                hipStream_t s, int stages, Scalars* h_sc, unsigned* h_flag, unsigned seq) {
   const int nbins = bin_count(n);
   if (nbins > kBinWomen) return hipErrorInvalidValue;
   if (stages & 1) {
@@ -1310,7 +1317,7 @@ hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cn
     const long long chunks = (long long)((total + kBinEntries - 1) / kBinEntries);
     const int grid = (int)std::max<long long>(1, std::min<long long>(chunks, (long long)cus * 16));
     hipLaunchKernelGGL(k_bin_prefix, dim3(grid), dim3(kBinThreads), 0, s, pref_m, n, K, nbins, bin_cnt, bins,
-                       bin_cap, gated, sc);
+                       bin_cap, gated, sc, h_sc, h_flag, seq);
   }
   if (stages & 2)
     hipLaunchKernelGGL(k_bin_sort, dim3(nbins * kSortSplit), dim3(1024), 0, s, bins, bin_cnt, bin_cap, cnt, bucket,

@@ -68,10 +68,13 @@ int bin_count(int n);
 constexpr int kBinCounterStride = 32;  // bin_cnt holds 256 * kBinCounterStride ints
 // gated = 1: the kernels do nothing unless the precheck (k_first_choices, queued
 // before them) found two men sharing a first choice and not all first choices equal.
-// stages: 1 = k_bin_prefix, 2 = k_bin_sort, 3 = both (in that order).
+// stages: 1 = k_bin_prefix, 2 = k_bin_sort, 3 = both (in that order).  With
+// h_flag, k_bin_prefix first publishes the control block to the host (h_sc, then
+// seq into *h_flag), as launch_publish_scalars does.
 hipError_t launch_bucket_prefix(const int32_t* pref_m, int n, int K, int* bin_cnt, unsigned long long* bins,
                                 int bin_cap, int* cnt, uint32_t* bucket, int cap, int gated, Scalars* sc, int cus,
-                                hipStream_t s, int stages = 3);
+                                hipStream_t s, int stages = 3, Scalars* h_sc = nullptr, unsigned* h_flag = nullptr,
+                                unsigned seq = 0);
 // The row pass (ranks of each woman's bucket entries into brank, in bucket
 // order; or the full rank matrix after an overflow), then k_scatter_prefix
 // (brank into the PR prefix).
