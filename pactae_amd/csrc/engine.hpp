@@ -637,11 +637,27 @@ struct smp_engine : public pactae::AbsSmpEngine {
     // (With ranks given there are no women's lists to walk: the general path.)
     if (!*perfect && n > 1 && h_sc->first_differs == 0 && !d_rank_in) {
       bins_pending = false;  // the gated bucket kernels found the path closed and did nothing
-      // The men's lists compared with man 0's and, ahead of the decision, the
-      // women's permutation check the master-list path needs, in one launch.
-      SMP_HIP_TRY(pactae::launch_master_scan(d_pref_m, d_pref_w, n, opts.validate, d_sc, dev.cus, stream));
+      // Speculatively the master-list path: the inverse of man 0's list (into
+      // d_seen, free after the precheck), then one launch that walks the
+      // women's lists as if every man held man 0's list, beside the scans that
+      // decide whether he does (and check the women's rows).  The men's partner
+      // words start unclaimed (the reset).
+      SMP_HIP_TRY(hipEventRecord(ev[0], stream));
+      SMP_HIP_TRY(pactae::launch_master_inverse(d_pref_m, n, d_seen, d_sc, stream));
+      SMP_HIP_TRY(pactae::launch_master_fused(d_pref_m, d_pref_w, d_seen, d_partner, n, opts.validate, d_sc,
+                                              dev.cus, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[2], stream));
       if (int st = read_scalars()) return st;
       master = h_sc->not_master == 0;
+      if (master) {
+        span_propose = {&ev[0], &ev[2]};
+      } else {
+        // Not a master list: the walk's partner words, counters and flags are
+        // void; back to the state the reset left, and on to the general path.
+        SMP_HIP_TRY(pactae::launch_reset_scalars(
+            d_sc, std::min(n, propose_grid() * 4), stream, nullptr, 0, may_sparse ? d_bin_cnt : nullptr,
+            counter_words(), opts.schedule == SMP_SCHEDULE_BAMBOO ? d_partner : nullptr, (size_t)n));
+      }
     }
     times.master_list = master ? 1 : 0;
     return SMP_OK;
@@ -662,15 +678,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
       return SMP_OK;
     }
     if (master) {
-      // Master list: the inverse of the shared list (in d_seen, free after the
-      // precheck); neither the rank matrix nor the PR prefix is needed.  (The
-      // women's rows were checked by the precheck's master-list scan.)
+      // Master list: the precheck's launch has walked it already; neither the
+      // rank matrix nor the PR prefix is needed.
       blocks_copied = 0;
-      SMP_HIP_TRY(pactae::launch_fill_u64(d_partner, n, pactae::kEmpty, stream));
-      SMP_HIP_TRY(hipEventRecord(ev[0], stream));
-      SMP_HIP_TRY(pactae::launch_master_inverse(d_pref_m, n, d_seen, d_sc, stream));
-      SMP_HIP_TRY(hipEventRecord(ev[2], stream));
-      span_rank = {&ev[0], &ev[2]};
       return SMP_OK;
     }
     const int chunk = invert_chunk();
@@ -777,11 +787,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       times.handoff_free = 0;
       return SMP_OK;
     }
-    if (master) {
-      SMP_HIP_TRY(pactae::launch_propose_master(d_pref_w, d_seen, d_partner, n, d_sc, grid, stream));
-      SMP_HIP_TRY(hipEventRecord(ev[4], stream));
-      span_propose = {&ev[2], &ev[4]};
-      if (int st = read_scalars()) return st;
+    if (master) {  // walked in the precheck's launch
       if (int st = check_device_error("women-proposing walk")) return st;
       times.handoff_free = 0;
       return SMP_OK;

@@ -8,7 +8,7 @@
 //   k_gather_pr                       F9PRMatrixInitKernel   src/smp/kernel.cu:688-709
 //   k_propose                         La3F9CoreKernel        src/smp/kernel.cu:633-670
 //   k_post                            PostProc               src/smp/smp_engine_bamboo.cu:299-303
-//   k_master_scan .. k_post_master    master-list path (every man holds the same list):
+//   k_master_fused .. k_post_master   master-list path (every man holds the same list):
 //                                     detection, women-proposing walk, matching
 //   k_mw_cas / k_post_cas             MW-Par-GPU-CAS baseline schedule
 //                                     src/smp/smp_engine_mw_3_f9.cu:37-97
@@ -972,33 +972,20 @@ __device__ void check_rows_blocks(const int32_t* __restrict__ pref, int n, Scala
   if (bad) atomicOr(&sc->error, bad);
 }
 
-// Master-list detection and the women's permutation check in one launch: the
-// first g_rows workgroups compare the men's lists with man 0's, the others check
-// the women's rows (when check != 0), so the two 4n^2-byte reads overlap instead of
-// running one after the other.
-__global__ __launch_bounds__(256) void k_master_scan(const int32_t* __restrict__ pref_m,
-                                                     const int32_t* __restrict__ pref_w, int n, int check,
-                                                     int g_rows, Scalars* sc) {
-  extern __shared__ uint32_t bits[];
-  if ((int)blockIdx.x < g_rows) {
-    rows_equal_blocks(pref_m, n, sc, (int)blockIdx.x, g_rows);
-    return;
-  }
-  if (check) check_rows_blocks(pref_w, n, sc, (int)blockIdx.x - g_rows, (int)gridDim.x - g_rows, bits, true);
-}
-
 // Women-proposing walk: one wave per chain (woman), 64 list positions a step.
 // Woman w's proposal to the man at position k of her list is the packed word
 // (Linv[w], w, k); the man keeps the smallest (atomicMin on his partner word), so
 // he holds the woman who comes first in L.  A displaced woman continues at her
 // stored position + 1.  Hints read before the atomic can only be stale upward
 // (words only decrease), so they never hide a candidate.
-__global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restrict__ pref_w,
-                                                        const int32_t* __restrict__ linv,
-                                                        unsigned long long* partner_m, int n, Scalars* sc) {
+// Wave `wave` of `waves` walks the women wave, wave + waves, ...  Run beside the
+// scan that decides whether the lists are a master list, it is speculative: it
+// stops once the scan has found two men's lists that differ (sc->not_master),
+// checked every 16 steps, and its result is then thrown away.
+__device__ void master_walk(const int32_t* __restrict__ pref_w, const int32_t* __restrict__ linv,
+                            unsigned long long* partner_m, int n, Scalars* sc, int wave, int waves) {
   const int lane = threadIdx.x & 63;
-  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
-  int next_init = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  int next_init = wave;
   int w = -1, k = 0;
   // Speculation: while a claim is in flight, the next segment of the woman it
   // would displace (by the hint) is loaded too; (spec_w, spec_k) says whose.
@@ -1016,6 +1003,9 @@ __global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restric
       if (lane == 0) atomicOr(&sc->error, kErrListEnd);
       break;
     }
+    if ((n_steps & 15) == 15 &&
+        __hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+      break;
     const uint32_t rw = (uint32_t)linv[w];
     const int kk = k + lane;
     bool valid = kk < n;
@@ -1067,6 +1057,33 @@ __global__ __launch_bounds__(256) void k_propose_master(const int32_t* __restric
     atomicAdd(&sc->atomics, n_atomics);
     atomicAdd(&sc->steps, n_steps);
   }
+}
+
+// The master-list path in one launch, three kinds of workgroup:
+//   [0, g_walk)                 the women-proposing walk, speculatively (it needs
+//                               only man 0's list, inverted into linv beforehand);
+//   [g_walk, g_walk + g_rows)   every men's list compared with man 0's;
+//   the rest (when check != 0)  the women's rows checked for being permutations.
+// The scans read 8n^2 bytes and set the HBM rate; the walk, latency-bound,
+// overlaps them instead of following them.  The walk's workgroups come first so
+// that they are dispatched first; nothing in the launch waits for another part.
+__global__ __launch_bounds__(256) void k_master_fused(const int32_t* __restrict__ pref_m,
+                                                      const int32_t* __restrict__ pref_w,
+                                                      const int32_t* __restrict__ linv,
+                                                      unsigned long long* partner_m, int n, int check, int g_walk,
+                                                      int g_rows, Scalars* sc) {
+  extern __shared__ uint32_t bits[];
+  const int b = (int)blockIdx.x;
+  if (b < g_walk) {
+    master_walk(pref_w, linv, partner_m, n, sc, b * 4 + (int)(threadIdx.x >> 6), g_walk * 4);
+    return;
+  }
+  const int r = b - g_walk;
+  if (r < g_rows) {
+    rows_equal_blocks(pref_m, n, sc, r, g_rows);
+    return;
+  }
+  if (check) check_rows_blocks(pref_w, n, sc, r - g_rows, (int)gridDim.x - g_walk - g_rows, bits, true);
 }
 
 // match[m] = the woman in man m's partner word; proposals (the man-proposing
@@ -1232,12 +1249,16 @@ hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s, int* z0,
   return hipGetLastError();
 }
 
-hipError_t launch_master_scan(const int32_t* pref_m, const int32_t* pref_w, int n, int check, Scalars* sc, int cus,
-                              hipStream_t s) {
+hipError_t launch_master_fused(const int32_t* pref_m, const int32_t* pref_w, const int32_t* linv,
+                               unsigned long long* partner_m, int n, int check, Scalars* sc, int cus, hipStream_t s) {
+  // Two walk waves per CU: measured 788 us at n = 20,000 against 822 (four),
+  // 843 (eight), 905 (one) and 870 for the scans and the walk one after the other.
+  const int g_walk = std::max(1, std::min((n + 3) / 4, std::max(1, cus / 2)));
   const int g_rows = std::max(1, std::min((n + 3) / 4, cus * 4));
   const int g_check = check ? std::max(1, std::min(n, cus * 4)) : 0;
   const size_t lds = (size_t)((n + 31) / 32) * 4;
-  hipLaunchKernelGGL(k_master_scan, dim3(g_rows + g_check), dim3(256), lds, s, pref_m, pref_w, n, check, g_rows, sc);
+  hipLaunchKernelGGL(k_master_fused, dim3(g_walk + g_rows + g_check), dim3(256), lds, s, pref_m, pref_w, linv,
+                     partner_m, n, check, g_walk, g_rows, sc);
   return hipGetLastError();
 }
 
@@ -1245,12 +1266,6 @@ hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars
   hipError_t e = launch_fill_i32(linv, n, -1, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_master_inverse, dim3((n + 255) / 256), dim3(256), 0, s, L, n, linv, sc);
-  return hipGetLastError();
-}
-
-hipError_t launch_propose_master(const int32_t* pref_w, const int32_t* linv, unsigned long long* partner_m,
-                                 int n, Scalars* sc, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_propose_master, dim3(grid), dim3(256), 0, s, pref_w, linv, partner_m, n, sc);
   return hipGetLastError();
 }
 

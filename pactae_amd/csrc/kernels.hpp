@@ -35,14 +35,14 @@ hipError_t launch_reset_scalars(Scalars* sc, int active, hipStream_t s, int* z0 
                                 int* z1 = nullptr, size_t n1 = 0, unsigned long long* partner = nullptr,
                                 size_t np = 0);
 hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars* sc, hipStream_t s);
-// Master-list detection (every men's list equal to man 0's; not_master set
-// otherwise) and, when check != 0, the permutation check of every women's row,
-// in one launch.
-hipError_t launch_master_scan(const int32_t* pref_m, const int32_t* pref_w, int n, int check, Scalars* sc, int cus,
-                              hipStream_t s);
+// The master-list path in one launch: the women-proposing walk over linv (the
+// inverse of man 0's list, from launch_master_inverse) into partner_m, beside the
+// comparison of every men's list with man 0's (not_master set if one differs:
+// the walk's result is then void) and, when check != 0, the permutation check of
+// every women's row.
+hipError_t launch_master_fused(const int32_t* pref_m, const int32_t* pref_w, const int32_t* linv,
+                               unsigned long long* partner_m, int n, int check, Scalars* sc, int cus, hipStream_t s);
 hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars* sc, hipStream_t s);
-hipError_t launch_propose_master(const int32_t* pref_w, const int32_t* linv, unsigned long long* partner_m,
-                                 int n, Scalars* sc, int grid, hipStream_t s);
 hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
                               Scalars* sc, hipStream_t s);
 hipError_t launch_mw_cas(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank_w, int* partner_rank,
