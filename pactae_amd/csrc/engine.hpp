@@ -1094,6 +1094,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // stream's queue, its copies wait for the kernel instead.  Seen from the events
   // (block 0 landing only after a kernel that ran well past one block's copy
   // time), the copy streams are replaced, which moves them to other queues.
+  // Both copy streams are checked, by the first block each carries (block 0 on
+  // copy_stream, block 1 on copy_stream2): a block the host tail reaches late
+  // makes it wait just the same.
   void check_copy_overlap() {
     float copy_ms = 0, kern_ms = 0;
     if (hipEventQuery(ev_blk[0]) != hipSuccess || hipEventQuery(ev[5]) != hipSuccess ||
@@ -1102,6 +1105,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
       (void)hipGetLastError();
       return;
     }
+    float copy1_ms = 0;
+    if (blocks_copied > 1 && hipEventQuery(ev_blk[1]) == hipSuccess &&
+        hipEventElapsedTime(&copy1_ms, ev[2], ev_blk[1]) == hipSuccess)
+      copy_ms = std::max(copy_ms, copy1_ms);
+    (void)hipGetLastError();
     if (kern_ms > 0.12f && copy_ms >= kern_ms && requeues < 8) {
       (void)hipStreamSynchronize(copy_stream);
       (void)hipStreamSynchronize(copy_stream2);
