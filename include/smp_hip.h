@@ -300,7 +300,9 @@ typedef struct smp_batch_times {
   int32_t group_lanes;                /* pool mode: lanes per proposal chain        */
   int32_t pool_slots;                 /* pool mode: instances open per workgroup    */
   int32_t rank_layout;                /* pool mode: SMP_RANK_*                      */
-  double rank_ms;                     /* pool mode: the rank pass alone, measured
+  double rank_ms;                     /* pool mode: the launch up to the end of the
+                                         rank pass (the master-list detection runs
+                                         beside it on a second stream), measured
                                          while tuning (min of two runs)             */
   int32_t reserved[4];
 } smp_batch_times;

@@ -276,7 +276,9 @@ struct smp_batch {
   unsigned long long* d_props = nullptr;
 
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;   // pool: the master-list detection, beside the rank pass
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t mid_ev = nullptr;  // recorded between the rank pass and the chains when set
   float rank_ms = 0;            // rank pass of the chosen configuration, measured while tuning
   hipGraph_t graph = nullptr;
@@ -307,6 +309,9 @@ struct smp_batch {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev_mid) (void)hipEventDestroy(ev_mid);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -316,19 +321,26 @@ struct smp_batch {
     if (e != hipSuccess) return e;
     if (c.mode == SMP_BATCH_POOL) {
       const bool tr = c.layout == SMP_RANK_MEN_ROWS;
+      // The master-list detection (men's lists) runs on the side stream while
+      // the rank pass (women's lists) runs here; neither writes what the other
+      // reads, and the chains wait for both.
+      if ((e = hipEventRecord(ev_fork, stream)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(side, ev_fork, 0)) != hipSuccess) return e;
       const int cgrid = std::max(1, std::min((count + 3) / 4, dev.cus * 8));
-      hipLaunchKernelGGL(pactae::k_master_probe, dim3(cgrid), dim3(256), 0, stream, d_tasks, count, d_pw, d_pm,
+      hipLaunchKernelGGL(pactae::k_master_probe, dim3(cgrid), dim3(256), 0, side, d_tasks, count, d_pw, d_pm,
                          d_master);
-      hipLaunchKernelGGL(pactae::k_master_rows, dim3(dev.cus * 8), dim3(256), 0, stream, d_tasks, d_row_task,
+      hipLaunchKernelGGL(pactae::k_master_rows, dim3(dev.cus * 8), dim3(256), 0, side, d_tasks, d_row_task,
                          (int)total_men, d_pw, d_pm, d_master);
       hipLaunchKernelGGL(pactae::k_master_linv, dim3(std::max(1, std::min(count, dev.cus * 8))), dim3(256), 0,
-                         stream, d_tasks, count, d_moffs, d_pw, d_pm, d_master, d_linv, d_ctl + 1);
+                         side, d_tasks, count, d_moffs, d_pw, d_pm, d_master, d_linv, d_ctl + 1);
       if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = hipEventRecord(ev_join, side)) != hipSuccess) return e;
       e = tr ? pactae::launch_rankT(stream, d_tasks, d_tiles, ntiles, max_n, d_ctl + 1, 0, dev.cus)
              : pactae::launch_rank_rows16(stream, d_tasks, count, d_row_task, (int)total_men, max_n, d_ctl + 1,
                                           0, dev.cus, dev.max_lds);
       if (e != hipSuccess) return e;
       if (mid_ev && (e = hipEventRecord(mid_ev, stream)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(stream, ev_join, 0)) != hipSuccess) return e;
       pactae::PoolArgs a{std::max(1, std::min((count + c.slots - 1) / c.slots, dev.cus * c.wpc)),
                          max_n, count, stream, d_desc, d_pm, d_rank, d_match, d_ctl, d_props, d_ctl + 1,
                          d_pw, d_master, d_linv};
@@ -603,6 +615,9 @@ int smp_batch_create(int32_t count, const int32_t* sizes, const int64_t* offsets
                           hipMemcpyHostToDevice));
   }
   SMP_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  SMP_HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  SMP_HIP_TRY(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  SMP_HIP_TRY(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   SMP_HIP_TRY(hipEventCreate(&b->ev0));
   SMP_HIP_TRY(hipEventCreate(&b->ev1));
   SMP_HIP_TRY(hipEventCreate(&b->ev_mid));
