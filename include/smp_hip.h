@@ -103,7 +103,10 @@ enum smp_tail_mode {
 typedef struct smp_opts {
   int32_t prefix_cols;    /* PR columns gathered at init: 0 = auto, >= n = full n x n
                              (the reference's layout); beyond it the kernel reads
-                             pref_m and the rank matrix directly                  */
+                             pref_m and the rank matrix directly.  Auto is 128 (or
+                             smp_tune's choice), widened to 256 for n >= 1024 when
+                             the sparse pass overflows a bucket (lists that share
+                             their top women)                                     */
   int32_t tail_mode;      /* enum smp_tail_mode                                    */
   int32_t tail_threshold; /* free-man count at which the host takes over; 0 = auto */
   int32_t device;         /* HIP device ordinal                                    */
