@@ -68,6 +68,34 @@ def test_parity_mid_sizes(gpu, oracle, kind, cfg):
         assert prof["proposals"] == t["proposals"]
 
 
+@pytest.mark.parametrize("cfg", [dict(), dict(tail_mode=1), dict(tail_mode=2, tail_threshold=37)])
+@pytest.mark.parametrize("n", [1024, 1500, 3001])
+def test_prefix_widened_after_overflow(gpu, oracle, n, cfg):
+    """Correlated lists overflow the sparse pass's buckets; with the automatic
+    prefix width the engine then gathers 256 columns (n >= 1024) and keeps them
+    for its later solves of the same lists.  Every solve is the oracle's
+    matching with the oracle's proposal count."""
+    smp = gpu
+    pm, pw = smp.generate(smp.Workload.CORRELATED, n, seed=600 + n)
+    want, t = oracle.gs_seq(n, pm, pw)
+    obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+    with smp.SmpEngineBamboo(obj, n, **cfg) as e:
+        for rep in range(3):
+            got = e.find_stable_matching()
+            prof = e.profile()
+            assert np.array_equal(got, want), (n, cfg, rep)
+            assert prof["proposals"] == t["proposals"], (n, cfg, rep)
+            # The first solve runs the sparse pass, which overflows; later ones
+            # go straight to the full init.
+            assert prof["rank_matrix"] == 1 and prof["sparse_init"] == (1 if rep == 0 else 0), (n, cfg, rep)
+            assert prof["prefix_cols"] == 256, (n, cfg, rep)
+    # A width the caller sets is kept.
+    with smp.SmpEngineBamboo(obj, n, prefix_cols=96, **cfg) as e:
+        got = e.find_stable_matching()
+        assert np.array_equal(got, want)
+        assert e.profile()["prefix_cols"] == 96
+
+
 @pytest.mark.parametrize("cfg", INIT_CONFIGS)
 @pytest.mark.parametrize("kind", range(8))
 def test_init_modes(gpu, oracle, kind, cfg):
