@@ -125,6 +125,36 @@ def test_first_choice_closed_form(oracle, smp, kind):
     assert np.array_equal(got, firsts)
 
 
+def test_is_perfect_false_branch(oracle, smp):
+    """IsPerfect (include/smp/smp_engine_bamboo.h:25-33) must say no whenever two
+    men share a first choice.  Checked on uniform and congested lists (first
+    choices collide with certainty at these sizes, counted here with numpy), and
+    on lists built so that row 0 is a permutation while column 0 repeats one
+    woman: an oracle that read row 0 (pref_m[m]) instead of column 0
+    (pref_m[m * n]) would answer yes on exactly that input."""
+    for kind, n in ((smp.Workload.UNIFORM, 300), (smp.Workload.CONGESTED, 300),
+                    (smp.Workload.CORRELATED, 257), (smp.Workload.REF_RANDOM, 200)):
+        pm, _ = smp.generate(kind, n, seed=31)
+        firsts = pm.reshape(n, n)[:, 0]
+        assert len(np.unique(firsts)) < n
+        assert not oracle.is_perfect(n, pm), kind
+    n = 6
+    rows = np.empty((n, n), np.int32)
+    rows[0] = np.arange(n)                       # row 0 is the identity permutation
+    for m in range(1, n):                        # every other man starts with woman 0
+        rows[m] = np.roll(np.arange(n), -((m * 5) % n) if m % 2 else 0)
+        rows[m, [0, list(rows[m]).index(0)]] = rows[m, [list(rows[m]).index(0), 0]]
+    assert (rows[:, 0] == 0).all() and all(sorted(r) == list(range(n)) for r in rows)
+    row0_misread = len(set(rows[0, :n].tolist())) == n   # what the planted bug would compute
+    assert row0_misread
+    assert not oracle.is_perfect(n, flat(rows))
+    # One collision among otherwise distinct first choices is enough.
+    rows = np.array([np.roll(np.arange(n), -m) for m in range(n)], np.int32)
+    assert oracle.is_perfect(n, flat(rows))
+    rows[4] = np.roll(np.arange(n), -1)          # man 4 now starts with woman 1, as man 1 does
+    assert not oracle.is_perfect(n, flat(rows))
+
+
 def test_serial_dictatorship_closed_form(oracle, smp):
     for n in (1, 2, 5, 64, 257):
         pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=n)
