@@ -259,6 +259,20 @@ int smp_rank_matrix(smp_engine* e, int32_t* rank_out);
 int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols);
 void smp_destroy(smp_engine* e);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  One solve of lists already in HBM (n <= 4096) into the caller's
+ * d_match (int32[n] in HBM), with *proposals (may be null) set to the proposal
+ * count.  The same result as smp_create_device + smp_exec + smp_result_device,
+ * by the GPU-only full-init pipeline (init_mode FULL, tail_mode GPU, precheck 0),
+ * with every intermediate buffer in a fixed 34.2 MiB device array of the library
+ * that each GPU it runs on holds until the process ends.  It calls no hipMalloc
+ * or hipFree.  n above 4096 returns SMP_ERR_INVALID.  Synchronous (default
+ * stream).  Calls that overlap on one GPU share that array and race: serialize
+ * them per device. */
+int smp_exec_device_noalloc(const int32_t* d_pref_m, const int32_t* d_pref_w, int32_t n, int32_t device,
+                            int32_t* d_match, int64_t* proposals);
+
 /* ---- batched: many independent instances in one persistent launch -------- */
 typedef struct smp_batch smp_batch;
 

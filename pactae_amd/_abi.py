@@ -234,6 +234,7 @@ SIGNATURES = [
     ("smp_rank_matrix", ctypes.c_int, [vp, i32p]),
     ("smp_pr_matrix", ctypes.c_int, [vp, i32p, i32p]),
     ("smp_destroy", None, [vp]),
+    ("smp_exec_device_noalloc", ctypes.c_int, [vp, vp, i32, i32, vp, i64p]),
     ("smp_batch_default_opts", None, [ctypes.POINTER(SmpBatchOpts)]),
     ("smp_batch_create", ctypes.c_int,
      [i32, i32p, i64p, i32p, i32p, ctypes.POINTER(SmpBatchOpts), ctypes.POINTER(vp)]),
