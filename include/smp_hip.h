@@ -113,9 +113,9 @@ typedef struct smp_opts {
   int32_t precheck;       /* 1 = IsPerfect fast path (default), 0 = skip           */
   int32_t waves_per_cu;   /* proposal-kernel occupancy; 0 = auto                   */
   int32_t validate;       /* 1 = reject rows that are not permutations (default)   */
-  int32_t coherent_hint;  /* 1 = partner-word hints read past non-coherent caches
-                             (default); 0 = plain cached reads (may be stale, which
-                             costs failed atomics, never correctness)             */
+  int32_t coherent_hint;  /* 1 = partner-word hints read past non-coherent caches;
+                             0 = plain cached reads (default; a stale hint costs a
+                             failed atomic, never correctness)                    */
   int32_t poll_interval;  /* proposal-kernel steps between reads of the shared
                              active-wave counter that triggers the hand-off; 0 = 16 */
   int32_t host_spin;      /* 1 = the host polls the runtime for device completion,
@@ -182,8 +182,8 @@ typedef struct smp_times {
   int32_t handoff_free;    /* free men handed to the host (0 = GPU finished)      */
   int32_t prefix_cols;     /* PR columns actually gathered                        */
   int32_t tail_threshold;  /* threshold used                                      */
-  int32_t bailed;          /* chains the host handed back to the GPU because they
-                              walked past the PR prefix                           */
+  int32_t bailed;          /* chains the host handed back to the GPU (past the PR
+                              prefix with no extension window left)               */
   int32_t master_list;     /* 1 if every man's list was the same permutation and
                               the women-proposing walk answered (the stable
                               matching is then unique)                            */
@@ -191,7 +191,15 @@ typedef struct smp_times {
   int32_t sparse_init;     /* 1 if init took the sparse prefix pass               */
   int32_t rank_matrix;     /* 1 if the n x n rank matrix was built (full init, a
                               bucket overflow, or a chain past the prefix)        */
-  int32_t reserved[3];
+  int32_t copy_requeues;   /* copy-stream pairs the engine has replaced so far
+                              because the PR-prefix copies had waited for the
+                              proposal kernel (their stream shared its hardware
+                              queue); at most 8                                   */
+  int32_t copy_serialized; /* 1 if this run's prefix copies waited for the
+                              proposal kernel and no replacement was left          */
+  int32_t tail_windows;    /* extension windows (the next 256 entries of one man's
+                              list) the GPU computed for host-tail chains that
+                              walked past the PR prefix                           */
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;
@@ -253,10 +261,13 @@ int smp_tune(smp_engine* e, int32_t trials, smp_tune_result* out);
 /* Copies of the engine's device matrices after smp_exec or smp_init_exec (the
  * reference's RetrieveRankMatrixForTest / RetrievePRMatrixForTest hooks,
  * include/smp/smp_engine_hybrid.cuh:26-32).  rank_out: int32[n*n].  pr_out:
- * int32[n * prefix_cols * 2] as PRNode pairs (woman, rank), include/utils/types.h:58-61;
- * *cols receives prefix_cols. */
+ * int32[n * cols_cap * 2], filled as n rows of *cols PRNode pairs (woman, rank),
+ * include/utils/types.h:58-61, where *cols receives the PR columns the last run
+ * gathered (its smp_times.prefix_cols; the automatic width can grow after a
+ * sparse-pass overflow, and smp_tune can change it).  pr_out == NULL only sets
+ * *cols; a cols_cap below it returns SMP_ERR_INVALID and writes nothing. */
 int smp_rank_matrix(smp_engine* e, int32_t* rank_out);
-int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols);
+int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t cols_cap, int32_t* cols);
 void smp_destroy(smp_engine* e);
 
 /* No counterpart in the reference: for callers that cannot allocate device

@@ -11,7 +11,8 @@ smp_oracle.cpp for the reference functions each one restates (file:line).
 Parity status of each function (what pins it, not what restates it):
   gs_seq, mw_seq, gs_par, mw_par  pinned: brute force (n <= 8, every kind),
       closed forms (first choices, serial dictatorship), mutual agreement,
-      zero blocking pairs, schedule-independent proposal count.
+      zero blocking pairs, schedule-independent proposal count.  With rank=
+      (ranks given): equal to the same engine without it, and to brute force.
   count_blocking_pairs            pinned: hand-built matchings with known counts.
   brute_force                     pinned: hand-worked n <= 3 cases.
   serial_dictatorship             pinned: equals brute force on congested n <= 8.
@@ -62,6 +63,12 @@ def _lib(variant: str = "ref") -> ctypes.CDLL:
     for fn in ("oracle_gs_par", "oracle_mw_par"):
         getattr(lib, fn).argtypes = [ctypes.c_int, i32p, i32p, ctypes.c_int, i32p, tp]
         getattr(lib, fn).restype = ctypes.c_int
+    for fn in ("oracle_gs_seq_ranked", "oracle_mw_seq_ranked"):
+        getattr(lib, fn).argtypes = [ctypes.c_int, i32p, i32p, i32p, i32p, tp]
+        getattr(lib, fn).restype = ctypes.c_int
+    for fn in ("oracle_gs_par_ranked", "oracle_mw_par_ranked"):
+        getattr(lib, fn).argtypes = [ctypes.c_int, i32p, i32p, i32p, ctypes.c_int, i32p, tp]
+        getattr(lib, fn).restype = ctypes.c_int
     lib.oracle_is_perfect.argtypes = [ctypes.c_int, i32p]
     lib.oracle_is_perfect.restype = ctypes.c_int
     lib.oracle_invert_rows.argtypes = [ctypes.c_int, i32p, i32p]
@@ -89,44 +96,61 @@ def _check(n: int, pm: np.ndarray, pw: np.ndarray) -> tuple[np.ndarray, np.ndarr
     return pm, pw
 
 
-def _run_seq(fn: str, n, pm, pw, variant):
+def _rank(n, rank):
+    rank = np.ascontiguousarray(rank, dtype=np.int32).reshape(-1)
+    if rank.size != n * n:
+        raise ValueError(f"expected an {n}x{n} rank matrix")
+    return rank
+
+
+def _run_seq(fn: str, n, pm, pw, variant, rank=None):
     pm, pw = _check(n, pm, pw)
     out = np.full(n, -1, dtype=np.int32)
     t = OracleTimes()
-    st = getattr(_lib(variant), fn)(n, _p(pm), _p(pw), _p(out), ctypes.byref(t))
+    if rank is None:
+        st = getattr(_lib(variant), fn)(n, _p(pm), _p(pw), _p(out), ctypes.byref(t))
+    else:
+        st = getattr(_lib(variant), fn + "_ranked")(n, _p(pm), _p(pw), _p(_rank(n, rank)), _p(out),
+                                                    ctypes.byref(t))
     if st:
         raise RuntimeError(f"{fn} failed ({st})")
     return out, t.as_dict()
 
 
-def _run_par(fn: str, n, pm, pw, threads, variant):
+def _run_par(fn: str, n, pm, pw, threads, variant, rank=None):
     pm, pw = _check(n, pm, pw)
     out = np.full(n, -1, dtype=np.int32)
     t = OracleTimes()
-    st = getattr(_lib(variant), fn)(n, _p(pm), _p(pw), int(threads), _p(out), ctypes.byref(t))
+    if rank is None:
+        st = getattr(_lib(variant), fn)(n, _p(pm), _p(pw), int(threads), _p(out), ctypes.byref(t))
+    else:
+        st = getattr(_lib(variant), fn + "_ranked")(n, _p(pm), _p(pw), _p(_rank(n, rank)), int(threads),
+                                                    _p(out), ctypes.byref(t))
     if st:
         raise RuntimeError(f"{fn} failed ({st})")
     return out, t.as_dict()
 
 
-def gs_seq(n, pm, pw, variant="ref"):
+# rank: the women's inverse rank matrix given as input (rank[w*n + m] = position of
+# man m in woman w's list); init then skips the inversion.  None = built in init.
+def gs_seq(n, pm, pw, variant="ref", rank=None):
     """GS-Seq (SmpEngineGsF9). Returns (match, times)."""
-    return _run_seq("oracle_gs_seq", n, pm, pw, variant)
+    return _run_seq("oracle_gs_seq", n, pm, pw, variant, rank)
 
 
-def mw_seq(n, pm, pw, variant="ref"):
+def mw_seq(n, pm, pw, variant="ref", rank=None):
     """MW-Seq (SmpEngineMwF9). Returns (match, times)."""
-    return _run_seq("oracle_mw_seq", n, pm, pw, variant)
+    return _run_seq("oracle_mw_seq", n, pm, pw, variant, rank)
 
 
-def gs_par(n, pm, pw, threads=0, variant="ref"):
+def gs_par(n, pm, pw, threads=0, variant="ref", rank=None):
     """GS-Par (SmpEngineGs2F9). Returns (match, times)."""
-    return _run_par("oracle_gs_par", n, pm, pw, threads, variant)
+    return _run_par("oracle_gs_par", n, pm, pw, threads, variant, rank)
 
 
-def mw_par(n, pm, pw, threads=0, variant="ref"):
+def mw_par(n, pm, pw, threads=0, variant="ref", rank=None):
     """MW-Par (SmpEngineMw2F9). Returns (match, times)."""
-    return _run_par("oracle_mw_par", n, pm, pw, threads, variant)
+    return _run_par("oracle_mw_par", n, pm, pw, threads, variant, rank)
 
 
 def is_perfect(n, pm) -> bool:

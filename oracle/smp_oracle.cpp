@@ -11,6 +11,10 @@
 //   oracle_mw_seq   SmpEngineMwF9       src/smp/smp_engine_mw_1_f9.cu:15-79
 //   oracle_gs_par   SmpEngineGs2F9      src/smp/smp_engine_gs_2_f9.cu:19-121
 //   oracle_mw_par   SmpEngineMw2F9      src/smp/smp_engine_mw_2_f9.cu:17-96
+//   oracle_*_ranked the same four engines with the women's rank matrix given as
+//                   input (the single-instance 2,000 configuration derives it on the
+//                   host), so init skips the inversion; pinned by equality with
+//                   the engines above and by brute force in the tests
 //   oracle_is_perfect          SmpEngineBamboo::IsPerfect include/smp/smp_engine_bamboo.h:25-33
 //   oracle_invert_rows         F9RankMatrixInitKernel src/smp/kernel.cu:672-686 (host form
 //                              as in smp_engine_gs_1_f9.cu:15-20)
@@ -92,15 +96,21 @@ static void fill_total(oracle_times* t) {
 // GS-Seq (SmpEngineGsF9, smp_engine_gs_1_f9.cu:13-79).
 // One FIFO queue of free men; the current man walks his list until accepted,
 // then the displaced man (if any) is queued and the next free man is popped.
-int oracle_gs_seq(int n, const int* pref_m, const int* pref_w, int* match, oracle_times* t) {
+static int gs_seq_impl(int n, const int* pref_m, const int* pref_w, const int* rank_given, int* match,
+                       oracle_times* t) {
   if (n <= 0 || !pref_m || !pref_w || !match) return 1;
   oracle_times tt{};
   auto t0 = Clock::now();
   tt.precheck_ms = ms_since(t0);  // IsPerfect() is `return false` in this engine
 
   t0 = Clock::now();
-  std::vector<int> rank_w((size_t)n * n);
-  invert_rows(n, pref_w, rank_w.data());
+  std::vector<int> rank_own;
+  const int* rank_w = rank_given;
+  if (!rank_w) {
+    rank_own.resize((size_t)n * n);
+    invert_rows(n, pref_w, rank_own.data());
+    rank_w = rank_own.data();
+  }
   std::queue<int> free_men;
   std::vector<int> partner_rank(n), next(n);
   for (int i = 0; i < n; ++i) {
@@ -143,15 +153,21 @@ int oracle_gs_seq(int n, const int* pref_m, const int* pref_w, int* match, oracl
 // MW-Seq (SmpEngineMwF9, smp_engine_mw_1_f9.cu:15-79).
 // McVitie-Wilson: men enter one at a time; each entering man's chain of
 // displacements runs to completion before the next man enters.
-int oracle_mw_seq(int n, const int* pref_m, const int* pref_w, int* match, oracle_times* t) {
+static int mw_seq_impl(int n, const int* pref_m, const int* pref_w, const int* rank_given, int* match,
+                       oracle_times* t) {
   if (n <= 0 || !pref_m || !pref_w || !match) return 1;
   oracle_times tt{};
   auto t0 = Clock::now();
   tt.precheck_ms = ms_since(t0);
 
   t0 = Clock::now();
-  std::vector<int> rank_w((size_t)n * n);
-  invert_rows(n, pref_w, rank_w.data());
+  std::vector<int> rank_own;
+  const int* rank_w = rank_given;
+  if (!rank_w) {
+    rank_own.resize((size_t)n * n);
+    invert_rows(n, pref_w, rank_own.data());
+    rank_w = rank_own.data();
+  }
   std::vector<int> partner_rank(n, n), next(n, 0);
   tt.init_ms = ms_since(t0);
 
@@ -193,8 +209,8 @@ int oracle_mw_seq(int n, const int* pref_m, const int* pref_w, int* match, oracl
 // thread; women's partner ranks are std::atomic<int> updated by CAS.  Displaced
 // men join the queue of the thread that displaced them.  Thread count is
 // clamped to n (the reference indexes an empty queue otherwise).
-int oracle_gs_par(int n, const int* pref_m, const int* pref_w, int threads, int* match,
-                  oracle_times* t) {
+static int gs_par_impl(int n, const int* pref_m, const int* pref_w, const int* rank_given, int threads, int* match,
+                       oracle_times* t) {
   if (n <= 0 || !pref_m || !pref_w || !match) return 1;
   if (threads <= 0) threads = (int)std::min(std::thread::hardware_concurrency(), 96u);
   threads = std::max(1, std::min(threads, n));
@@ -203,8 +219,13 @@ int oracle_gs_par(int n, const int* pref_m, const int* pref_w, int threads, int*
   tt.precheck_ms = ms_since(t0);
 
   t0 = Clock::now();
-  std::vector<int> rank_w((size_t)n * n);
-  invert_rows(n, pref_w, rank_w.data());
+  std::vector<int> rank_own;
+  const int* rank_w = rank_given;
+  if (!rank_w) {
+    rank_own.resize((size_t)n * n);
+    invert_rows(n, pref_w, rank_own.data());
+    rank_w = rank_own.data();
+  }
   std::vector<std::atomic<int>> partner(n);
   std::vector<std::atomic<int>> next(n);
   for (int i = 0; i < n; ++i) {
@@ -274,8 +295,8 @@ int oracle_gs_par(int n, const int* pref_m, const int* pref_w, int threads, int*
 // MW-Par (SmpEngineMw2F9, smp_engine_mw_2_f9.cu:17-96).
 // Men split into ceil(n/threads)-sized blocks; each thread enters its men one
 // at a time and follows each displacement chain; CAS on atomic partner ranks.
-int oracle_mw_par(int n, const int* pref_m, const int* pref_w, int threads, int* match,
-                  oracle_times* t) {
+static int mw_par_impl(int n, const int* pref_m, const int* pref_w, const int* rank_given, int threads, int* match,
+                       oracle_times* t) {
   if (n <= 0 || !pref_m || !pref_w || !match) return 1;
   if (threads <= 0) threads = (int)std::min(std::thread::hardware_concurrency(), 96u);
   threads = std::max(1, std::min(threads, n));
@@ -284,8 +305,13 @@ int oracle_mw_par(int n, const int* pref_m, const int* pref_w, int threads, int*
   tt.precheck_ms = ms_since(t0);
 
   t0 = Clock::now();
-  std::vector<int> rank_w((size_t)n * n);
-  invert_rows(n, pref_w, rank_w.data());
+  std::vector<int> rank_own;
+  const int* rank_w = rank_given;
+  if (!rank_w) {
+    rank_own.resize((size_t)n * n);
+    invert_rows(n, pref_w, rank_own.data());
+    rank_w = rank_own.data();
+  }
   std::vector<std::atomic<int>> partner(n);
   std::vector<std::atomic<int>> next(n);
   for (int i = 0; i < n; ++i) {
@@ -342,6 +368,38 @@ int oracle_mw_par(int n, const int* pref_m, const int* pref_w, int threads, int*
   fill_total(&tt);
   if (t) *t = tt;
   return 0;
+}
+
+// The four engines as the reference runs them (rank matrix built in init), and
+// with the women's inverse rank matrix given (rank_w[w*n + m] = position of man m
+// in woman w's list; init then only sets up the partner and queue state).
+int oracle_gs_seq(int n, const int* pref_m, const int* pref_w, int* match, oracle_times* t) {
+  return gs_seq_impl(n, pref_m, pref_w, nullptr, match, t);
+}
+int oracle_mw_seq(int n, const int* pref_m, const int* pref_w, int* match, oracle_times* t) {
+  return mw_seq_impl(n, pref_m, pref_w, nullptr, match, t);
+}
+int oracle_gs_par(int n, const int* pref_m, const int* pref_w, int threads, int* match, oracle_times* t) {
+  return gs_par_impl(n, pref_m, pref_w, nullptr, threads, match, t);
+}
+int oracle_mw_par(int n, const int* pref_m, const int* pref_w, int threads, int* match, oracle_times* t) {
+  return mw_par_impl(n, pref_m, pref_w, nullptr, threads, match, t);
+}
+int oracle_gs_seq_ranked(int n, const int* pref_m, const int* pref_w, const int* rank_w, int* match,
+                         oracle_times* t) {
+  return rank_w ? gs_seq_impl(n, pref_m, pref_w, rank_w, match, t) : 1;
+}
+int oracle_mw_seq_ranked(int n, const int* pref_m, const int* pref_w, const int* rank_w, int* match,
+                         oracle_times* t) {
+  return rank_w ? mw_seq_impl(n, pref_m, pref_w, rank_w, match, t) : 1;
+}
+int oracle_gs_par_ranked(int n, const int* pref_m, const int* pref_w, const int* rank_w, int threads, int* match,
+                         oracle_times* t) {
+  return rank_w ? gs_par_impl(n, pref_m, pref_w, rank_w, threads, match, t) : 1;
+}
+int oracle_mw_par_ranked(int n, const int* pref_m, const int* pref_w, const int* rank_w, int threads, int* match,
+                         oracle_times* t) {
+  return rank_w ? mw_par_impl(n, pref_m, pref_w, rank_w, threads, match, t) : 1;
 }
 
 // ---------------------------------------------------------------------------

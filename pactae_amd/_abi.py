@@ -81,7 +81,9 @@ class SmpTimes(ctypes.Structure):
         ("schedule", i32),
         ("sparse_init", i32),
         ("rank_matrix", i32),
-        ("reserved", i32 * 3),
+        ("copy_requeues", i32),
+        ("copy_serialized", i32),
+        ("tail_windows", i32),
     ]
 
     def as_dict(self) -> dict:
@@ -232,7 +234,7 @@ SIGNATURES = [
     ("smp_init_exec", ctypes.c_int, [vp, i32, ctypes.POINTER(SmpInitTimes)]),
     ("smp_tune", ctypes.c_int, [vp, i32, ctypes.POINTER(SmpTuneResult)]),
     ("smp_rank_matrix", ctypes.c_int, [vp, i32p]),
-    ("smp_pr_matrix", ctypes.c_int, [vp, i32p, i32p]),
+    ("smp_pr_matrix", ctypes.c_int, [vp, i32p, i32, i32p]),
     ("smp_destroy", None, [vp]),
     ("smp_exec_device_noalloc", ctypes.c_int, [vp, vp, i32, i32, vp, i64p]),
     ("smp_batch_default_opts", None, [ctypes.POINTER(SmpBatchOpts)]),

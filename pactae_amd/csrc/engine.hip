@@ -252,8 +252,8 @@ int smp_rank_matrix(smp_engine* e, int32_t* rank_out) {
   return SMP_OK;
 }
 
-int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols) {
-  if (!e || !pr_out || !cols) {
+int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t cols_cap, int32_t* cols) {
+  if (!e || !cols) {
     pactae::set_error("null argument");
     return SMP_ERR_INVALID;
   }
@@ -261,8 +261,15 @@ int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols) {
     pactae::set_error("smp_pr_matrix: the last run built no PR prefix (first-choice or master-list path)");
     return SMP_ERR_STATE;
   }
-  SMP_HIP_TRY(hipStreamSynchronize(e->stream));
   const int n = e->n, K = e->K;
+  *cols = K;
+  if (!pr_out) return SMP_OK;  // size query
+  if (cols_cap < K) {
+    pactae::set_error("smp_pr_matrix: the buffer holds " + std::to_string(cols_cap) + " columns per man, the PR prefix has " +
+                      std::to_string(K));
+    return SMP_ERR_INVALID;
+  }
+  SMP_HIP_TRY(hipStreamSynchronize(e->stream));
   std::vector<uint32_t> packed(pactae::pr_entries(n, K));
   SMP_HIP_TRY(hipMemcpy(packed.data(), e->d_pr, packed.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
   for (int m = 0; m < n; ++m)
@@ -272,7 +279,6 @@ int smp_pr_matrix(smp_engine* e, int32_t* pr_out, int32_t* cols) {
       pr_out[2 * i] = (int32_t)pactae::pr_woman(v);
       pr_out[2 * i + 1] = (int32_t)pactae::pr_rank(v);
     }
-  *cols = e->K;
   return SMP_OK;
 }
 

@@ -172,6 +172,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int32_t* h_match = nullptr;  // pinned result (man -> woman)
   int2* d_starts = nullptr;     // chains handed back by the host tail
   bool host_finished = false;   // the host tail completed every chain
+  // Extension windows: when a host chain walks past the host's PR prefix, the
+  // GPU computes the next kWinE entries (woman, rank) of that man's list into
+  // pinned host memory and the host continues the chain itself.  A man's window
+  // replaces his previous one (his list position only grows).  Slots are handed
+  // out per solve, at most kMaxWindows; past that the chains go back to the GPU.
+  static constexpr int kFlagWindow = 2;
+  static constexpr int kWinE = 256;
+  static constexpr int kMaxWindows = 1024;
+  uint32_t* h_win = nullptr;   // kMaxWindows * kWinE entries
+  uint32_t* h_win_dev = nullptr;
+  int4* h_winreq = nullptr;    // (man, first position, slot, 0) per request of a round
+  int4* h_winreq_dev = nullptr;
+  std::vector<int> win_slot, win_start, win_men;
 
   hipStream_t stream = nullptr, copy_stream = nullptr, copy_stream2 = nullptr;
   static constexpr int kMaxBlocks = 8;  // PR prefix blocks copied to the host (K <= 256)
@@ -260,6 +273,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipHostFree(h_parked);
     (void)hipHostFree(h_flags);
     h_flags = h_flags_dev = nullptr;
+    (void)hipHostFree(h_win);
+    (void)hipHostFree(h_winreq);
+    h_win = h_win_dev = nullptr;
+    h_winreq = h_winreq_dev = nullptr;
     d_rank_w = nullptr;
     d_pr = nullptr;
     d_partner = nullptr;
@@ -335,6 +352,17 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error("allocation of the result buffers failed");
       return SMP_ERR_NOMEM;
     }
+    if (hipHostMalloc(&h_win, (size_t)kMaxWindows * kWinE * sizeof(uint32_t), pactae::kHostAllocFlags) !=
+            hipSuccess ||
+        hipHostMalloc(&h_winreq, (size_t)kMaxWindows * sizeof(int4), pactae::kHostAllocFlags) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_win_dev), h_win, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&h_winreq_dev), h_winreq, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      pactae::set_error("allocation of the host tail's extension windows failed");
+      return SMP_ERR_NOMEM;
+    }
+    win_slot.assign(n, -1);
+    win_start.assign(n, 0);
     // Sparse prefix pass buffers, for prefixes up to 256 columns (any width when
     // n <= 2048, where n x n buckets are small).
     K_sparse_alloc = n <= 2048 ? K_alloc : std::min(K_alloc, 256);
@@ -937,18 +965,38 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // runs until its next acceptance, prefetches the row of the man it displaced
   // and yields to the next chain, so the row fetches of several chains overlap
   // (the stable matching does not depend on the order of proposals).  A chain
-  // that walks past the host's PR prefix is handed back: it is collected, and
-  // once every other chain is done the GPU resumes all of them from the shared
-  // partner words (the host never stalls on single-row fetches).
+  // that walks past the host's PR prefix (or past its man's extension window)
+  // stops and is collected; once every other chain is done, the GPU computes an
+  // extension window for each collected man (the next kWinE entries of his list,
+  // by searching the women's rows, or from the rank matrix when one exists) and
+  // the host continues them, in rounds until every chain has ended.  Only if the
+  // windows run out are the collected chains handed back to the GPU, which
+  // resumes them from the shared partner words with the full rank matrix.
   int host_tail() {
     auto t0 = Clock::now();
-    const int parked = h_sc->parked_count;
+    int parked = h_sc->parked_count;
     if (!state_copied) {
       SMP_HIP_TRY(hipMemcpyAsync(h_partner, d_partner, (size_t)n * sizeof(unsigned long long),
                                  hipMemcpyDeviceToHost, stream));
       SMP_HIP_TRY(hipMemcpyAsync(h_parked, d_parked, (size_t)parked * sizeof(int2),
                                  hipMemcpyDeviceToHost, stream));
       SMP_HIP_TRY(wait_stream(stream));
+    }
+    // The GPU finished alone (no prefix copy beside the kernel) but parked chains
+    // at the end of a sparse prefix: bring the prefix over now, so that the host
+    // walks the displaced men's chains from it instead of asking for a window
+    // for each of them.
+    {
+      const int nb = std::min(kMaxBlocks, (std::min(K, n) + pactae::kPrBlock - 1) / pactae::kPrBlock);
+      if (blocks_copied < nb) {
+        for (; blocks_ready < blocks_copied; ++blocks_ready) SMP_HIP_TRY(wait_event(ev_blk[blocks_ready]));
+        const size_t blk = (size_t)n * pactae::kPrBlock;
+        SMP_HIP_TRY(hipMemcpyAsync(h_pr + blocks_copied * blk, d_pr + blocks_copied * blk,
+                                   (size_t)(nb - blocks_copied) * blk * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   stream));
+        SMP_HIP_TRY(wait_stream(stream));
+        blocks_copied = blocks_ready = nb;
+      }
     }
     times.handoff_ms = ms_since(t0);
 
@@ -958,66 +1006,115 @@ struct smp_engine : public pactae::AbsSmpEngine {
       const unsigned long long v = h_partner[w];
       hst[w] = v == pactae::kEmpty ? kHostFree : hs_pack(pactae::pw_rank(v), pactae::pw_man(v), pactae::pw_pos(v));
     }
+    for (int m : win_men) win_slot[m] = -1;  // windows of an earlier solve
+    win_men.clear();
     long long props = 0;
     int bailed = 0;
     const uint32_t* pr = h_pr;
     uint64_t* st = hst.data();
     const int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
-    int2* bail = h_parked;  // handed-back chains overwrite the consumed parked list (bailed <= taken)
+    int2* bail = h_parked;  // stopped chains overwrite the consumed parked list (bailed <= taken)
     int cm[kTailSlots], ck[kTailSlots];
-    int act = 0, taken = 0;
-    // Puts parked chain `taken` into slot s; false if it names a man outside [0,n).
-    auto take = [&](int s) {
-      cm[s] = h_parked[taken].x;
-      ck[s] = h_parked[taken].y;
-      ++taken;
-      return (unsigned)cm[s] < (unsigned)n;
-    };
-    for (; act < kTailSlots && taken < parked; ++act)
-      if (!take(act)) return tail_error("host tail: parked man outside [0,n)");
-    int i = 0;
-    while (act > 0) {
-      if (i >= act) i = 0;
-      int m = cm[i], k = ck[i];
-      bool ended = false;  // chain reached a free woman, or was handed back
-      for (;;) {
-        if (k >= kcap) {
-          bail[bailed++] = make_int2(m, k);
-          ended = true;
-          break;
-        }
-        if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
-          while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(wait_event(ev_blk[blocks_ready++]));
-        }
-        const uint32_t e = pr[pactae::pr_index(m, k, n)];
-        ++props;
-        const uint32_t w = pactae::pr_woman(e), r = pactae::pr_rank(e);
-        if (w >= (uint32_t)n) return tail_error("host tail: preference entry outside [0,n)");
-        const uint64_t v = st[w];
-        if (r < (uint32_t)(v & 0xFFFFu)) {
-          st[w] = hs_pack(r, (uint32_t)m, (uint32_t)k);
-          if ((v & 0xFFFFu) == kHostFree) {
-            ended = true;
+    // Walks the `parked` chains of h_parked; the chains that leave the entries
+    // the host holds are collected in bail[0..bailed).
+    auto walk = [&]() -> int {
+      int act = 0, taken = 0;
+      bailed = 0;
+      // Puts parked chain `taken` into slot s; false if it names a man outside [0,n).
+      auto take = [&](int sl) {
+        cm[sl] = h_parked[taken].x;
+        ck[sl] = h_parked[taken].y;
+        ++taken;
+        return (unsigned)cm[sl] < (unsigned)n;
+      };
+      for (; act < kTailSlots && taken < parked; ++act)
+        if (!take(act)) return tail_error("host tail: parked man outside [0,n)");
+      int i = 0;
+      while (act > 0) {
+        if (i >= act) i = 0;
+        int m = cm[i], k = ck[i];
+        bool ended = false;  // chain reached a free woman, or left the entries the host holds
+        for (;;) {
+          uint32_t e;
+          if (k < kcap) {
+            if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
+              while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(wait_event(ev_blk[blocks_ready++]));
+            }
+            e = pr[pactae::pr_index(m, k, n)];
+          } else {
+            const int sl = win_slot[m];
+            const int off = k - win_start[m];
+            if (sl < 0 || off < 0 || off >= kWinE) {
+              bail[bailed++] = make_int2(m, k);
+              ended = true;
+              break;
+            }
+            e = h_win[(size_t)sl * kWinE + off];
+            if (e == pactae::kWinInvalid)
+              return tail_error(k >= n ? "host tail: a man exhausted his list" : "host tail: malformed list entry");
+          }
+          ++props;
+          const uint32_t w = pactae::pr_woman(e), r = pactae::pr_rank(e);
+          if (w >= (uint32_t)n) return tail_error("host tail: preference entry outside [0,n)");
+          const uint64_t v = st[w];
+          if (r < (uint32_t)(v & 0xFFFFu)) {
+            st[w] = hs_pack(r, (uint32_t)m, (uint32_t)k);
+            if ((v & 0xFFFFu) == kHostFree) {
+              ended = true;
+              break;
+            }
+            m = (int)((v >> 16) & 0xFFFFu);
+            k = (int)(v >> 32) + 1;
+            if (k < kcap) __builtin_prefetch(pr + pactae::pr_index(m, k, n));
             break;
           }
-          m = (int)((v >> 16) & 0xFFFFu);
-          k = (int)(v >> 32) + 1;
-          if (k < kcap) __builtin_prefetch(pr + pactae::pr_index(m, k, n));
-          break;
+          ++k;
         }
-        ++k;
+        if (!ended) {
+          cm[i] = m;
+          ck[i] = k;
+          ++i;
+        } else if (taken < parked) {
+          if (!take(i)) return tail_error("host tail: parked man outside [0,n)");
+        } else {
+          --act;
+          cm[i] = cm[act];
+          ck[i] = ck[act];
+        }
       }
-      if (!ended) {
-        cm[i] = m;
-        ck[i] = k;
-        ++i;
-      } else if (taken < parked) {
-        if (!take(i)) return tail_error("host tail: parked man outside [0,n)");
-      } else {
-        --act;
-        cm[i] = cm[act];
-        ck[i] = ck[act];
+      return SMP_OK;
+    };
+    for (;;) {
+      if (int rc = walk()) return rc;
+      if (bailed == 0) break;
+      // Windows for the stopped chains (a man keeps his slot; his new window
+      // starts where he stopped).  Out of slots: hand them back to the GPU.
+      int need = 0;
+      for (int b = 0; b < bailed; ++b) need += win_slot[bail[b].x] < 0;
+      if ((int)win_men.size() + need > kMaxWindows) break;
+      for (int b = 0; b < bailed; ++b) {
+        const int m = bail[b].x;
+        if (win_slot[m] < 0) {
+          win_slot[m] = (int)win_men.size();
+          win_men.push_back(m);
+        }
+        win_start[m] = bail[b].y;
+        h_winreq[b] = make_int4(m, bail[b].y, win_slot[m], 0);
       }
+      // Search the women's rows while that reads less than building the rank
+      // matrix would (6 n^2 bytes); otherwise build it once and gather.
+      const bool have_rank = rank_built || d_rank_in;
+      if (!have_rank && (long long)bailed * kWinE * 4 > 6ll * n) {
+        if (int rc = ensure_rank_matrix()) return rc;
+      }
+      const bool gather = rank_built || d_rank_in;
+      const unsigned seq = ++flag_seq;
+      SMP_HIP_TRY(pactae::launch_window(d_pref_m, d_pref_w, gather && !d_rank_in ? d_rank_w : nullptr, d_rank_in, n,
+                                        h_winreq_dev, bailed, kWinE, h_win_dev, d_sc, h_flags_dev + kFlagWindow, seq,
+                                        stream));
+      SMP_HIP_TRY(wait_flag(kFlagWindow, seq));
+      times.tail_windows += bailed;
+      parked = bailed;  // the stopped chains, in h_parked[0..bailed), go round again
     }
     times.host_proposals = props;
     times.bailed = bailed;
@@ -1047,7 +1144,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     // Resume the handed-back chains on the GPU (no hand-off this time), with the
     // rank matrix for the positions past the prefix.
     t0 = Clock::now();
-    if (int st = ensure_rank_matrix()) return st;
+    if (int rc = ensure_rank_matrix()) return rc;
     SMP_HIP_TRY(hipMemcpyAsync(d_partner, h_partner, (size_t)n * sizeof(unsigned long long),
                                hipMemcpyHostToDevice, stream));
     SMP_HIP_TRY(hipMemcpyAsync(d_starts, bail, (size_t)bailed * sizeof(int2), hipMemcpyHostToDevice,
@@ -1118,7 +1215,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
         hipEventElapsedTime(&copy1_ms, ev[2], ev_blk[1]) == hipSuccess)
       copy_ms = std::max(copy_ms, copy1_ms);
     (void)hipGetLastError();
-    if (kern_ms > 0.12f && copy_ms >= kern_ms && requeues < 8) {
+    const bool waited = kern_ms > 0.12f && copy_ms >= kern_ms;
+    times.copy_serialized = waited && requeues >= 8 ? 1 : 0;
+    if (waited && requeues < 8) {
       (void)hipStreamSynchronize(copy_stream);
       (void)hipStreamSynchronize(copy_stream2);
       hipStream_t a = nullptr, b = nullptr;
@@ -1194,6 +1293,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.prefix_cols = K;
     times.tail_threshold = T;
     if (!times.is_perfect && T > 0 && blocks_copied > 0 && opts.prefix_copy == 0) check_copy_overlap();
+    times.copy_requeues = requeues;
     take_span(span_bucket, &times.bucket_kernel_ms);
     take_span(span_rank, &times.rank_kernel_ms);
     take_span(span_gather, &times.gather_kernel_ms);

@@ -820,6 +820,100 @@ __global__ __launch_bounds__(256) void k_export_state(Scalars* sc, const unsigne
   }
 }
 
+// Extension windows for the host tail: for request i = (m, k0, slot) and j < E,
+// entry (w, rank) of position k0 + j of man m's list goes to out[slot * E + j]
+// (pinned host memory, written through its device-visible address), or
+// kWinInvalid for a position past the list's end or a malformed entry.  The rank
+// comes from the rank matrix when one exists (rank16, or rank32 given as input),
+// else from a search of woman w's row for m by one workgroup per entry.  The last
+// workgroup to finish releases the writes at system scope and stores seq into
+// *h_flag.
+__device__ inline void publish_blocks(Scalars* sc, unsigned* h_flag, unsigned seq) {
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&sc->export_blocks, 1) == (int)gridDim.x - 1) {
+    sc->export_blocks = 0;
+    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_window_search(const int32_t* __restrict__ pref_m,
+                                                       const int32_t* __restrict__ pref_w, int n,
+                                                       const int4* __restrict__ req, int E,
+                                                       uint32_t* __restrict__ out, Scalars* sc, unsigned* h_flag,
+                                                       unsigned seq) {
+  __shared__ int found;
+  const int i = (int)blockIdx.x / E, j = (int)blockIdx.x - i * E;
+  const int4 q = req[i];
+  const int m = q.x, k = q.y + j;
+  uint32_t* dst = out + (size_t)q.z * E + j;
+  if (threadIdx.x == 0) found = -1;
+  __syncthreads();
+  int w = -1;
+  if ((unsigned)m < (unsigned)n && (unsigned)k < (unsigned)n) w = pref_m[(size_t)m * n + k];
+  if ((unsigned)w < (unsigned)n) {
+    const int32_t* row = pref_w + (size_t)w * n;
+    if ((n & 3) == 0) {
+      const int4* row4 = reinterpret_cast<const int4*>(row);
+      const int nq = n >> 2;
+      for (int c0 = (int)threadIdx.x; c0 < nq; c0 += 256 * 4) {
+        int4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = row4[min(c0 + 256 * u, nq - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int c = c0 + 256 * u;
+          if (c < nq) {
+            if (v[u].x == m) found = 4 * c;
+            if (v[u].y == m) found = 4 * c + 1;
+            if (v[u].z == m) found = 4 * c + 2;
+            if (v[u].w == m) found = 4 * c + 3;
+          }
+        }
+      }
+    } else {
+      for (int r = (int)threadIdx.x; r < n; r += 256)
+        if (row[r] == m) found = r;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int r = found;
+    if ((unsigned)w < (unsigned)n && r >= 0) {
+      *dst = pr_pack((uint32_t)w, (uint32_t)r);
+    } else {
+      *dst = kWinInvalid;
+      if ((unsigned)k < (unsigned)n) atomicOr(&sc->error, (unsigned)w < (unsigned)n ? kErrNotPerm : kErrEntryRange);
+    }
+  }
+  publish_blocks(sc, h_flag, seq);
+}
+
+template <typename RankT>
+__global__ __launch_bounds__(256) void k_window_gather(const int32_t* __restrict__ pref_m,
+                                                       const RankT* __restrict__ rank_w, int n,
+                                                       const int4* __restrict__ req, int count, int E,
+                                                       uint32_t* __restrict__ out, Scalars* sc, unsigned* h_flag,
+                                                       unsigned seq) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < count * E) {
+    const int i = t / E, j = t - i * E;
+    const int4 q = req[i];
+    const int m = q.x, k = q.y + j;
+    uint32_t e = kWinInvalid;
+    if ((unsigned)m < (unsigned)n && (unsigned)k < (unsigned)n) {
+      const int w = pref_m[(size_t)m * n + k];
+      if ((unsigned)w < (unsigned)n) {
+        const uint32_t r = (uint32_t)rank_w[(size_t)w * n + m];
+        if (r < (uint32_t)n) e = pr_pack((uint32_t)w, r);
+      }
+      if (e == kWinInvalid) atomicOr(&sc->error, kErrEntryRange);
+    }
+    out[(size_t)q.z * E + j] = e;
+  }
+  publish_blocks(sc, h_flag, seq);
+}
+
 // Precheck publish: the control block into pinned host memory, then seq into
 // the host's flag word with a system-scope release.  Queued behind the
 // precheck, it stands in for a copy and an event, each of which would leave
@@ -1416,6 +1510,23 @@ hipError_t launch_publish_scalars(const Scalars* sc, Scalars* h_sc, unsigned* h_
 
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_fold_wstats, dim3(1), dim3(256), 0, s, wstats, waves, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_window(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank16, const int32_t* rank32,
+                         int n, const int4* req, int count, int E, uint32_t* out, Scalars* sc, unsigned* h_flag,
+                         unsigned seq, hipStream_t s) {
+  if (count <= 0) return hipErrorInvalidValue;
+  if (rank32) {
+    hipLaunchKernelGGL(k_window_gather<int32_t>, dim3((count * E + 255) / 256), dim3(256), 0, s, pref_m, rank32, n,
+                       req, count, E, out, sc, h_flag, seq);
+  } else if (rank16) {
+    hipLaunchKernelGGL(k_window_gather<uint16_t>, dim3((count * E + 255) / 256), dim3(256), 0, s, pref_m, rank16, n,
+                       req, count, E, out, sc, h_flag, seq);
+  } else {
+    hipLaunchKernelGGL(k_window_search, dim3(count * E), dim3(256), 0, s, pref_m, pref_w, n, req, E, out, sc, h_flag,
+                       seq);
+  }
   return hipGetLastError();
 }
 

@@ -82,6 +82,14 @@ hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int 
                               const int* cnt, const uint32_t* bucket, int cap, uint16_t* brank, uint32_t* pr,
                               Scalars* sc, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
+// Extension windows for the host tail: E list entries (woman, rank) from position
+// req[i].y of man req[i].x into out[req[i].z * E ..] (pinned host memory, device
+// address), kWinInvalid past the list's end.  Ranks from rank32 or rank16 when
+// given, else by searching the woman's row (pref_w).  The kernel then stores seq
+// into *h_flag (pinned) once every entry is visible to the host.
+hipError_t launch_window(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank16, const int32_t* rank32,
+                         int n, const int4* req, int count, int E, uint32_t* out, Scalars* sc, unsigned* h_flag,
+                         unsigned seq, hipStream_t s);
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);
 // Folds the proposal waves' counters, then writes the control block, the partner
 // words and the parked chains into host memory (device-visible pinned pointers)

@@ -30,6 +30,9 @@ SMP_HD uint32_t pr_rank(uint32_t e) { return e & 0xFFFFu; }
 // the first block (positions 0..31 of every man, where nearly all late proposals
 // fall) is one contiguous piece that the host can copy and cache first.
 constexpr int kPrBlock = 32;
+// A list entry the extension windows of the host tail could not fill (past the
+// list's end, or malformed lists).
+constexpr uint32_t kWinInvalid = 0xFFFFFFFFu;
 SMP_HD size_t pr_index(int m, int k, int n) {
   return ((size_t)(k >> 5) * (size_t)n + (size_t)m) * kPrBlock + (size_t)(k & (kPrBlock - 1));
 }

@@ -306,11 +306,11 @@ class SmpEngineBamboo:
 
     def pr_matrix(self) -> np.ndarray:
         """PR matrix as PRNode pairs: array [n, cols, 2] of (woman, rank)."""
-        cols = self.profile()["prefix_cols"] or min(self.n, 128)
-        out = np.empty(self.n * max(cols, self.n) * 2, dtype=np.int32)
         c = ctypes.c_int32(0)
-        _check(_lib.smp_pr_matrix(self._h, _i32p(out), ctypes.byref(c)), "smp_pr_matrix")
-        return out[: self.n * c.value * 2].reshape(self.n, c.value, 2)
+        _check(_lib.smp_pr_matrix(self._h, None, 0, ctypes.byref(c)), "smp_pr_matrix")
+        out = np.empty(self.n * c.value * 2, dtype=np.int32)
+        _check(_lib.smp_pr_matrix(self._h, _i32p(out), c.value, ctypes.byref(c)), "smp_pr_matrix")
+        return out.reshape(self.n, c.value, 2)
 
     def close(self) -> None:
         if self._h:

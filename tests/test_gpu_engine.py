@@ -308,6 +308,42 @@ def test_config2_uniform_20000(gpu, oracle):
     assert oracle.count_blocking_pairs(n, pm, pw, want) == 0
 
 
+def test_tail_extension_windows_20000(gpu, oracle):
+    """Seed 0x5EED0007 of the n = 20,000 uniform workload has two men whose final
+    list positions are 128 and 131, past a 128-column prefix.  The host tail asks
+    the GPU for extension windows of their lists and finishes every chain itself
+    (nothing handed back); with the GPU finishing alone, the parked chains take
+    the same route.  Both bit-exact against the oracle."""
+    smp = gpu
+    n = 20000
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=0x5EED0007)
+    want, t = oracle.gs_seq(n, pm, pw)
+    pos = np.argmax(pm.reshape(n, n) == want[:, None], axis=1)
+    assert int(pos.max()) >= 128
+    for cfg in (dict(prefix_cols=128, tail_mode=2, tail_threshold=24),
+                dict(prefix_cols=96, tail_mode=2, tail_threshold=64),
+                dict(prefix_cols=128, tail_mode=1)):
+        got, prof = solve(smp, n, pm, pw, **cfg)
+        assert np.array_equal(got, want), cfg
+        assert prof["proposals"] == t["proposals"], cfg
+        assert prof["tail_windows"] > 0 and prof["bailed"] == 0, (cfg, prof["tail_windows"], prof["bailed"])
+        assert prof["rank_matrix"] == 0, cfg
+
+
+def test_tail_windows_run_out_falls_back(gpu, oracle):
+    """More stopped chains than extension windows (a 1-column prefix, 1,500 men
+    handed to the host): the chains go back to the GPU, which resumes them with
+    the full rank matrix.  Bit-exact either way."""
+    smp = gpu
+    n = 3000
+    pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=77)
+    want, t = oracle.gs_seq(n, pm, pw)
+    got, prof = solve(smp, n, pm, pw, prefix_cols=1, tail_mode=2, tail_threshold=1500, init_mode=2)
+    assert np.array_equal(got, want)
+    assert prof["proposals"] == t["proposals"]
+    assert prof["bailed"] > 0 and prof["rank_matrix"] == 1
+
+
 def test_config3_congested_and_shifted_20000(gpu, oracle):
     smp = gpu
     n = 20000

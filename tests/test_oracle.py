@@ -98,6 +98,30 @@ def test_engines_equal_brute_force(oracle, smp, kind):
             assert t_mw["proposals"] == expected_props
 
 
+@pytest.mark.parametrize("kind", [0, 1, 3, 4, 7])
+def test_ranked_engines_equal_brute_force_and_unranked(oracle, smp, kind):
+    """The engines with the women's rank matrix given (numpy argsort of their
+    lists, derived here) answer what brute force answers for n <= 8, and what the
+    same engine answers from the lists at n = 400."""
+    for n in (4, 6, 8, 400):
+        pm, pw = smp.generate(kind, n, seed=n + 17, param=2 if kind == smp.Workload.REF_RANDOM else 0)
+        rank = np.argsort(pw.reshape(n, n), axis=1).astype(np.int32)
+        want = oracle.brute_force(n, pm, pw)[0] if n <= 8 else oracle.gs_seq(n, pm, pw)[0]
+        for fn in (oracle.gs_seq, oracle.mw_seq):
+            got, t = fn(n, pm, pw, rank=rank)
+            assert np.array_equal(got, want), (fn.__name__, kind, n)
+            assert t["proposals"] == fn(n, pm, pw)[1]["proposals"]
+        for fn in (oracle.gs_par, oracle.mw_par):
+            got, _ = fn(n, pm, pw, threads=3, rank=rank)
+            assert np.array_equal(got, want), (fn.__name__, kind, n)
+    # A transposed rank matrix (the men's ranks where the women's belong) must not pass.
+    n = 200
+    pm, pw = smp.generate(0, n, seed=5)
+    rank = np.argsort(pw.reshape(n, n), axis=1).astype(np.int32)
+    bad, _ = oracle.gs_seq(n, pm, pw, rank=np.ascontiguousarray(rank.T))
+    assert oracle.count_blocking_pairs(n, pm, pw, bad) != 0
+
+
 def test_brute_force_on_exhaustive_n3(oracle):
     # Every pair of preference profiles for n = 3 with the men's side fixed to a
     # handful of profiles: engines == brute force on all of them.
