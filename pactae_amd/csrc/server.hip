@@ -169,6 +169,18 @@ struct smp_server {
   std::thread worker;
   smp_server_stats stats{};
   int inflight = 0;
+  // Work counters read without the lock by threads that spin briefly for new
+  // work before they block (a wake-up through a condition variable costs tens
+  // of microseconds, a visible share of a small request's latency).
+  std::atomic<uint64_t> submits{0};
+  std::atomic<int> large_queued{0};
+  static constexpr auto kSpin = std::chrono::microseconds(300);
+  template <typename Pred>
+  static void spin_for(Pred ready) {
+    const auto end = Clock::now() + kSpin;
+    while (!ready() && Clock::now() < end) {
+    }
+  }
 
   bool is_large(const Request& r) const { return opts.mode == SMP_SERVE_BATCHED && r.n >= opts.large_n; }
 
@@ -261,10 +273,16 @@ struct smp_server {
     (void)hipSetDevice(opts.device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
+      if (!stop_large && ready_large.empty()) {
+        lk.unlock();
+        spin_for([this] { return large_queued.load(std::memory_order_acquire) > 0; });
+        lk.lock();
+      }
       cv_large.wait(lk, [this] { return stop_large || !ready_large.empty(); });
       if (ready_large.empty()) break;
       Request* r = ready_large.front();
       ready_large.pop_front();
+      large_queued.fetch_sub(1, std::memory_order_relaxed);
       solve_plain(lk, r, -1, eng);
       ++stats.large_solved;
       cv_work.notify_one();  // its arena block is free for admission
@@ -316,6 +334,7 @@ struct smp_server {
     pending.clear();
     ready.clear();
     ready_large.clear();
+    large_queued.store(0);
     cv_done.notify_all();
   }
 
@@ -362,13 +381,35 @@ struct smp_server {
       }
       r->state = kReady;
       if (is_large(*r) && !large_workers.empty()) {
-        ready_large.push_back(r);
-        cv_large.notify_one();
+        push_large(r);
       } else {
         ready.push_back(r);
       }
     }
     return hipSuccess;
+  }
+
+  void push_large(Request* r) {  // with mu held
+    ready_large.push_back(r);
+    large_queued.fetch_add(1, std::memory_order_release);
+    cv_large.notify_one();
+  }
+
+  // A large request with its lists in HBM needs no arena room of its own (the
+  // worker's pipeline has its buffers), so when nothing waits ahead of it in
+  // arrival order it goes straight to the workers from the submitting thread.
+  bool admit_large_now(Request* r) {  // with mu held
+    if (opts.mode != SMP_SERVE_BATCHED || !r->on_device || !is_large(*r) || large_workers.empty() ||
+        !pending.empty())
+      return false;
+    r->bytes = footprint(*r);
+    if (!arena.alloc(r->bytes, &r->off)) return false;
+    r->has_block = true;
+    ++inflight;
+    stats.peak_inflight = std::max<int32_t>(stats.peak_inflight, inflight);
+    r->state = kReady;
+    push_large(r);
+    return true;
   }
 
   hipError_t launch(Lane& L, int lane_id) {  // with mu held
@@ -482,8 +523,10 @@ struct smp_server {
         for (auto& L : lanes) busy += L.busy;
         stats.peak_lanes_busy = std::max<int32_t>(stats.peak_lanes_busy, busy);
         if (busy && !progressed) {
+          // A lane is in flight: poll again soon (a timed sleep would last tens
+          // of microseconds past its request, the timer slack).
           lk.unlock();
-          std::this_thread::sleep_for(std::chrono::microseconds(20));
+          std::this_thread::yield();
           lk.lock();
           continue;
         }
@@ -492,6 +535,13 @@ struct smp_server {
       stats.peak_arena_bytes = (int64_t)arena.peak();
       const bool idle = pending.empty() && ready.empty() && inflight == 0;
       if (stop && (idle || fatal)) break;
+      if (!progressed && pending.empty() && !stop) {
+        const uint64_t seen = submits.load(std::memory_order_acquire);
+        lk.unlock();
+        spin_for([&] { return submits.load(std::memory_order_acquire) != seen; });
+        lk.lock();
+        if (submits.load(std::memory_order_acquire) != seen) continue;
+      }
       if (!progressed) cv_work.wait_for(lk, std::chrono::milliseconds(2));
     }
   }
@@ -572,9 +622,12 @@ int smp_server_submit(smp_server* s, const int32_t* pm, const int32_t* pw, int32
     }
     r->ticket = s->next_ticket++;
     *ticket = r->ticket;
-    s->pending.push_back(r.get());
+    smp_server::Request* rp = r.get();
     s->table.emplace(r->ticket, std::move(r));
     ++s->stats.submitted;
+    if (s->admit_large_now(rp)) return SMP_OK;
+    s->pending.push_back(rp);
+    s->submits.fetch_add(1, std::memory_order_release);
   }
   s->cv_work.notify_one();
   return SMP_OK;
