@@ -220,6 +220,17 @@ def ranked_leg(smp, args, n, pm, pw, dm, match):
     return res
 
 
+def cpu_check_fig9(legs, keep):
+    """GS-Seq on the Fig 9 legs' instances: the bit-exact check of their GPU
+    matchings, and one CPU solve each (the reference's host flags) beside them."""
+    from oracle import oracle
+    for res, (n, pm, pw, match) in zip(legs, keep):
+        want, t = oracle.gs_seq(n, pm, pw)
+        res["bit_exact"] = bool(np.array_equal(want, match))
+        res["cpu_gs_seq_ms"] = round(t["total_ms"], 1)
+        log(f"cpu GS-Seq on {res['workload']}: {t['total_ms']:.0f} ms, bit-exact {res['bit_exact']}")
+
+
 def cpu_baseline_single(args, n, pm, pw, gpu_match, inst):
     """The oracle's restatements of the reference's four Fig 9 CPU engines on this
     host's cores, on the tuning seed's instance: 1 warm-up + 5 timed solves each,
@@ -276,6 +287,36 @@ def cpu_baseline_single(args, n, pm, pw, gpu_match, inst):
             "sample": f"the same n={n} instance (the tuning seed), 1 warm-up + {runs} timed solves per engine and "
                       f"build, precheck+init+core as AbsSmpEngine times it ({cpu_s:.0f} s of CPU work in all)"}, \
         ok and stable and all(seeds_ok), seeds_ok
+
+
+def fig9_legs(smp, args):
+    """The reference's own Fig 9 inputs (benchmark/figure_9.cu:40-57) at n = 20,000,
+    beside the headline: RANDOM with groups of 5 (about n^2 / 2 proposals under
+    heavy contention) and SOLO (one displacement chain of n^2 - n + 1 proposals,
+    walked by the host tail from the whole PR matrix).  Default engine options;
+    one warm-up solve (it allocates the whole-matrix buffers), then timed solves."""
+    out, keep = [], []
+    for name, kind, param, runs in (("REF_RANDOM(group 5)", smp.Workload.REF_RANDOM, 5, 3),
+                                    ("REF_SOLO", smp.Workload.REF_SOLO, 0, 1)):
+        n = args.fig9_n
+        pm, pw = smp.generate(kind, n, seed=args.seed + 0x90, param=param, threads=16)
+        dm, dw = smp.DeviceArray(pm), smp.DeviceArray(pw)
+        with smp.SmpEngineBamboo.from_device(dm, dw, n) as e:
+            e.exec()
+            walls = _timed(e, runs)
+            p = e.profile()
+            match = e.result()
+        dm.free()
+        dw.free()
+        res = {"workload": name, "n": n, "seed": args.seed + 0x90, "ms_per_solve": round(statistics.median(walls), 3),
+               "runs": runs, "proposals": p["proposals"], "propose_kernel_ms": round(p["propose_kernel_ms"], 3),
+               "host_tail_ms": round(p["host_tail_ms"], 3), "host_proposals": p["host_proposals"],
+               "tail_windows": p["tail_windows"], "tail_full_matrix": p["tail_full_matrix"],
+               "resume_ms": round(p["resume_ms"], 3)}
+        log(f"fig9 {name}: {res}")
+        out.append(res)
+        keep.append((n, pm, pw, match))
+    return out, keep
 
 
 def batched(smp, args):
@@ -458,6 +499,8 @@ def main() -> int:
                     help="only run the 16-byte streaming copy kernel over this many MiB "
                          "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-fig9", action="store_true", help="skip the reference's RANDOM(5) and SOLO legs")
+    ap.add_argument("--fig9-n", type=int, default=20000)
     ap.add_argument("--no-ranked", action="store_true",
                     help="skip the leg with the women's rank matrix given as input")
     args = ap.parse_args()
@@ -545,6 +588,12 @@ def main() -> int:
         out["bit_exact"] = ok
         out["bit_exact_timing_seeds"] = seeds_ok
         out["speedup_vs_cpu"] = round(cpu["value"] / value, 2)
+    if not args.no_fig9:
+        legs, keep = fig9_legs(smp, args)
+        if not args.no_cpu:
+            cpu_check_fig9(legs, keep)
+        del keep
+        out["fig9"] = legs
     if not args.no_batch:
         bres, _ = batched(smp, args)
         out["batched"] = bres

@@ -200,6 +200,10 @@ typedef struct smp_times {
   int32_t tail_windows;    /* extension windows (the next 256 entries of one man's
                               list) the GPU computed for host-tail chains that
                               walked past the PR prefix                           */
+  int32_t tail_full_matrix; /* 1 if the host tail's chains went so deep that the
+                              whole PR matrix was gathered and copied to the host
+                              (4 n^2 bytes, buffers allocated on first use)       */
+  int32_t reserved[3];
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;

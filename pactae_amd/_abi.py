@@ -84,6 +84,8 @@ class SmpTimes(ctypes.Structure):
         ("copy_requeues", i32),
         ("copy_serialized", i32),
         ("tail_windows", i32),
+        ("tail_full_matrix", i32),
+        ("reserved", i32 * 3),
     ]
 
     def as_dict(self) -> dict:

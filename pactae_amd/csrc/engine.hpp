@@ -179,12 +179,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // out per solve, at most kMaxWindows; past that the chains go back to the GPU.
   static constexpr int kFlagWindow = 2;
   static constexpr int kWinE = 256;
-  static constexpr int kMaxWindows = 1024;
+  static constexpr int kMaxWindows = 4096;
   uint32_t* h_win = nullptr;   // kMaxWindows * kWinE entries
   uint32_t* h_win_dev = nullptr;
   int4* h_winreq = nullptr;    // (man, first position, slot, 0) per request of a round
   int4* h_winreq_dev = nullptr;
   std::vector<int> win_slot, win_start, win_men;
+  // Rounds of windows after which the host takes the whole PR matrix instead
+  // (allocated on first use: 4 n^2 bytes in HBM and as much pinned host memory):
+  // when the rounds' GPU round trips (~50 us each) have cost about what copying
+  // the matrix over PCIe (~50 GB/s) would, 1.6e-6 n^2 rounds, and at least 8.
+  int window_rounds_limit() const { return std::max(8, (int)std::min<long long>(1 << 20, 16ll * n * n / 10000000)); }
+  uint32_t* d_pr_full = nullptr;
+  uint32_t* h_pr_full = nullptr;
 
   hipStream_t stream = nullptr, copy_stream = nullptr, copy_stream2 = nullptr;
   static constexpr int kMaxBlocks = 8;  // PR prefix blocks copied to the host (K <= 256)
@@ -273,6 +280,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipHostFree(h_parked);
     (void)hipHostFree(h_flags);
     h_flags = h_flags_dev = nullptr;
+    (void)hipFree(d_pr_full);
+    (void)hipHostFree(h_pr_full);
+    d_pr_full = h_pr_full = nullptr;
     (void)hipHostFree(h_win);
     (void)hipHostFree(h_winreq);
     h_win = h_win_dev = nullptr;
@@ -958,6 +968,41 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return true;
   }
 
+  // The whole PR matrix (every position of every man's list with its rank, in
+  // the column-blocked PR layout) gathered on the GPU into d_pr_full and copied
+  // into h_pr_full, for a host tail whose chains walk deep into the lists (the
+  // reference copies its whole PR matrix to the host for the same purpose,
+  // smp_engine_bamboo.h:41-61).  Leaves h_pr_full null, and returns SMP_OK, if
+  // the buffers cannot be allocated: the caller then keeps to windows.
+  int host_full_matrix() {
+    const size_t entries = pactae::pr_entries(n_alloc, n_alloc);
+    if (!d_pr_full) {
+      if (hipMalloc(&d_pr_full, entries * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        d_pr_full = nullptr;
+        return SMP_OK;
+      }
+    }
+    if (!h_pr_full) {
+      if (hipHostMalloc(&h_pr_full, entries * sizeof(uint32_t), pactae::kHostAllocFlags) != hipSuccess) {
+        (void)hipGetLastError();
+        h_pr_full = nullptr;
+        return SMP_OK;
+      }
+    }
+    if (d_rank_in) {
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_in, d_pr_full, n, n, d_sc, stream));
+    } else {
+      if (int rc = ensure_rank_matrix()) return rc;
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr_full, n, n, d_sc, stream));
+    }
+    SMP_HIP_TRY(hipMemcpyAsync(h_pr_full, d_pr_full, pactae::pr_entries(n, n) * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, stream));
+    SMP_HIP_TRY(wait_stream(stream));
+    times.tail_full_matrix = 1;
+    return SMP_OK;
+  }
+
   // Host tail: McVitie-Wilson chains (smp_engine_mw_1_f9.cu:15-79, and the
   // reference's host LAProcedure smp_engine_bamboo.cu:197-230) over the partner
   // state the kernel left, kept as one packed word per woman (hst) so that it
@@ -1012,7 +1057,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     int bailed = 0;
     const uint32_t* pr = h_pr;
     uint64_t* st = hst.data();
-    const int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
+    int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
+    bool full = false;  // the host holds every man's whole list (pr = h_pr_full)
     int2* bail = h_parked;  // stopped chains overwrite the consumed parked list (bailed <= taken)
     int cm[kTailSlots], ck[kTailSlots];
     // Walks the `parked` chains of h_parked; the chains that leave the entries
@@ -1036,7 +1082,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
         bool ended = false;  // chain reached a free woman, or left the entries the host holds
         for (;;) {
           uint32_t e;
-          if (k < kcap) {
+          if (full) {
+            if (k >= n) return tail_error("host tail: a man exhausted his list");
+            e = pr[pactae::pr_index(m, k, n)];
+          } else if (k < kcap) {
             if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
               while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(wait_event(ev_blk[blocks_ready++]));
             }
@@ -1084,9 +1133,24 @@ struct smp_engine : public pactae::AbsSmpEngine {
       }
       return SMP_OK;
     };
+    int rounds = 0;
     for (;;) {
       if (int rc = walk()) return rc;
       if (bailed == 0) break;
+      // Many rounds of windows mean chains that walk deep into long lists (one
+      // GPU round trip per 256 positions of a man): the whole PR matrix is then
+      // gathered on the GPU and copied to the host once, and the host walks
+      // every chain from it.
+      if (rounds >= window_rounds_limit()) {
+        if (int rc = host_full_matrix()) return rc;
+        if (h_pr_full) {
+          pr = h_pr_full;
+          kcap = n;
+          full = true;
+          parked = bailed;
+          continue;
+        }
+      }
       // Windows for the stopped chains (a man keeps his slot; his new window
       // starts where he stopped).  Out of slots: hand them back to the GPU.
       int need = 0;
@@ -1113,6 +1177,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
                                         h_winreq_dev, bailed, kWinE, h_win_dev, d_sc, h_flags_dev + kFlagWindow, seq,
                                         stream));
       SMP_HIP_TRY(wait_flag(kFlagWindow, seq));
+      ++rounds;
       times.tail_windows += bailed;
       parked = bailed;  // the stopped chains, in h_parked[0..bailed), go round again
     }

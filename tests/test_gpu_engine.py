@@ -331,17 +331,35 @@ def test_tail_extension_windows_20000(gpu, oracle):
 
 
 def test_tail_windows_run_out_falls_back(gpu, oracle):
-    """More stopped chains than extension windows (a 1-column prefix, 1,500 men
-    handed to the host): the chains go back to the GPU, which resumes them with
-    the full rank matrix.  Bit-exact either way."""
+    """More stopped chains than extension windows (a 1-column prefix, 5,000 men
+    handed to the host, 4,096 windows): the chains go back to the GPU, which
+    resumes them with the full rank matrix.  Bit-exact either way."""
     smp = gpu
-    n = 3000
+    n = 6000
     pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=77)
     want, t = oracle.gs_seq(n, pm, pw)
-    got, prof = solve(smp, n, pm, pw, prefix_cols=1, tail_mode=2, tail_threshold=1500, init_mode=2)
+    got, prof = solve(smp, n, pm, pw, prefix_cols=1, tail_mode=2, tail_threshold=5000, init_mode=2)
     assert np.array_equal(got, want)
     assert prof["proposals"] == t["proposals"]
     assert prof["bailed"] > 0 and prof["rank_matrix"] == 1
+
+
+@pytest.mark.parametrize("kind,param", [(4, 5), (7, 0)])
+def test_fig9_workloads_4000(gpu, oracle, kind, param):
+    """The reference's own Fig 9 inputs (benchmark/figure_9.cu:40-57) at n = 4,000:
+    RANDOM with groups of 5 (about n^2 / 2 proposals under heavy contention) and
+    SOLO (one displacement chain of n^2 - n + 1 proposals).  SOLO's chain walks
+    every list to its end, so the host tail takes the whole PR matrix."""
+    smp = gpu
+    n = 4000
+    pm, pw = smp.generate(kind, n, seed=0x5EED0009, param=param)
+    want, t = oracle.gs_seq(n, pm, pw)
+    got, prof = solve(smp, n, pm, pw)
+    assert np.array_equal(got, want)
+    assert prof["proposals"] == t["proposals"]
+    if kind == 7:
+        assert t["proposals"] == n * n - n + 1
+        assert prof["tail_full_matrix"] == 1 and prof["bailed"] == 0
 
 
 def test_config3_congested_and_shifted_20000(gpu, oracle):
