@@ -146,10 +146,28 @@ enum smp_init_mode {
 
 enum smp_schedule {
   SMP_SCHEDULE_BAMBOO = 0, /* this build's pipeline (default)                       */
-  SMP_SCHEDULE_MW_CAS = 1  /* the reference's GPU baseline MW-Par-GPU-CAS
-                              (src/smp/smp_engine_mw_3_f9.cu:37-97): one thread per
-                              man, 32-bit CAS on the partner rank; no precheck     */
+  SMP_SCHEDULE_MW_CAS = 1, /* the reference's GPU baseline MW-Par-GPU-CAS
+                              (src/smp/smp_engine_mw_3_f9.cu:37-97, and the CAS
+                              retry of src/smp/kernel.cu:5-55): one thread per man,
+                              32-bit CAS on the partner rank, pref_m + rank_w      */
+  SMP_SCHEDULE_LA_CAS = 2, /* the same with the PR matrix's (woman, rank) entries
+                              and a read-and-skip before the CAS
+                              (src/smp/kernel.cu:122-165)                          */
+  SMP_SCHEDULE_MW_MIN = 3, /* pref_m + rank_w, read-and-skip, then 32-bit atomicMin
+                              (src/smp/kernel.cu:468-512)                          */
+  SMP_SCHEDULE_LA_MIN = 4, /* PR entries, atomicMin without the read
+                              (src/smp/kernel.cu:515-552)                          */
+  SMP_SCHEDULE_FRONTIER = 5 /* rounds over a compacted frontier of free men: each
+                              round every free man skips the women whose partner
+                              word already beats him and proposes to the next one;
+                              proposals to one woman are reduced in LDS before one
+                              global 64-bit atomicMin per workgroup; the free men
+                              of the next round are compacted by wave ballots and
+                              prefix sums.  For lists that share their top women  */
 };
+/* The MW/LA schedules skip the precheck and the master-list path, as the
+ * reference's engines do; all schedules return the same matching. */
+#define SMP_SCHEDULE_COUNT 6
 
 typedef struct smp_times {
   /* Wall clock of the timed phases.  precheck/init/core are the three phases
@@ -247,19 +265,26 @@ int smp_result(smp_engine* e, int32_t* match_m2w);
 int smp_result_device(smp_engine* e, const int32_t** d_match);
 int smp_profile(const smp_engine* e, smp_times* t);
 int smp_init_exec(smp_engine* e, int32_t init_pr, smp_init_times* t);
-/* Measure the hand-off threshold and PR prefix width on this instance: every
- * candidate (threshold in {0 (GPU finishes), 1, 2, 4, 8, 16, 24, 32, 48, 64, 96,
- * 128, 256, 1024}, prefix in
- * {64, 96, 128, 256}) is solved `trials` times (0 = 5) and ranked by median wall
- * time; the four best and the untuned defaults are solved 3 * trials times more,
- * and the lowest of those medians is kept and used by later smp_exec calls in
+/* Measure the configuration on this instance.  First the default solve: when
+ * it takes more than 5 ms (lists that share their top women) and the caller
+ * left schedule and waves_per_cu at their defaults, the schedules (this build's
+ * pipeline at 16, 8 and 4 waves per CU, the frontier, and the reference's four
+ * thread-per-man walks) are each solved `trials` times and the fastest is kept
+ * in the engine's options; nothing more is swept then.  Otherwise the hand-off
+ * threshold and PR prefix width: every candidate (threshold in {0 (GPU
+ * finishes), 1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 256, 1024}, prefix in {64,
+ * 96, 128, 256}) is solved `trials` times (0 = 5) and ranked by median wall time;
+ * the four best and the untuned defaults are solved 3 * trials times more, and
+ * the lowest of those medians is kept and used by later smp_exec calls in
  * SMP_TAIL_AUTO mode. */
 typedef struct smp_tune_result {
   int32_t prefix_cols, tail_threshold;
-  double best_ms;      /* median wall ms of the chosen pair              */
-  double gpu_only_ms;  /* best median with the GPU finishing every chain */
-  int32_t candidates;
-  int32_t reserved[7];
+  double best_ms;      /* median wall ms of the chosen configuration      */
+  double gpu_only_ms;  /* best median with the GPU finishing every chain  */
+  int32_t candidates;  /* threshold / prefix pairs measured               */
+  int32_t schedule;    /* enum smp_schedule chosen                        */
+  int32_t waves_per_cu; /* proposal-kernel occupancy chosen (0 = default) */
+  int32_t reserved[5];
 } smp_tune_result;
 int smp_tune(smp_engine* e, int32_t trials, smp_tune_result* out);
 /* Copies of the engine's device matrices after smp_exec or smp_init_exec (the

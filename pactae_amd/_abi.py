@@ -115,7 +115,9 @@ class SmpTuneResult(ctypes.Structure):
         ("best_ms", ctypes.c_double),
         ("gpu_only_ms", ctypes.c_double),
         ("candidates", i32),
-        ("reserved", i32 * 7),
+        ("schedule", i32),
+        ("waves_per_cu", i32),
+        ("reserved", i32 * 5),
     ]
 
     def as_dict(self) -> dict:
@@ -207,6 +209,10 @@ class SmpServerStats(ctypes.Structure):
 
 SMP_SCHEDULE_BAMBOO = 0
 SMP_SCHEDULE_MW_CAS = 1
+SMP_SCHEDULE_LA_CAS = 2
+SMP_SCHEDULE_MW_MIN = 3
+SMP_SCHEDULE_LA_MIN = 4
+SMP_SCHEDULE_FRONTIER = 5
 SMP_INIT_AUTO = 0
 SMP_INIT_FULL = 1
 SMP_INIT_SPARSE = 2

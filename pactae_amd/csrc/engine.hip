@@ -54,7 +54,7 @@ static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
   else
     smp_default_opts(&e->opts);
   if (e->opts.init_mode < SMP_INIT_AUTO || e->opts.init_mode > SMP_INIT_SPARSE ||
-      e->opts.schedule < SMP_SCHEDULE_BAMBOO || e->opts.schedule > SMP_SCHEDULE_MW_CAS) {
+      e->opts.schedule < SMP_SCHEDULE_BAMBOO || e->opts.schedule >= SMP_SCHEDULE_COUNT) {
     pactae::set_error("opts: unknown init_mode or schedule");
     return SMP_ERR_INVALID;
   }
@@ -109,10 +109,12 @@ int smp_create_device(const int32_t* d_pref_m, const int32_t* d_pref_w, int32_t 
 }
 
 // Ranks given (the women's inverse rank matrix derived by the caller): no
-// women's lists, so no MW-CAS schedule (it maps ranks back to men through them).
+// women's lists, so only this build's pipeline: the reference's walks map ranks
+// back to men through the women's lists, and they and the frontier schedule
+// build the rank matrix from them.
 static int check_ranked(const smp_opts* opts) {
-  if (opts && opts->schedule == SMP_SCHEDULE_MW_CAS) {
-    pactae::set_error("opts: the MW-CAS schedule needs the women's lists, not their ranks");
+  if (opts && opts->schedule != SMP_SCHEDULE_BAMBOO) {
+    pactae::set_error("opts: the MW/LA and frontier schedules need the women's lists, not their ranks");
     return SMP_ERR_INVALID;
   }
   return SMP_OK;

@@ -134,7 +134,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int32_t* d_match = nullptr;
   int* d_seen = nullptr;     // precheck counts; the master-list path's Linv afterwards
   bool master = false;       // this run takes the master-list path
-  int* d_cas_next = nullptr; // MW-CAS schedule: list position after each man's last claim
+  int* d_cas_next = nullptr; // thread-per-man schedules: list position after each man's last claim
+  uint32_t* d_front = nullptr;     // frontier schedule: two lists of free men (n each)
+  pactae::FrontierCtl* d_fctl = nullptr;
+  bool thread_walk() const { return opts.schedule >= SMP_SCHEDULE_MW_CAS && opts.schedule <= SMP_SCHEDULE_LA_MIN; }
   bool matrices_valid = false;  // d_rank_w / d_pr hold the last run's matrices
   Scalars* d_sc = nullptr;
   int2* d_parked = nullptr;
@@ -257,6 +260,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipFree(d_match);
     (void)hipFree(d_seen);
     (void)hipFree(d_cas_next);
+    (void)hipFree(d_front);
+    (void)hipFree(d_fctl);
+    d_front = nullptr;
+    d_fctl = nullptr;
     (void)hipFree(d_sc);
     (void)hipFree(d_parked);
     (void)hipFree(d_bucket);
@@ -325,6 +332,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
         hipMalloc(&d_match, (size_t)n * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&d_seen, (size_t)n * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_cas_next, (size_t)n * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_front, 2 * (size_t)n * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_fctl, sizeof(pactae::FrontierCtl)) != hipSuccess ||
         hipMalloc(&d_sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&d_parked, (size_t)n * sizeof(int2)) != hipSuccess ||
         hipMalloc(&d_wstats, (size_t)dev.cus * 32 * 2 * sizeof(unsigned long long)) != hipSuccess) {
@@ -508,6 +517,25 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // are identical for every pair (the matching is unique); only the time differs.
   int tune(int trials, smp_tune_result* out) {
     if (trials <= 0) trials = 5;
+    // Slow instances (lists that share their top women: contention) first pick
+    // the schedule and the proposal kernel's occupancy; the threshold / prefix
+    // sweep below only runs where one solve is cheap.
+    int best_schedule = opts.schedule, best_waves = opts.waves_per_cu;
+    double sched_ms = 0;
+    if (int st = tune_schedule(trials, &best_schedule, &best_waves, &sched_ms)) return st;
+    if (sched_ms > kSweepLimitMs) {
+      if (out) {
+        *out = smp_tune_result{};
+        out->prefix_cols = K;
+        out->tail_threshold = T;
+        out->best_ms = sched_ms;
+        out->gpu_only_ms = 0;
+        out->candidates = 0;
+        out->schedule = best_schedule;
+        out->waves_per_cu = best_waves;
+      }
+      return SMP_OK;
+    }
     const int Ks[] = {64, 96, 128, 256};
     const int Ts[] = {0, 1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 256, 1024};
     const smp_opts saved = opts;
@@ -575,7 +603,70 @@ struct smp_engine : public pactae::AbsSmpEngine {
       out->best_ms = best.med;
       out->gpu_only_ms = gpu_only;
       out->candidates = (int)cands.size();
+      out->schedule = opts.schedule;
+      out->waves_per_cu = opts.waves_per_cu;
     }
+    return SMP_OK;
+  }
+
+  // Instances slower than this per solve skip the threshold / prefix sweep.
+  static constexpr double kSweepLimitMs = 5.0;
+  // The schedule and occupancy, measured on this instance when its default
+  // solve is slow (kSweepLimitMs): this build's pipeline at 16, 8 and 4 waves per
+  // CU, the frontier rounds and the reference's four thread-per-man walks, each
+  // solved `trials` times (after one untimed solve); the lowest median is kept
+  // in the engine's options for later solves.  Only where the caller left the
+  // schedule and occupancy at their defaults, and not with ranks given (the
+  // other schedules need the women's lists).
+  int tune_schedule(int trials, int* best_schedule, int* best_waves, double* best_ms) {
+    const smp_opts saved = opts;
+    auto median_of = [&](int runs, double* med) -> int {
+      std::vector<double> w(runs);
+      for (int i = 0; i < runs; ++i) {
+        auto t0 = Clock::now();
+        if (int st = exec()) return st;
+        w[i] = ms_since(t0);
+      }
+      std::sort(w.begin(), w.end());
+      *med = w[runs / 2];
+      return SMP_OK;
+    };
+    configure(n);
+    double base = 0;
+    if (int st = median_of(std::max(1, trials / 2), &base)) return st;
+    *best_ms = base;
+    *best_schedule = opts.schedule;
+    *best_waves = opts.waves_per_cu;
+    if (base <= kSweepLimitMs || d_rank_in || saved.schedule != SMP_SCHEDULE_BAMBOO || saved.waves_per_cu != 0)
+      return SMP_OK;
+    struct C {
+      int schedule, waves;
+    };
+    const C cands[] = {{SMP_SCHEDULE_BAMBOO, 8},  {SMP_SCHEDULE_BAMBOO, 4}, {SMP_SCHEDULE_FRONTIER, 0},
+                       {SMP_SCHEDULE_MW_CAS, 0},  {SMP_SCHEDULE_LA_CAS, 0}, {SMP_SCHEDULE_MW_MIN, 0},
+                       {SMP_SCHEDULE_LA_MIN, 0}};
+    for (const C& c : cands) {
+      opts.schedule = c.schedule;
+      opts.waves_per_cu = c.waves;
+      configure(n);
+      double med = 0;
+      int st = exec();  // untimed: first run of the schedule
+      if (st == SMP_OK) st = median_of(trials, &med);
+      if (st != SMP_OK) {
+        opts = saved;
+        configure(n);
+        return st;
+      }
+      if (med < *best_ms) {
+        *best_ms = med;
+        *best_schedule = c.schedule;
+        *best_waves = c.waves;
+      }
+    }
+    opts = saved;
+    opts.schedule = *best_schedule;
+    opts.waves_per_cu = *best_waves;
+    configure(n);
     return SMP_OK;
   }
 
@@ -645,12 +736,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
     // sparse pass's bin counters, and marks every woman unclaimed (the
     // proposal kernel's start state; the master-list path refills the words
     // for its men).
-    const bool run_precheck = opts.precheck && opts.schedule != SMP_SCHEDULE_MW_CAS;
+    const bool run_precheck = opts.precheck && !thread_walk();
     const bool may_sparse = opts.schedule == SMP_SCHEDULE_BAMBOO && use_sparse();
     SMP_HIP_TRY(pactae::launch_reset_scalars(
         d_sc, std::min(n, propose_grid() * 4), stream, run_precheck ? d_seen : nullptr, (size_t)n,
-        may_sparse ? d_bin_cnt : nullptr, counter_words(),
-        opts.schedule == SMP_SCHEDULE_BAMBOO ? d_partner : nullptr, (size_t)n));
+        may_sparse ? d_bin_cnt : nullptr, counter_words(), thread_walk() ? nullptr : d_partner, (size_t)n));
     if (!run_precheck) {
       *perfect = false;
       return SMP_OK;
@@ -694,7 +784,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
         // void; back to the state the reset left, and on to the general path.
         SMP_HIP_TRY(pactae::launch_reset_scalars(
             d_sc, std::min(n, propose_grid() * 4), stream, nullptr, 0, may_sparse ? d_bin_cnt : nullptr,
-            counter_words(), opts.schedule == SMP_SCHEDULE_BAMBOO ? d_partner : nullptr, (size_t)n));
+            counter_words(), thread_walk() ? nullptr : d_partner, (size_t)n));
       }
     }
     times.master_list = master ? 1 : 0;
@@ -703,16 +793,26 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   // -- init: rank matrix + PR prefix gather (smp_engine_bamboo.cu:33-106) -----
   int InitProc() override {
-    if (opts.schedule == SMP_SCHEDULE_MW_CAS) {
-      // The baseline needs the women's rank matrix only.
+    if (thread_walk() || (opts.schedule == SMP_SCHEDULE_FRONTIER && !master)) {
+      // The reference's walks and the frontier: the women's rank matrix, and for
+      // the PR-entry (LA) walks and the frontier the PR prefix gathered from it.
       blocks_copied = 0;
+      matrices_valid = true;
       SMP_HIP_TRY(hipEventRecord(ev[0], stream));
       SMP_HIP_TRY(pactae::launch_invert_rows(d_pref_w, d_rank_w, n, invert_chunk(), opts.validate, d_sc, stream,
                                              /*nontemporal=*/1));
       rank_built = true;
       times.rank_matrix = 1;
+      if (opts.schedule == SMP_SCHEDULE_LA_CAS || opts.schedule == SMP_SCHEDULE_LA_MIN ||
+          opts.schedule == SMP_SCHEDULE_FRONTIER) {
+        SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+        SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
+        span_rank = {&ev[0], &ev[1]};
+        span_gather = {&ev[1], &ev[2]};
+      } else {
+        span_rank = {&ev[0], &ev[2]};
+      }
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
-      span_rank = {&ev[0], &ev[2]};
       return SMP_OK;
     }
     if (master) {
@@ -816,12 +916,40 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int CoreProc() override {
     host_finished = false;
     const int grid = propose_grid();
-    if (opts.schedule == SMP_SCHEDULE_MW_CAS) {
-      SMP_HIP_TRY(pactae::launch_mw_cas(d_pref_m, d_pref_w, d_rank_w, d_seen, d_cas_next, n, d_sc, stream));
+    if (thread_walk()) {
+      SMP_HIP_TRY(pactae::launch_walk_thread(opts.schedule, d_pref_m, d_pref_w, d_rank_w, d_pr, K, d_seen, d_cas_next,
+                                             n, d_sc, stream));
       SMP_HIP_TRY(hipEventRecord(ev[4], stream));
       span_propose = {&ev[2], &ev[4]};
       if (int st = read_scalars()) return st;
-      if (int st = check_device_error("MW-CAS walk")) return st;
+      if (int st = check_device_error("thread-per-man walk")) return st;
+      times.handoff_free = 0;
+      return SMP_OK;
+    }
+    if (opts.schedule == SMP_SCHEDULE_FRONTIER && !master) {
+      pactae::FrontierArgs fa;
+      fa.pr = d_pr;
+      fa.pref_m = d_pref_m;
+      fa.rank_w = d_rank_w;
+      fa.partner = d_partner;
+      fa.f0 = d_front;
+      fa.f1 = d_front + n_alloc;
+      fa.ctl = d_fctl;
+      fa.sc = d_sc;
+      fa.n = n;
+      fa.K = K;
+      SMP_HIP_TRY(pactae::launch_frontier(fa, dev.cus, stream));
+      SMP_HIP_TRY(hipEventRecord(ev[4], stream));
+      span_propose = {&ev[2], &ev[4]};
+      pactae::FrontierCtl fc{};
+      SMP_HIP_TRY(hipMemcpyAsync(&fc, d_fctl, sizeof(fc), hipMemcpyDeviceToHost, stream));
+      if (int st = read_scalars()) return st;
+      if (fc.abort) {
+        pactae::set_error("frontier schedule: a grid barrier wait ran out");
+        return SMP_ERR_TIMEOUT;
+      }
+      if (int st = check_device_error("frontier walk")) return st;
+      times.steps = fc.rounds;
       times.handoff_free = 0;
       return SMP_OK;
     }
@@ -1237,7 +1365,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     if (perfect) {
       SMP_HIP_TRY(pactae::launch_first_match(d_pref_m, n, d_match, stream));
-    } else if (opts.schedule == SMP_SCHEDULE_MW_CAS) {
+    } else if (thread_walk()) {
       SMP_HIP_TRY(pactae::launch_post_cas(d_pref_w, d_seen, d_cas_next, n, d_match, d_sc, stream));
     } else if (master) {
       SMP_HIP_TRY(pactae::launch_post_master(d_partner, d_seen, n, d_match, d_sc, stream));
@@ -1254,7 +1382,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     } else {
       times.proposals = (int64_t)h_sc->proposals;
       times.atomics = (int64_t)h_sc->atomics;
-      times.steps = (int64_t)h_sc->steps;
+      if (opts.schedule != SMP_SCHEDULE_FRONTIER || master) times.steps = (int64_t)h_sc->steps;  // frontier: rounds
     }
     return SMP_OK;
   }

@@ -1207,17 +1207,26 @@ __global__ __launch_bounds__(256) void k_post_master(const unsigned long long* _
 }
 
 // ---------------------------------------------------------------------------
-// MW-Par-GPU-CAS, the reference's GPU baseline (SmpEngineMw3F9::CoreProc,
-// src/smp/smp_engine_mw_3_f9.cu:37-82): one thread per man walks his chain; a
-// woman's partner is the 32-bit rank of her current man (n = free), claimed by
-// atomicCAS; the displaced man is pref_w[w][old rank] and resumes at the list
-// position recorded in next[] when he was accepted.  next[m] is written before
-// the claim, so a displacer that reads it after its own claim sees that value or
-// an earlier one (an earlier one only repeats refused proposals).
-__global__ __launch_bounds__(256) void k_mw_cas(const int32_t* __restrict__ pref_m,
-                                                const int32_t* __restrict__ pref_w,
-                                                const uint16_t* __restrict__ rank_w, int* partner_rank,
-                                                int* next, int n, Scalars* sc) {
+// The reference's thread-per-man GPU walks as schedules (SmpEngineMw3F9
+// src/smp/smp_engine_mw_3_f9.cu:37-82 and the variants of src/smp/kernel.cu:
+// 5-55, 122-165, 468-512, 515-552): one thread per man walks his chain; a
+// woman's partner is the 32-bit rank of her current man (n = free); the
+// displaced man is pref_w[w][old rank] and resumes at the list position recorded
+// in next[] when he was accepted.  The variants differ in
+//   LA    entries (woman, rank) from the PR prefix for positions < K (the
+//         reference's PR matrix, types.h:58-61), else pref_m + rank_w;
+//   MIN   claim with atomicMin (the woman keeps the smaller rank) instead of a
+//         compare-and-swap retry loop;
+//   SKIP  read the partner rank first and skip women who hold a better man.
+// next[m] is written before the claim and fenced, so a displacer that reads it
+// after its own claim sees that value or an earlier one (an earlier one only
+// repeats refused proposals); the reference writes it after the claim.
+template <bool LA, bool MIN, bool SKIP>
+__global__ __launch_bounds__(256) void k_walk_thread(const int32_t* __restrict__ pref_m,
+                                                     const int32_t* __restrict__ pref_w,
+                                                     const uint16_t* __restrict__ rank_w,
+                                                     const uint32_t* __restrict__ pr, int K, int* partner_rank,
+                                                     int* next, int n, Scalars* sc) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long tries = 0;
   int bad = 0;
@@ -1228,30 +1237,56 @@ __global__ __launch_bounds__(256) void k_mw_cas(const int32_t* __restrict__ pref
         bad |= kErrListEnd;
         break;
       }
-      const int w = pref_m[(size_t)m * n + k];
-      if ((unsigned)w >= (unsigned)n) {
+      int w;
+      int r;
+      if (LA && k < K) {
+        const uint32_t e = pr[pr_index(m, k, n)];
+        w = (int)pr_woman(e);
+        r = (int)pr_rank(e);
+      } else {
+        w = pref_m[(size_t)m * n + k];
+        if ((unsigned)w >= (unsigned)n) {
+          bad |= kErrEntryRange;
+          break;
+        }
+        r = rank_w[(size_t)w * n + m];
+      }
+      if ((unsigned)w >= (unsigned)n || (unsigned)r >= (unsigned)n) {
         bad |= kErrEntryRange;
         break;
       }
-      const int r = rank_w[(size_t)w * n + m];
       ++k;
-      int cur = __hip_atomic_load(&partner_rank[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int cur = n + 1;  // only tested when SKIP or in the CAS loop
+      if (SKIP || !MIN) {
+        cur = __hip_atomic_load(&partner_rank[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur < r) continue;  // she holds a better man
+      }
+      int prev;
       bool won = false;
-      while (cur > r) {
+      if (MIN) {
         __hip_atomic_store(&next[m], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         ++tries;
-        const int prev = atomicCAS(&partner_rank[w], cur, r);
-        if (prev == cur) {
-          won = true;
-          break;
+        prev = atomicMin(&partner_rank[w], r);
+        won = prev > r;
+      } else {
+        prev = cur;
+        while (prev > r) {
+          __hip_atomic_store(&next[m], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __threadfence();
+          ++tries;
+          const int seen = atomicCAS(&partner_rank[w], prev, r);
+          if (seen == prev) {
+            won = true;
+            break;
+          }
+          prev = seen;
         }
-        cur = prev;
       }
       if (!won) continue;
-      if (cur >= n) break;  // she was free
+      if (prev >= n) break;  // she was free
       __threadfence();
-      m = pref_w[(size_t)w * n + cur];
+      m = pref_w[(size_t)w * n + prev];
       if ((unsigned)m >= (unsigned)n) {
         bad |= kErrEntryRange;
         break;
@@ -1264,6 +1299,236 @@ __global__ __launch_bounds__(256) void k_mw_cas(const int32_t* __restrict__ pref
   if ((threadIdx.x & 63) == 0) {
     if (tries) atomicAdd(&sc->atomics, tries);
     if (bb) atomicOr(&sc->error, kErrEntryRange | kErrListEnd);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Frontier schedule: rounds over a compacted list of free men (entries
+// k << 16 | m in f0 / f1, the sizes in ctl->cnt rotating by round % 3).  In a
+// round every free man is taken by a group of G lanes, which scans up to
+// kFrontierScan * G positions of his list from his position for the first woman
+// whose partner word he beats (a plain read: words only decrease, so a woman
+// whose word is already smaller would refuse him, and skipping her is her
+// refusal).  The workgroup's candidates for one woman are reduced in an LDS hash
+// table (64-bit atomicMin on the packed (rank, man, position) word) and only the
+// smallest goes to the global word, with one atomicMin.  A man who loses in LDS
+// or to the global word, or who found no candidate in his window, goes to the
+// next round's list; a man displaced by a successful claim goes there at his
+// stored position + 1; a claim of a free woman ends the chain.  The next list is
+// compacted by a ballot and prefix sum per wave and one atomicAdd per wave.  A
+// grid barrier (all workgroups co-resident: a cooperative launch) separates the
+// rounds; its waits are bounded, and a wait that runs out sets ctl->abort, which
+// ends every workgroup at its next barrier or round.
+constexpr int kFrontierScan = 4;  // loads per lane of a round's window (G * 4 positions)
+constexpr long long kBarrierSpinLimit = 1ll << 27;
+
+__device__ bool frontier_barrier(FrontierCtl* c, int nblocks) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();
+    const int gen = __hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (atomicAdd(&c->bar_count, 1) == nblocks - 1) {
+      __hip_atomic_store(&c->bar_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&c->bar_gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long long spins = 0;
+      while (__hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kBarrierSpinLimit ||
+            __hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          atomicOr(&c->abort, 1);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+__global__ void k_frontier_init(uint32_t* f0, int n, FrontierCtl* c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f0[i] = (uint32_t)i;  // man i at position 0
+  if (i == 0) {
+    c->cnt[0] = n;
+    c->cnt[1] = c->cnt[2] = 0;
+    c->bar_count = c->bar_gen = 0;
+    c->rounds = 0;
+    c->abort = 0;
+  }
+}
+
+template <int BS, int G>
+__global__ __launch_bounds__(BS) void k_frontier(FrontierArgs a) {
+  constexpr int MPW = BS / G;   // men per workgroup per chunk
+  constexpr int TBL = 2 * MPW;  // LDS hash slots (power of two)
+  static_assert((TBL & (TBL - 1)) == 0, "table size");
+  __shared__ uint32_t keys[TBL];
+  __shared__ unsigned long long vals[TBL];
+  const int n = a.n, K = a.K;
+  const int lane = threadIdx.x & 63, sub = lane & (G - 1), gbase = lane & ~(G - 1);
+  const int grp = threadIdx.x / G;
+  const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1) << gbase;
+  unsigned long long n_atomics = 0;
+  int bad = 0;
+  if (__hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;  // uniform
+  for (int round = 0;; ++round) {
+    const int size = __hip_atomic_load(&a.ctl->cnt[round % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (size == 0 || __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.ctl->cnt[(round + 2) % 3] = 0;
+      a.ctl->rounds = round + 1;
+    }
+    const uint32_t* cur = (round & 1) ? a.f1 : a.f0;
+    uint32_t* nxt = (round & 1) ? a.f0 : a.f1;
+    int* nxt_cnt = &a.ctl->cnt[(round + 1) % 3];
+    for (int base = (int)blockIdx.x * MPW; base < size; base += (int)gridDim.x * MPW) {
+      for (int i = threadIdx.x; i < TBL; i += BS) {
+        keys[i] = 0xFFFFFFFFu;
+        vals[i] = kEmpty;
+      }
+      __syncthreads();
+      const int item = base + grp;
+      const bool have = item < size;
+      int m = 0, k = 0;
+      if (have) {
+        const uint32_t e = cur[item];
+        m = (int)(e & 0xFFFFu);
+        k = (int)(e >> 16);
+      }
+      // The window: positions k + sub + G * j, j < kFrontierScan, all loaded at
+      // once (entries, then the women's words); the first candidate in list
+      // order is the lowest j with a candidate lane, then the lowest lane.
+      bool cand = false;
+      uint32_t cw = 0;
+      unsigned long long cmine = 0;
+      int ck = 0;
+      uint32_t wv[kFrontierScan];
+      unsigned long long mv[kFrontierScan];
+#pragma unroll
+      for (int j = 0; j < kFrontierScan; ++j) {
+        const int pos = k + sub + G * j;
+        wv[j] = 0xFFFFFFFFu;
+        mv[j] = 0;
+        if (have && pos < n) {
+          uint32_t w = 0, r = 0;
+          if (pos < K) {
+            const uint32_t e = a.pr[pr_index(m, pos, n)];
+            w = pr_woman(e);
+            r = pr_rank(e);
+          } else {
+            const int wi = a.pref_m[(size_t)m * n + pos];
+            if ((unsigned)wi < (unsigned)n) {
+              w = (uint32_t)wi;
+              r = a.rank_w[(size_t)w * n + m];
+            } else {
+              w = (uint32_t)n;
+            }
+          }
+          if (w < (uint32_t)n && r < (uint32_t)n) {
+            wv[j] = w;
+            mv[j] = pw_pack(r, (uint32_t)m, (uint32_t)pos);
+          } else {
+            bad |= kErrEntryRange;
+          }
+        }
+      }
+      unsigned long long hv[kFrontierScan];
+#pragma unroll
+      for (int j = 0; j < kFrontierScan; ++j) hv[j] = wv[j] != 0xFFFFFFFFu ? a.partner[wv[j]] : 0ull;
+      int jf = kFrontierScan, lf = gbase;
+#pragma unroll
+      for (int j = kFrontierScan - 1; j >= 0; --j) {
+        const unsigned long long bm = __ballot(wv[j] != 0xFFFFFFFFu && mv[j] < hv[j]) & gmask;
+        if (bm) {
+          jf = j;
+          lf = __ffsll((long long)bm) - 1;
+        }
+      }
+      // Every lane takes part in the shuffles; jf and lf are the same in a group.
+      uint32_t wsel = wv[0];
+      unsigned long long msel = mv[0];
+#pragma unroll
+      for (int j = 1; j < kFrontierScan; ++j)
+        if (jf == j) wsel = wv[j], msel = mv[j];
+      const uint32_t wj = (uint32_t)__shfl((int)wsel, lf);
+      const unsigned long long mj = __shfl(msel, lf);
+      bool list_end = false;
+      if (have) {
+        if (jf < kFrontierScan) {
+          cand = true;
+          cw = wj;
+          cmine = mj;
+          ck = k + G * jf + (lf - gbase);
+        } else {
+          k += G * kFrontierScan;
+          if (k >= n) list_end = true;
+        }
+      }
+      if (list_end && !cand) bad |= kErrListEnd;
+      // LDS pre-reduction: the group's leader inserts its candidate.
+      int slot = -1;
+      if (cand && sub == 0) {
+        int h = (int)((cw * 2654435761u) >> 16) & (TBL - 1);
+        for (int probe = 0; probe < TBL; ++probe) {
+          const uint32_t prev = atomicCAS(&keys[h], 0xFFFFFFFFu, cw);
+          if (prev == 0xFFFFFFFFu || prev == cw) {
+            slot = h;
+            break;
+          }
+          h = (h + 1) & (TBL - 1);
+        }
+        if (slot >= 0) atomicMin(&vals[slot], cmine);
+      }
+      __syncthreads();
+      bool push = false;
+      uint32_t out = 0;
+      if (sub == 0 && have) {
+        if (cand) {
+          if (slot >= 0 && vals[slot] == cmine) {
+            const unsigned long long old = atomicMin(&a.partner[cw], cmine);
+            ++n_atomics;
+            if (old > cmine) {
+              if (old != kEmpty) {
+                push = true;
+                out = ((pw_pos(old) + 1) << 16) | pw_man(old);
+              }
+            } else {
+              push = true;
+              out = ((uint32_t)(ck + 1) << 16) | (uint32_t)m;
+            }
+          } else {
+            push = true;
+            out = ((uint32_t)(ck + 1) << 16) | (uint32_t)m;
+          }
+        } else if (!list_end) {
+          push = true;
+          out = ((uint32_t)k << 16) | (uint32_t)m;
+        }
+      }
+      // Compaction: one reservation per wave, each pushing lane at its prefix.
+      const unsigned long long pm_ = __ballot(push);
+      int wbase = 0;
+      if (lane == 0 && pm_) wbase = atomicAdd(nxt_cnt, __popcll(pm_));
+      wbase = __shfl(wbase, 0);
+      if (push) {
+        const int at = wbase + __popcll(pm_ & ((1ull << lane) - 1));
+        if (at < n) nxt[at] = out;
+        else bad |= kErrListEnd;
+      }
+      __syncthreads();  // the LDS table is cleared for the next chunk
+    }
+    if (!frontier_barrier(a.ctl, (int)gridDim.x)) break;
+  }
+  for (int off = 32; off > 0; off >>= 1) n_atomics += __shfl_down(n_atomics, off);
+  const unsigned long long bb = __ballot(bad != 0);
+  if (lane == 0) {
+    if (n_atomics) atomicAdd(&a.sc->atomics, n_atomics);
+    if (bb) atomicOr(&a.sc->error, bad);
   }
 }
 
@@ -1369,13 +1634,33 @@ hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t
   return hipGetLastError();
 }
 
-hipError_t launch_mw_cas(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank_w, int* partner_rank,
-                         int* next, int n, Scalars* sc, hipStream_t s) {
+hipError_t launch_walk_thread(int schedule, const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank_w,
+                              const uint32_t* pr, int K, int* partner_rank, int* next, int n, Scalars* sc,
+                              hipStream_t s) {
   hipError_t e = launch_fill_i32(partner_rank, n, n, s);
   if (e != hipSuccess) return e;
   if ((e = launch_fill_i32(next, n, 0, s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mw_cas, dim3((n + 255) / 256), dim3(256), 0, s, pref_m, pref_w, rank_w, partner_rank, next,
-                     n, sc);
+  const dim3 g((n + 255) / 256), b(256);
+  switch (schedule) {
+    case SMP_SCHEDULE_MW_CAS:
+      hipLaunchKernelGGL((k_walk_thread<false, false, true>), g, b, 0, s, pref_m, pref_w, rank_w, pr, K, partner_rank,
+                         next, n, sc);
+      break;
+    case SMP_SCHEDULE_LA_CAS:
+      hipLaunchKernelGGL((k_walk_thread<true, false, true>), g, b, 0, s, pref_m, pref_w, rank_w, pr, K, partner_rank,
+                         next, n, sc);
+      break;
+    case SMP_SCHEDULE_MW_MIN:
+      hipLaunchKernelGGL((k_walk_thread<false, true, true>), g, b, 0, s, pref_m, pref_w, rank_w, pr, K, partner_rank,
+                         next, n, sc);
+      break;
+    case SMP_SCHEDULE_LA_MIN:
+      hipLaunchKernelGGL((k_walk_thread<true, true, false>), g, b, 0, s, pref_m, pref_w, rank_w, pr, K, partner_rank,
+                         next, n, sc);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -1528,6 +1813,25 @@ hipError_t launch_window(const int32_t* pref_m, const int32_t* pref_w, const uin
                        seq);
   }
   return hipGetLastError();
+}
+
+// The frontier schedule: initial list (every man at position 0), then one
+// cooperative launch of k_frontier with as many workgroups as fit on the device
+// at once (at most one per MPW men).
+hipError_t launch_frontier(const FrontierArgs& a, int cus, hipStream_t s) {
+  constexpr int BS = 1024, G = 8;
+  hipLaunchKernelGGL(k_frontier_init, dim3((a.n + 255) / 256), dim3(256), 0, s, a.f0, a.n, a.ctl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  int per_cu = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_frontier<BS, G>), BS, 0);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) return hipErrorInvalidConfiguration;
+  const int grid = std::max(1, std::min(cus * std::min(per_cu, 2), (a.n + BS / G - 1) / (BS / G)));
+  FrontierArgs args = a;
+  void* params[] = {&args};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_frontier<BS, G>), dim3(grid), dim3(BS), params,
+                                    0, s);
 }
 
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s) {

@@ -27,6 +27,30 @@ struct ProposeArgs {
                                //   (sc->prefix_overflow), the kernel does nothing.
 };
 
+// Control block of the frontier schedule (device memory).
+struct FrontierCtl {
+  int cnt[3];     // list sizes, rotating by round % 3
+  int bar_count;  // grid barrier: arrivals
+  int bar_gen;    // grid barrier: generation
+  int rounds;     // rounds run
+  int abort;      // a bounded barrier wait ran out
+  int pad;
+};
+
+struct FrontierArgs {
+  const uint32_t* pr;       // PR prefix (positions < K)
+  const int32_t* pref_m;    // positions >= K: pref_m and the rank matrix
+  const uint16_t* rank_w;
+  unsigned long long* partner;
+  uint32_t* f0;             // free-man lists, n entries each: k << 16 | m
+  uint32_t* f1;
+  FrontierCtl* ctl;
+  Scalars* sc;
+  int n, K;
+};
+
+hipError_t launch_frontier(const FrontierArgs& a, int cus, hipStream_t s);
+
 hipError_t launch_copy16(void* dst, const void* src, size_t bytes, int cus, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t count, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int* p, size_t count, int v, hipStream_t s);
@@ -45,8 +69,12 @@ hipError_t launch_master_fused(const int32_t* pref_m, const int32_t* pref_w, con
 hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars* sc, hipStream_t s);
 hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
                               Scalars* sc, hipStream_t s);
-hipError_t launch_mw_cas(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank_w, int* partner_rank,
-                         int* next, int n, Scalars* sc, hipStream_t s);
+// The reference's thread-per-man walks (schedule SMP_SCHEDULE_MW_CAS, _LA_CAS,
+// _MW_MIN or _LA_MIN) over 32-bit partner ranks and next[]; pr/K: the PR prefix
+// the LA variants read (positions >= K from pref_m and rank_w).
+hipError_t launch_walk_thread(int schedule, const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank_w,
+                              const uint32_t* pr, int K, int* partner_rank, int* next, int n, Scalars* sc,
+                              hipStream_t s);
 hipError_t launch_post_cas(const int32_t* pref_w, const int* partner_rank, const int* next, int n, int32_t* match,
                            Scalars* sc, hipStream_t s);
 hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipStream_t s);

@@ -191,10 +191,16 @@ def default_opts(**overrides) -> _abi.SmpOpts:
 
 
 class Schedule(enum.IntEnum):
-    """enum smp_schedule: this build's pipeline, or the reference's GPU baseline
-    MW-Par-GPU-CAS (SmpEngineMw3F9, src/smp/smp_engine_mw_3_f9.cu:37-97)."""
+    """enum smp_schedule: this build's pipeline; the reference's thread-per-man
+    GPU walks (MW-Par-GPU-CAS, SmpEngineMw3F9, src/smp/smp_engine_mw_3_f9.cu:37-97,
+    and the CAS/MIN variants of src/smp/kernel.cu:5-55, 122-165, 468-512,
+    515-552); or the frontier rounds with LDS pre-reduction per woman."""
     BAMBOO = _abi.SMP_SCHEDULE_BAMBOO
     MW_CAS = _abi.SMP_SCHEDULE_MW_CAS
+    LA_CAS = _abi.SMP_SCHEDULE_LA_CAS
+    MW_MIN = _abi.SMP_SCHEDULE_MW_MIN
+    LA_MIN = _abi.SMP_SCHEDULE_LA_MIN
+    FRONTIER = _abi.SMP_SCHEDULE_FRONTIER
 
 
 class InitMode(enum.IntEnum):

@@ -198,6 +198,63 @@ def test_mw_cas_baseline_schedule(gpu, oracle, kind):
         assert prof["proposals"] == t["proposals"], (kind, n)
 
 
+@pytest.mark.parametrize("schedule", [2, 3, 4, 5])
+@pytest.mark.parametrize("kind", range(8))
+def test_alternative_schedules(gpu, oracle, kind, schedule):
+    """The reference's PR-entry CAS walk (kernel.cu:122-165), its atomicMin walks
+    with and without the read-and-skip (kernel.cu:468-512, 515-552) and the
+    frontier rounds with LDS pre-reduction: bit-exact, same proposal count, on
+    every generator, with a 64-column prefix (the LA walks and the frontier read
+    past it through the rank matrix)."""
+    smp = gpu
+    for n in (1, 2, 5, 64, 65, 300, 1000):
+        if kind == 7 and n < 4:
+            continue
+        pm, pw = smp.generate(kind, n, seed=2500 + n)
+        want, t = oracle.gs_seq(n, pm, pw)
+        for cfg in (dict(), dict(prefix_cols=64)):
+            got, prof = solve(smp, n, pm, pw, schedule=schedule, **cfg)
+            assert prof["schedule"] == schedule
+            assert np.array_equal(got, want), (kind, n, schedule, cfg)
+            assert prof["proposals"] == t["proposals"], (kind, n, schedule, cfg)
+
+
+def test_tuner_picks_a_schedule_on_contention(gpu, oracle):
+    """All-shared lists with the precheck off at n = 5,000 take more than 5 ms per
+    solve, so the tuner measures the schedules and keeps the fastest; later
+    solves use it and stay bit-exact.  A fast instance keeps the default."""
+    smp = gpu
+    n = 5000
+    pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=41)
+    want, t = oracle.gs_seq(n, pm, pw)
+    obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+    with smp.SmpEngineBamboo(obj, n, precheck=0) as e:
+        r = e.tune(2)
+        assert 0 <= r["schedule"] < 6 and r["candidates"] == 0
+        got = e.find_stable_matching()
+        assert e.profile()["schedule"] == r["schedule"]
+        assert np.array_equal(got, want)
+        assert e.profile()["proposals"] == t["proposals"]
+    pm, pw = smp.generate(smp.Workload.UNIFORM, 2000, seed=42)
+    with smp.SmpEngineBamboo(smp.SmpObj.create_from_flat_lists(pm, pw, 2000), 2000) as e:
+        r = e.tune(2)
+        assert r["schedule"] == 0 and r["candidates"] > 0
+        assert np.array_equal(e.find_stable_matching(), oracle.gs_seq(2000, pm, pw)[0])
+
+
+def test_frontier_schedule_20000(gpu, oracle):
+    """The frontier schedule at n = 20,000 on uniform and correlated lists, and on
+    all-shared lists with the precheck off (no fast path)."""
+    smp = gpu
+    n = 20000
+    for kind, opts in ((0, {}), (3, {}), (1, dict(precheck=0))):
+        pm, pw = smp.generate(kind, n, seed=0x5EED0003)
+        want, t = oracle.gs_seq(n, pm, pw)
+        got, prof = solve(smp, n, pm, pw, schedule=5, **opts)
+        assert np.array_equal(got, want), kind
+        assert prof["proposals"] == t["proposals"], kind
+
+
 def host_ranks(pw, n):
     """The women's inverse rank matrix derived on the host, the ranked engine's
     input: rank[w, pref_w[w, r]] = r."""
