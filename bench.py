@@ -406,6 +406,8 @@ def serving(smp, args):
         extra = {"large_n": args.serve_large_n} if args.serve_large_n > 0 else {}
         if getattr(args, "serve_large_workers", 0) > 0:
             extra["large_workers"] = args.serve_large_workers
+        if getattr(args, "serve_spill_us", 0) != 0:
+            extra["large_spill_us"] = args.serve_spill_us
         with smp.SmpServer(mode=mode, lanes=3, max_batch=64, max_n=8192, **extra) as srv:
             # warm-up request (kernel load, first allocations)
             n0, d0m, d0w, _, _ = pool[0]
@@ -491,6 +493,9 @@ def main() -> int:
                     help="server large_n (requests this large use the single-instance pipeline); 0 = default")
     ap.add_argument("--serve-large-workers", type=int, default=0,
                     help="server large_workers (threads with their own pipeline); 0 = default")
+    ap.add_argument("--serve-spill-us", type=int, default=0,
+                    help="server large_spill_us (a second large-request worker starts on a request only "
+                         "after it waited this long); 0 = at once (the default)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the measured choice of hand-off threshold and prefix width")

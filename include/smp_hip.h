@@ -408,7 +408,12 @@ typedef struct smp_server_opts {
                                0 = 512; above max_n = never                        */
   int32_t large_workers;    /* threads (each with its own pipeline) serving the large
                                requests; 0 = 3                                     */
-  int32_t reserved[6];
+  int32_t large_spill_us;   /* > 0: worker i > 0 takes the oldest waiting large
+                               request only when more than i are waiting or it has
+                               waited this long (so that at low load a request runs
+                               alone on the GPU); 0 = any idle worker at once
+                               (default: measured best for p50 and p99)            */
+  int32_t reserved[5];
 } smp_server_opts;
 
 typedef struct smp_request_info {

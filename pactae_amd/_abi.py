@@ -170,7 +170,8 @@ class SmpServerOpts(ctypes.Structure):
         ("hbm_budget_bytes", i64),
         ("large_n", i32),
         ("large_workers", i32),
-        ("reserved", i32 * 6),
+        ("large_spill_us", i32),
+        ("reserved", i32 * 5),
     ]
 
 

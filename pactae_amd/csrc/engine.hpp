@@ -187,7 +187,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
   uint32_t* h_win_dev = nullptr;
   int4* h_winreq = nullptr;    // (man, first position, slot, 0) per request of a round
   int4* h_winreq_dev = nullptr;
-  std::vector<int> win_slot, win_start, win_men;
+  std::vector<int> win_slot, win_start, win_len, win_men;
+  // A man's first window holds kWinFirst entries and each later one twice the
+  // last, up to kWinE: most chains that pass the prefix end a few dozen
+  // positions past it, and each entry of a window costs a search of a row.
+  static constexpr int kWinFirst = 64;
   // Rounds of windows after which the host takes the whole PR matrix instead
   // (allocated on first use: 4 n^2 bytes in HBM and as much pinned host memory):
   // when the rounds' GPU round trips (~50 us each) have cost about what copying
@@ -382,6 +386,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     win_slot.assign(n, -1);
     win_start.assign(n, 0);
+    win_len.assign(n, 0);
     // Sparse prefix pass buffers, for prefixes up to 256 columns (any width when
     // n <= 2048, where n x n buckets are small).
     K_sparse_alloc = n <= 2048 ? K_alloc : std::min(K_alloc, 256);
@@ -1096,8 +1101,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return true;
   }
 
-  // The whole PR matrix (every position of every man's list with its rank, in
-  // the column-blocked PR layout) gathered on the GPU into d_pr_full and copied
+  // The whole PR matrix (every position of every man's list with its rank, row
+  // after row: a man's list is contiguous) gathered on the GPU into d_pr_full and copied
   // into h_pr_full, for a host tail whose chains walk deep into the lists (the
   // reference copies its whole PR matrix to the host for the same purpose,
   // smp_engine_bamboo.h:41-61).  Leaves h_pr_full null, and returns SMP_OK, if
@@ -1119,13 +1124,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
       }
     }
     if (d_rank_in) {
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_in, d_pr_full, n, n, d_sc, stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_in, d_pr_full, n, n, d_sc, stream, /*row_major=*/1));
     } else {
       if (int rc = ensure_rank_matrix()) return rc;
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr_full, n, n, d_sc, stream));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr_full, n, n, d_sc, stream, /*row_major=*/1));
     }
-    SMP_HIP_TRY(hipMemcpyAsync(h_pr_full, d_pr_full, pactae::pr_entries(n, n) * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, stream));
+    SMP_HIP_TRY(hipMemcpyAsync(h_pr_full, d_pr_full, (size_t)n * n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               stream));
     SMP_HIP_TRY(wait_stream(stream));
     times.tail_full_matrix = 1;
     return SMP_OK;
@@ -1179,7 +1184,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       const unsigned long long v = h_partner[w];
       hst[w] = v == pactae::kEmpty ? kHostFree : hs_pack(pactae::pw_rank(v), pactae::pw_man(v), pactae::pw_pos(v));
     }
-    for (int m : win_men) win_slot[m] = -1;  // windows of an earlier solve
+    for (int m : win_men) win_slot[m] = -1, win_len[m] = 0;  // windows of an earlier solve
     win_men.clear();
     long long props = 0;
     int bailed = 0;
@@ -1212,7 +1217,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
           uint32_t e;
           if (full) {
             if (k >= n) return tail_error("host tail: a man exhausted his list");
-            e = pr[pactae::pr_index(m, k, n)];
+            e = pr[(size_t)m * n + k];
           } else if (k < kcap) {
             if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
               while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(wait_event(ev_blk[blocks_ready++]));
@@ -1221,7 +1226,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
           } else {
             const int sl = win_slot[m];
             const int off = k - win_start[m];
-            if (sl < 0 || off < 0 || off >= kWinE) {
+            if (sl < 0 || off < 0 || off >= win_len[m]) {
               bail[bailed++] = make_int2(m, k);
               ended = true;
               break;
@@ -1242,7 +1247,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
             }
             m = (int)((v >> 16) & 0xFFFFu);
             k = (int)(v >> 32) + 1;
-            if (k < kcap) __builtin_prefetch(pr + pactae::pr_index(m, k, n));
+            if (k < kcap) __builtin_prefetch(pr + (full ? (size_t)m * n + k : pactae::pr_index(m, k, n)));
             break;
           }
           ++k;
@@ -1284,6 +1289,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       int need = 0;
       for (int b = 0; b < bailed; ++b) need += win_slot[bail[b].x] < 0;
       if ((int)win_men.size() + need > kMaxWindows) break;
+      int span = 0;
       for (int b = 0; b < bailed; ++b) {
         const int m = bail[b].x;
         if (win_slot[m] < 0) {
@@ -1291,19 +1297,21 @@ struct smp_engine : public pactae::AbsSmpEngine {
           win_men.push_back(m);
         }
         win_start[m] = bail[b].y;
-        h_winreq[b] = make_int4(m, bail[b].y, win_slot[m], 0);
+        win_len[m] = win_len[m] == 0 ? kWinFirst : std::min(kWinE, 2 * win_len[m]);
+        h_winreq[b] = make_int4(m, bail[b].y, win_slot[m], win_len[m]);
+        span = std::max(span, win_len[m]);
       }
       // Search the women's rows while that reads less than building the rank
       // matrix would (6 n^2 bytes); otherwise build it once and gather.
       const bool have_rank = rank_built || d_rank_in;
-      if (!have_rank && (long long)bailed * kWinE * 4 > 6ll * n) {
+      if (!have_rank && (long long)bailed * span * 4 > 6ll * n) {
         if (int rc = ensure_rank_matrix()) return rc;
       }
       const bool gather = rank_built || d_rank_in;
       const unsigned seq = ++flag_seq;
       SMP_HIP_TRY(pactae::launch_window(d_pref_m, d_pref_w, gather && !d_rank_in ? d_rank_w : nullptr, d_rank_in, n,
-                                        h_winreq_dev, bailed, kWinE, h_win_dev, d_sc, h_flags_dev + kFlagWindow, seq,
-                                        stream));
+                                        h_winreq_dev, bailed, span, kWinE, h_win_dev, d_sc, h_flags_dev + kFlagWindow,
+                                        seq, stream));
       SMP_HIP_TRY(wait_flag(kFlagWindow, seq));
       ++rounds;
       times.tail_windows += bailed;

@@ -418,11 +418,13 @@ constexpr int kGatherCols = 256;
 
 // RankT: uint16_t (the rank matrix built on the device) or int32_t (a rank
 // matrix given as input); a rank outside [0, n) is reported as a bad entry.
+// row_major: out[m * K + k] (a whole man's list contiguous, for a host walking
+// long chains) instead of the column-blocked pr_index layout.
 template <typename RankT>
 __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ pref_m,
                                                    const RankT* __restrict__ rank_w,
                                                    uint32_t* __restrict__ pr, int n, int K,
-                                                   int tiles, int per_xcd, Scalars* sc) {
+                                                   int tiles, int per_xcd, Scalars* sc, int row_major) {
   const int b = blockIdx.x;
   const int t = (b & 7) * per_xcd + (b >> 3);
   if (t >= tiles) return;
@@ -444,7 +446,7 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
     } else {
       bad = 1;
     }
-    pr[pr_index(m, k, n)] = e;
+    pr[row_major ? (size_t)m * K + k : pr_index(m, k, n)] = e;
   }
   if (bad) atomicOr(&sc->error, kErrEntryRange);
 }
@@ -820,8 +822,9 @@ __global__ __launch_bounds__(256) void k_export_state(Scalars* sc, const unsigne
   }
 }
 
-// Extension windows for the host tail: for request i = (m, k0, slot) and j < E,
-// entry (w, rank) of position k0 + j of man m's list goes to out[slot * E + j]
+// Extension windows for the host tail: for request i = (m, k0, slot, len) and
+// j < len <= E, entry (w, rank) of position k0 + j of man m's list goes to
+// out[slot * E + j]
 // (pinned host memory, written through its device-visible address), or
 // kWinInvalid for a position past the list's end or a malformed entry.  The rank
 // comes from the rank matrix when one exists (rank16, or rank32 given as input),
@@ -837,48 +840,61 @@ __device__ inline void publish_blocks(Scalars* sc, unsigned* h_flag, unsigned se
   }
 }
 
-__global__ __launch_bounds__(256) void k_window_search(const int32_t* __restrict__ pref_m,
-                                                       const int32_t* __restrict__ pref_w, int n,
-                                                       const int4* __restrict__ req, int E,
-                                                       uint32_t* __restrict__ out, Scalars* sc, unsigned* h_flag,
-                                                       unsigned seq) {
-  __shared__ int found;
-  const int i = (int)blockIdx.x / E, j = (int)blockIdx.x - i * E;
-  const int4 q = req[i];
-  const int m = q.x, k = q.y + j;
-  uint32_t* dst = out + (size_t)q.z * E + j;
-  if (threadIdx.x == 0) found = -1;
+// Four entries per 1024-thread workgroup, one per 256-thread quarter (fewer
+// workgroups: each one's system-scope release before the flag costs more than
+// its search).
+__global__ __launch_bounds__(1024) void k_window_search(const int32_t* __restrict__ pref_m,
+                                                        const int32_t* __restrict__ pref_w, int n,
+                                                        const int4* __restrict__ req, int count, int span, int E,
+                                                        uint32_t* __restrict__ out, Scalars* sc, unsigned* h_flag,
+                                                        unsigned seq) {
+  __shared__ int found[4];
+  const int quarter = (int)threadIdx.x >> 8, tq = (int)threadIdx.x & 255;
+  const int entry = (int)blockIdx.x * 4 + quarter;
+  const int i = entry / span, j = entry - i * span;
+  int m = -1, k = -1, len = 0, slot = 0;
+  if (i < count) {
+    const int4 q = req[i];
+    m = q.x;
+    k = q.y + j;
+    slot = q.z;
+    len = q.w;  // entries of this window (at most E)
+  }
+  const bool used = i < count && j < len;
+  if (tq == 0) found[quarter] = -1;
   __syncthreads();
   int w = -1;
-  if ((unsigned)m < (unsigned)n && (unsigned)k < (unsigned)n) w = pref_m[(size_t)m * n + k];
+  if (used && (unsigned)m < (unsigned)n && (unsigned)k < (unsigned)n) w = pref_m[(size_t)m * n + k];
   if ((unsigned)w < (unsigned)n) {
     const int32_t* row = pref_w + (size_t)w * n;
     if ((n & 3) == 0) {
+      // 16 loads per thread in flight: one round for rows up to 16,384 entries.
       const int4* row4 = reinterpret_cast<const int4*>(row);
       const int nq = n >> 2;
-      for (int c0 = (int)threadIdx.x; c0 < nq; c0 += 256 * 4) {
-        int4 v[4];
+      for (int c0 = tq; c0 < nq; c0 += 256 * 16) {
+        int4 v[16];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = row4[min(c0 + 256 * u, nq - 1)];
+        for (int u = 0; u < 16; ++u) v[u] = row4[min(c0 + 256 * u, nq - 1)];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 16; ++u) {
           const int c = c0 + 256 * u;
           if (c < nq) {
-            if (v[u].x == m) found = 4 * c;
-            if (v[u].y == m) found = 4 * c + 1;
-            if (v[u].z == m) found = 4 * c + 2;
-            if (v[u].w == m) found = 4 * c + 3;
+            if (v[u].x == m) found[quarter] = 4 * c;
+            if (v[u].y == m) found[quarter] = 4 * c + 1;
+            if (v[u].z == m) found[quarter] = 4 * c + 2;
+            if (v[u].w == m) found[quarter] = 4 * c + 3;
           }
         }
       }
     } else {
-      for (int r = (int)threadIdx.x; r < n; r += 256)
-        if (row[r] == m) found = r;
+      for (int r = tq; r < n; r += 256)
+        if (row[r] == m) found[quarter] = r;
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int r = found;
+  if (tq == 0 && used) {
+    uint32_t* dst = out + (size_t)slot * E + j;
+    const int r = found[quarter];
     if ((unsigned)w < (unsigned)n && r >= 0) {
       *dst = pr_pack((uint32_t)w, (uint32_t)r);
     } else {
@@ -892,12 +908,12 @@ __global__ __launch_bounds__(256) void k_window_search(const int32_t* __restrict
 template <typename RankT>
 __global__ __launch_bounds__(256) void k_window_gather(const int32_t* __restrict__ pref_m,
                                                        const RankT* __restrict__ rank_w, int n,
-                                                       const int4* __restrict__ req, int count, int E,
+                                                       const int4* __restrict__ req, int count, int span, int E,
                                                        uint32_t* __restrict__ out, Scalars* sc, unsigned* h_flag,
                                                        unsigned seq) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t < count * E) {
-    const int i = t / E, j = t - i * E;
+  if (t < count * span && t % span < req[t / span].w) {
+    const int i = t / span, j = t - i * span;
     const int4 q = req[i];
     const int m = q.x, k = q.y + j;
     uint32_t e = kWinInvalid;
@@ -1755,25 +1771,25 @@ hipError_t set_invert_lds_limit(int bytes) {
 
 template <typename RankT>
 static hipError_t launch_gather_pr_t(const int32_t* pref_m, const RankT* rank_w, uint32_t* pr, int n, int K,
-                                     Scalars* sc, hipStream_t s) {
+                                     Scalars* sc, hipStream_t s, int row_major) {
   const int men_blocks = (n + kGatherMen - 1) / kGatherMen;
   const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
   const long long tiles = (long long)men_blocks * col_tiles;
   if (tiles > (1ll << 30)) return hipErrorInvalidValue;
   const int per_xcd = (int)((tiles + 7) / 8);
   hipLaunchKernelGGL((k_gather_pr<RankT>), dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
-                     (int)tiles, per_xcd, sc);
+                     (int)tiles, per_xcd, sc, row_major);
   return hipGetLastError();
 }
 
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s) {
-  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s);
+                            Scalars* sc, hipStream_t s, int row_major) {
+  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s, row_major);
 }
 
 hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s) {
-  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s);
+                            Scalars* sc, hipStream_t s, int row_major) {
+  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s, row_major);
 }
 
 hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, int waves,
@@ -1799,18 +1815,18 @@ hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scala
 }
 
 hipError_t launch_window(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank16, const int32_t* rank32,
-                         int n, const int4* req, int count, int E, uint32_t* out, Scalars* sc, unsigned* h_flag,
-                         unsigned seq, hipStream_t s) {
-  if (count <= 0) return hipErrorInvalidValue;
+                         int n, const int4* req, int count, int span, int E, uint32_t* out, Scalars* sc,
+                         unsigned* h_flag, unsigned seq, hipStream_t s) {
+  if (count <= 0 || span <= 0 || span > E) return hipErrorInvalidValue;
   if (rank32) {
-    hipLaunchKernelGGL(k_window_gather<int32_t>, dim3((count * E + 255) / 256), dim3(256), 0, s, pref_m, rank32, n,
-                       req, count, E, out, sc, h_flag, seq);
+    hipLaunchKernelGGL(k_window_gather<int32_t>, dim3((count * span + 255) / 256), dim3(256), 0, s, pref_m, rank32, n,
+                       req, count, span, E, out, sc, h_flag, seq);
   } else if (rank16) {
-    hipLaunchKernelGGL(k_window_gather<uint16_t>, dim3((count * E + 255) / 256), dim3(256), 0, s, pref_m, rank16, n,
-                       req, count, E, out, sc, h_flag, seq);
+    hipLaunchKernelGGL(k_window_gather<uint16_t>, dim3((count * span + 255) / 256), dim3(256), 0, s, pref_m, rank16,
+                       n, req, count, span, E, out, sc, h_flag, seq);
   } else {
-    hipLaunchKernelGGL(k_window_search, dim3(count * E), dim3(256), 0, s, pref_m, pref_w, n, req, E, out, sc, h_flag,
-                       seq);
+    hipLaunchKernelGGL(k_window_search, dim3((count * span + 3) / 4), dim3(1024), 0, s, pref_m, pref_w, n, req, count,
+                       span, E, out, sc, h_flag, seq);
   }
   return hipGetLastError();
 }
