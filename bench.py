@@ -53,7 +53,8 @@ def measured_traffic(kernel, n, K=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration), if
     they were taken at this n (and prefix width K, where it matters); else None."""
-    path = os.path.join(ROOT, "profiles", f"counters_{kernel}_n{n}.json")
+    name = f"counters_{kernel}_n{n}" + (f"_K{K}" if K is not None else "") + ".json"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -504,6 +505,8 @@ def main() -> int:
                     help="only run the 16-byte streaming copy kernel over this many MiB "
                          "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the headline and CPU legs (for profiling the batched or serving leg alone)")
     ap.add_argument("--no-fig9", action="store_true", help="skip the reference's RANDOM(5) and SOLO legs")
     ap.add_argument("--fig9-n", type=int, default=20000)
     ap.add_argument("--no-ranked", action="store_true",
@@ -527,6 +530,15 @@ def main() -> int:
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
         return 0
 
+    if args.no_single:  # the other legs alone (profiling runs)
+        out = {"metric": "ms/solve at n=20k (bit-exact); batched solves/s vs looped plain launches", "value": None,
+               "unit": "ms/solve", "n_gpus": 1, "note": "--no-single: the headline leg was not run"}
+        if not args.no_batch:
+            out["batched"], _ = batched(smp, args)
+        if not args.no_serve:
+            out["serving"], _ = serving(smp, args)
+        print(json.dumps(out), flush=True)
+        return 0
     pm, pw, match, inst, walls, profs, elapsed, headline, tune, gpu_base, ranked = single_instance(smp, args)
     n = args.n
     ms_per_solve = elapsed * 1e3 / args.steps

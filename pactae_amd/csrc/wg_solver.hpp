@@ -599,29 +599,77 @@ __global__ __launch_bounds__(BLOCK) void k_rank_rows16(const Task* __restrict__ 
     int bad = 0;
     for (int c = lane; c < n; c += 64) buf[c] = 0xFFFFu;
     wave_sync_lds();
+    // The row as 16-byte loads between a scalar head (up to 16-byte alignment:
+    // rows of odd-sized instances start anywhere) and a scalar tail, kIn vectors
+    // per lane in flight.
     const int32_t* src = tasks[ti].pw + (size_t)row * n;
-    for (int r0 = lane; r0 < n; r0 += 64 * kIn) {
-      int v[kIn];
+    const int head = min(n, (int)((4 - (((uintptr_t)src >> 2) & 3)) & 3));
+    const int nvec = (n - head) >> 2;
+    const int tail0 = head + 4 * nvec;
+    if (lane < head) {
+      const int v = __builtin_nontemporal_load(src + lane);
+      if ((unsigned)v < (unsigned)n) buf[v] = (uint16_t)lane;
+      else bad |= kErrEntryRange;
+    }
+    if (lane < n - tail0) {
+      const int v = __builtin_nontemporal_load(src + tail0 + lane);
+      if ((unsigned)v < (unsigned)n) buf[v] = (uint16_t)(tail0 + lane);
+      else bad |= kErrEntryRange;
+    }
+    const int4* src4 = reinterpret_cast<const int4*>(src + head);
+    for (int q0 = lane; q0 < nvec; q0 += 64 * kIn) {
+      int4 v[kIn];
 #pragma unroll
       for (int j = 0; j < kIn; ++j) {
-        const int r = r0 + 64 * j;
-        v[j] = r < n ? __builtin_nontemporal_load(src + r) : 0;
+        const int q = min(q0 + 64 * j, nvec - 1);
+        v[j].x = __builtin_nontemporal_load(&src4[q].x);
+        v[j].y = __builtin_nontemporal_load(&src4[q].y);
+        v[j].z = __builtin_nontemporal_load(&src4[q].z);
+        v[j].w = __builtin_nontemporal_load(&src4[q].w);
       }
 #pragma unroll
       for (int j = 0; j < kIn; ++j) {
-        const int r = r0 + 64 * j;
-        if (r < n) {
-          if ((unsigned)v[j] < (unsigned)n) buf[v[j]] = (uint16_t)r;
-          else bad |= kErrEntryRange;
+        const int q = q0 + 64 * j;
+        if (q < nvec) {
+          const int r = head + 4 * q;
+          const int e[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            if ((unsigned)e[t] < (unsigned)n) buf[e[t]] = (uint16_t)(r + t);
+            else bad |= kErrEntryRange;
+          }
         }
       }
     }
     wave_sync_lds();
+    // The ranks as 16-byte stores (eight per store) between a scalar head and
+    // tail, the same way.
     int16_t* dst = tasks[ti].rank + (size_t)row * n;
-    for (int c = lane; c < n; c += 64) {
-      const uint16_t v = buf[c];
+    const int whead = min(n, (int)((8 - (((uintptr_t)dst >> 1) & 7)) & 7));
+    const int wvec = (n - whead) >> 3;
+    const int wtail0 = whead + 8 * wvec;
+    if (lane < whead) {
+      const uint16_t v = buf[lane];
       if (v == 0xFFFFu) bad |= kErrNotPerm;
-      dst[c] = (int16_t)v;
+      dst[lane] = (int16_t)v;
+    }
+    if (lane < n - wtail0) {
+      const uint16_t v = buf[wtail0 + lane];
+      if (v == 0xFFFFu) bad |= kErrNotPerm;
+      dst[wtail0 + lane] = (int16_t)v;
+    }
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    v4u* dst8 = reinterpret_cast<v4u*>(dst + whead);
+    for (int q = lane; q < wvec; q += 64) {
+      const uint16_t* b = buf + whead + 8 * q;
+      uint32_t wd[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t lo = b[2 * t], hi = b[2 * t + 1];
+        if (lo == 0xFFFFu || hi == 0xFFFFu) bad |= kErrNotPerm;
+        wd[t] = lo | (hi << 16);
+      }
+      __builtin_nontemporal_store(v4u{wd[0], wd[1], wd[2], wd[3]}, dst8 + q);
     }
     if (bad) atomicOr(&errs[(size_t)ti * err_stride], bad);
     wave_sync_lds();  // every lane has read the row before the next reset
