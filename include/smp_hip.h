@@ -210,9 +210,9 @@ typedef struct smp_times {
   int32_t rank_matrix;     /* 1 if the n x n rank matrix was built (full init, a
                               bucket overflow, or a chain past the prefix)        */
   int32_t copy_requeues;   /* copy-stream pairs the engine has replaced so far
-                              because the PR-prefix copies had waited for the
-                              proposal kernel (their stream shared its hardware
-                              queue); at most 8                                   */
+                              because the PR-prefix copies had started only after
+                              the proposal kernel (their stream shared its
+                              hardware queue); at most 8                          */
   int32_t copy_serialized; /* 1 if this run's prefix copies waited for the
                               proposal kernel and no replacement was left          */
   int32_t tail_windows;    /* extension windows (the next 256 entries of one man's

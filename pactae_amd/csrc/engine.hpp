@@ -880,11 +880,23 @@ struct smp_engine : public pactae::AbsSmpEngine {
     SMP_HIP_TRY(hipEventRecord(ev[2], stream));
     blocks_copied = 0;
     blocks_ready = 0;
+    return SMP_OK;
+  }
+
+  // The PR prefix copies to the host on the copy streams, each waiting for the
+  // end of init (ev[2]).  Queued after the proposal kernel and the state export
+  // (not before them): a copy stream that shares a hardware queue with the
+  // engine's stream then waits behind the kernel, which check_copy_overlap sees
+  // and repairs, instead of holding the kernel back until its copy is done.
+  int queue_prefix_copies() {
     if (T > 0 && opts.prefix_copy == 0) {
       const size_t blk = (size_t)n * pactae::kPrBlock;
       const int nb = (K + pactae::kPrBlock - 1) / pactae::kPrBlock;
       SMP_HIP_TRY(hipStreamWaitEvent(copy_stream, ev[2], 0));
       SMP_HIP_TRY(hipStreamWaitEvent(copy_stream2, ev[2], 0));
+      // When each copy stream starts its first copy (check_copy_overlap).
+      SMP_HIP_TRY(hipEventRecord(ev[7], copy_stream));
+      SMP_HIP_TRY(hipEventRecord(ev[8], copy_stream2));
       for (int b = 0; b < nb && b < kMaxBlocks; ++b) {
         hipStream_t cs = b == 0 ? copy_stream : copy_stream2;
         SMP_HIP_TRY(hipMemcpyAsync(h_pr + b * blk, d_pr + b * blk, blk * sizeof(uint32_t),
@@ -1054,6 +1066,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
                                               h_partner_dev, h_parked_dev, h_flags_dev + kFlagExport,
                                               export_seq, stream));
       SMP_HIP_TRY(hipEventRecord(ev[5], stream));
+      if (int st = queue_prefix_copies()) return st;
       // While the kernel runs: pull the PR prefix blocks into the host caches as
       // they land (a DMA-written buffer starts out in DRAM only).  Block 0 always;
       // later blocks only while the kernel is still running.
@@ -1214,6 +1227,32 @@ struct smp_engine : public pactae::AbsSmpEngine {
         int m = cm[i], k = ck[i];
         bool ended = false;  // chain reached a free woman, or left the entries the host holds
         for (;;) {
+          // Eight positions of one 32-entry block of the prefix (or of the whole
+          // matrix) at a time, without branches: the women's words are eight
+          // independent loads, and the first position where the man beats the
+          // woman's partner is found from a bit mask.  Positions before it are
+          // refusals; the candidate itself goes through the step below.
+          if ((full ? k + 8 <= n : (k + 8 <= kcap && (k & 31) <= 24 && (k >> 5) < blocks_ready))) {
+            const uint32_t* p8 = pr + (full ? (size_t)m * n + k : pactae::pr_index(m, k, n));
+            uint32_t cand = 0, range = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const uint32_t e8 = p8[j];
+              const uint32_t w8 = pactae::pr_woman(e8);
+              range |= (uint32_t)(w8 >= (uint32_t)n) << j;
+              const uint32_t wc = w8 < (uint32_t)n ? w8 : 0u;
+              cand |= (uint32_t)(pactae::pr_rank(e8) < (uint32_t)(st[wc] & 0xFFFFu)) << j;
+            }
+            const uint32_t stop = cand | range;  // a malformed entry goes to the step below
+            if (stop == 0) {
+              props += 8;
+              k += 8;
+              continue;
+            }
+            const int j0 = __builtin_ctz(stop);
+            props += j0;
+            k += j0;
+          }
           uint32_t e;
           if (full) {
             if (k >= n) return tail_error("host tail: a man exhausted his list");
@@ -1397,26 +1436,26 @@ struct smp_engine : public pactae::AbsSmpEngine {
 
   // The prefix copies must run beside the proposal kernel.  Streams share the
   // device's few hardware queues; when a copy stream lands on the compute
-  // stream's queue, its copies wait for the kernel instead.  Seen from the events
-  // (block 0 landing only after a kernel that ran well past one block's copy
-  // time), the copy streams are replaced, which moves them to other queues.
-  // Both copy streams are checked, by the first block each carries (block 0 on
-  // copy_stream, block 1 on copy_stream2): a block the host tail reaches late
-  // makes it wait just the same.
+  // stream's queue, its copies wait for the kernel instead.  Seen from the
+  // events, the copy streams are then replaced, which moves them to other
+  // queues (at most 8 times per engine).  Both copy streams are checked.
   void check_copy_overlap() {
-    float copy_ms = 0, kern_ms = 0;
-    if (hipEventQuery(ev_blk[0]) != hipSuccess || hipEventQuery(ev[5]) != hipSuccess ||
-        hipEventElapsedTime(&copy_ms, ev[2], ev_blk[0]) != hipSuccess ||
-        hipEventElapsedTime(&kern_ms, ev[2], ev[5]) != hipSuccess) {
+    // A copy stream that shares the engine's hardware queue starts its copies
+    // only once the kernel and the export ahead of it in that queue are done:
+    // its start event then comes at or after the end of the phase (ev[5]).  A
+    // copy that merely outlasts the kernel starts right after init and is fine.
+    float kern_ms = 0, s0 = 0, s1 = 0;
+    if (hipEventQuery(ev[5]) != hipSuccess || hipEventQuery(ev[7]) != hipSuccess ||
+        hipEventElapsedTime(&kern_ms, ev[2], ev[5]) != hipSuccess ||
+        hipEventElapsedTime(&s0, ev[2], ev[7]) != hipSuccess) {
       (void)hipGetLastError();
       return;
     }
-    float copy1_ms = 0;
-    if (blocks_copied > 1 && hipEventQuery(ev_blk[1]) == hipSuccess &&
-        hipEventElapsedTime(&copy1_ms, ev[2], ev_blk[1]) == hipSuccess)
-      copy_ms = std::max(copy_ms, copy1_ms);
+    if (blocks_copied > 1 && hipEventQuery(ev[8]) == hipSuccess &&
+        hipEventElapsedTime(&s1, ev[2], ev[8]) == hipSuccess)
+      s0 = std::max(s0, s1);
     (void)hipGetLastError();
-    const bool waited = kern_ms > 0.12f && copy_ms >= kern_ms;
+    const bool waited = kern_ms > 0.05f && s0 >= 0.9f * kern_ms;
     times.copy_serialized = waited && requeues >= 8 ? 1 : 0;
     if (waited && requeues < 8) {
       (void)hipStreamSynchronize(copy_stream);
