@@ -312,7 +312,7 @@ def fig9_legs(smp, args):
         res = {"workload": name, "n": n, "seed": args.seed + 0x90, "ms_per_solve": round(statistics.median(walls), 3),
                "runs": runs, "proposals": p["proposals"], "propose_kernel_ms": round(p["propose_kernel_ms"], 3),
                "host_tail_ms": round(p["host_tail_ms"], 3), "host_proposals": p["host_proposals"],
-               "tail_windows": p["tail_windows"], "tail_full_matrix": p["tail_full_matrix"],
+               "tail_windows": p["tail_windows"], "tail_rounds": p["tail_rounds"], "tail_bands": p["tail_bands"], "tail_full_matrix": p["tail_full_matrix"],
                "resume_ms": round(p["resume_ms"], 3)}
         log(f"fig9 {name}: {res}")
         out.append(res)

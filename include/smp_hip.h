@@ -131,7 +131,11 @@ typedef struct smp_opts {
                              blocks on a copy stream                              */
   int32_t schedule;       /* enum smp_schedule                                     */
   int32_t init_mode;      /* enum smp_init_mode                                    */
-  int32_t reserved[3];
+  int32_t tail_window_entries; /* host-tail extension-window storage per solve, in
+                             list entries: 0 = auto (512 n, between 65,536 and
+                             4,194,304); a smaller value caps it.  Chains that
+                             find no storage left go back to the GPU            */
+  int32_t reserved[2];
 } smp_opts;
 
 enum smp_init_mode {
@@ -215,13 +219,18 @@ typedef struct smp_times {
                               hardware queue); at most 8                          */
   int32_t copy_serialized; /* 1 if this run's prefix copies waited for the
                               proposal kernel and no replacement was left          */
-  int32_t tail_windows;    /* extension windows (the next 256 entries of one man's
-                              list) the GPU computed for host-tail chains that
-                              walked past the PR prefix                           */
+  int32_t tail_windows;    /* extension windows (the next 64 to 4,096 entries of
+                              one man's list) the GPU computed for host-tail
+                              chains that walked past the PR prefix, and for men
+                              matched near its end                                */
   int32_t tail_full_matrix; /* 1 if the host tail's chains went so deep that the
                               whole PR matrix was gathered and copied to the host
                               (4 n^2 bytes, buffers allocated on first use)       */
-  int32_t reserved[3];
+  int32_t tail_rounds;     /* GPU round trips for extension windows               */
+  int32_t tail_bands;      /* bands of columns of every man's list (the whole
+                              matrix included) the host tail took because chains
+                              kept leaving what it held                           */
+  int32_t reserved[1];
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;

@@ -48,7 +48,8 @@ class SmpOpts(ctypes.Structure):
         ("prefix_copy", i32),
         ("schedule", i32),
         ("init_mode", i32),
-        ("reserved", i32 * 3),
+        ("tail_window_entries", i32),
+        ("reserved", i32 * 2),
     ]
 
 
@@ -85,7 +86,9 @@ class SmpTimes(ctypes.Structure):
         ("copy_serialized", i32),
         ("tail_windows", i32),
         ("tail_full_matrix", i32),
-        ("reserved", i32 * 3),
+        ("tail_rounds", i32),
+        ("tail_bands", i32),
+        ("reserved", i32 * 1),
     ]
 
     def as_dict(self) -> dict:

@@ -54,8 +54,9 @@ static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
   else
     smp_default_opts(&e->opts);
   if (e->opts.init_mode < SMP_INIT_AUTO || e->opts.init_mode > SMP_INIT_SPARSE ||
-      e->opts.schedule < SMP_SCHEDULE_BAMBOO || e->opts.schedule >= SMP_SCHEDULE_COUNT) {
-    pactae::set_error("opts: unknown init_mode or schedule");
+      e->opts.schedule < SMP_SCHEDULE_BAMBOO || e->opts.schedule >= SMP_SCHEDULE_COUNT ||
+      e->opts.tail_window_entries < 0) {
+    pactae::set_error("opts: unknown init_mode or schedule, or a negative tail_window_entries");
     return SMP_ERR_INVALID;
   }
   if (int st = e->setup()) return st;

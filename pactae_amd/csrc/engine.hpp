@@ -23,6 +23,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -176,29 +177,52 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int2* d_starts = nullptr;     // chains handed back by the host tail
   bool host_finished = false;   // the host tail completed every chain
   // Extension windows: when a host chain walks past the host's PR prefix, the
-  // GPU computes the next kWinE entries (woman, rank) of that man's list into
-  // pinned host memory and the host continues the chain itself.  A man's window
-  // replaces his previous one (his list position only grows).  Slots are handed
-  // out per solve, at most kMaxWindows; past that the chains go back to the GPU.
+  // GPU computes the next entries (woman, rank) of that man's list into pinned
+  // host memory and the host continues the chain itself.  A man's window
+  // replaces his previous one (his list position only grows).  Window storage
+  // (win_cap entries) is handed out in order during a solve; when it runs out
+  // the chains go back to the GPU.
   static constexpr int kFlagWindow = 2;
-  static constexpr int kWinE = 256;
-  static constexpr int kMaxWindows = 4096;
-  uint32_t* h_win = nullptr;   // kMaxWindows * kWinE entries
+  size_t win_cap = 0;
+  size_t win_used = 0;
+  uint32_t* h_win = nullptr;   // win_cap entries
   uint32_t* h_win_dev = nullptr;
-  int4* h_winreq = nullptr;    // (man, first position, slot, 0) per request of a round
+  int4* h_winreq = nullptr;    // (man, first position, offset in the round, length); n of them
   int4* h_winreq_dev = nullptr;
-  std::vector<int> win_slot, win_start, win_len, win_men;
+  std::vector<int> win_off, win_start, win_len, win_men;
   // A man's first window holds kWinFirst entries and each later one twice the
-  // last, up to kWinE: most chains that pass the prefix end a few dozen
-  // positions past it, and each entry of a window costs a search of a row.
+  // last, up to kWinMax: most chains that pass the prefix end a few dozen
+  // positions past it, and without a rank matrix each entry costs a search of
+  // a row.
   static constexpr int kWinFirst = 64;
-  // Rounds of windows after which the host takes the whole PR matrix instead
-  // (allocated on first use: 4 n^2 bytes in HBM and as much pinned host memory):
-  // when the rounds' GPU round trips (~50 us each) have cost about what copying
-  // the matrix over PCIe (~50 GB/s) would, 1.6e-6 n^2 rounds, and at least 8.
-  int window_rounds_limit() const { return std::max(8, (int)std::min<long long>(1 << 20, 16ll * n * n / 10000000)); }
-  uint32_t* d_pr_full = nullptr;
-  uint32_t* h_pr_full = nullptr;
+  static constexpr int kWinMax = 4096;
+  // From the second round of windows on (or from the first when a rank matrix
+  // exists), each round also gives a window to every man matched at most
+  // kWinAhead positions before the end of the entries the host holds for him,
+  // so that a chain that displaces him continues without another round trip.
+  // deep_w collects the women whose partner may be such a man.
+  static constexpr int kWinAhead = 8;
+  size_t window_cap() const {
+    return opts.tail_window_entries > 0 ? std::min(win_cap, (size_t)opts.tail_window_entries) : win_cap;
+  }
+  std::vector<int> deep_w, deep_stamp;
+  int deep_round = 0;
+  // Bands: when chains keep leaving what the host holds one man at a time, each
+  // a GPU round trip, the host takes a band of columns of every man's list
+  // instead (row-major, gathered on the GPU into d_band and copied into h_band,
+  // both grown on demand).  A band starts at the end of the prefix and reaches
+  // 64 positions past the deepest position a chain has stopped at, and is at
+  // least as wide as the prefix, 256 columns and twice the last band; one wider
+  // than n / 4 columns is the whole matrix instead.  A band of c columns is
+  // taken once the rounds since the last one (for the whole matrix: all rounds
+  // of the solve) have cost about what copying it over PCIe would (~50 GB/s
+  // against ~50 us per round: 1.6e-6 n c rounds, and at least 8).
+  int band_rounds(int cols) const {
+    return std::max(8, (int)std::min<long long>(1 << 20, 16ll * n * cols / 10000000));
+  }
+  uint32_t* d_band = nullptr;
+  uint32_t* h_band = nullptr;
+  size_t band_cap = 0;  // entries of both
 
   hipStream_t stream = nullptr, copy_stream = nullptr, copy_stream2 = nullptr;
   static constexpr int kMaxBlocks = 8;  // PR prefix blocks copied to the host (K <= 256)
@@ -291,9 +315,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     (void)hipHostFree(h_parked);
     (void)hipHostFree(h_flags);
     h_flags = h_flags_dev = nullptr;
-    (void)hipFree(d_pr_full);
-    (void)hipHostFree(h_pr_full);
-    d_pr_full = h_pr_full = nullptr;
+    (void)hipFree(d_band);
+    (void)hipHostFree(h_band);
+    d_band = h_band = nullptr;
+    band_cap = 0;
     (void)hipHostFree(h_win);
     (void)hipHostFree(h_winreq);
     h_win = h_win_dev = nullptr;
@@ -375,18 +400,19 @@ struct smp_engine : public pactae::AbsSmpEngine {
       pactae::set_error("allocation of the result buffers failed");
       return SMP_ERR_NOMEM;
     }
-    if (hipHostMalloc(&h_win, (size_t)kMaxWindows * kWinE * sizeof(uint32_t), pactae::kHostAllocFlags) !=
-            hipSuccess ||
-        hipHostMalloc(&h_winreq, (size_t)kMaxWindows * sizeof(int4), pactae::kHostAllocFlags) != hipSuccess ||
+    win_cap = std::min<size_t>(1u << 22, std::max<size_t>(1u << 16, (size_t)8 * n * kWinFirst));
+    if (hipHostMalloc(&h_win, win_cap * sizeof(uint32_t), pactae::kHostAllocFlags) != hipSuccess ||
+        hipHostMalloc(&h_winreq, (size_t)std::max(n, 1) * sizeof(int4), pactae::kHostAllocFlags) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&h_win_dev), h_win, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&h_winreq_dev), h_winreq, 0) != hipSuccess) {
       (void)hipGetLastError();
       pactae::set_error("allocation of the host tail's extension windows failed");
       return SMP_ERR_NOMEM;
     }
-    win_slot.assign(n, -1);
+    win_off.assign(n, -1);
     win_start.assign(n, 0);
     win_len.assign(n, 0);
+    deep_stamp.assign(n, 0);
     // Sparse prefix pass buffers, for prefixes up to 256 columns (any width when
     // n <= 2048, where n x n buckets are small).
     K_sparse_alloc = n <= 2048 ? K_alloc : std::min(K_alloc, 256);
@@ -1114,38 +1140,43 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return true;
   }
 
-  // The whole PR matrix (every position of every man's list with its rank, row
-  // after row: a man's list is contiguous) gathered on the GPU into d_pr_full and copied
-  // into h_pr_full, for a host tail whose chains walk deep into the lists (the
-  // reference copies its whole PR matrix to the host for the same purpose,
-  // smp_engine_bamboo.h:41-61).  Leaves h_pr_full null, and returns SMP_OK, if
-  // the buffers cannot be allocated: the caller then keeps to windows.
-  int host_full_matrix() {
-    const size_t entries = pactae::pr_entries(n_alloc, n_alloc);
-    if (!d_pr_full) {
-      if (hipMalloc(&d_pr_full, entries * sizeof(uint32_t)) != hipSuccess) {
+  // Columns c0 .. c0 + cols - 1 of the PR matrix (those positions of every
+  // man's list with their ranks, row after row: a man's band is contiguous)
+  // gathered on the GPU into d_band and copied into h_band, for a host tail
+  // whose chains walk deep into the lists; c0 = 0, cols = n is the whole matrix
+  // (the reference copies its whole PR matrix to the host for the same
+  // purpose, smp_engine_bamboo.h:41-61).  Sets *ok to false, and returns
+  // SMP_OK, if the buffers cannot be allocated: the caller then keeps to
+  // windows.
+  int host_band(int c0, int cols, bool* ok) {
+    *ok = false;
+    const size_t entries = (size_t)n * cols;
+    if (entries > band_cap) {
+      (void)hipFree(d_band);
+      (void)hipHostFree(h_band);
+      d_band = h_band = nullptr;
+      band_cap = 0;
+      if (hipMalloc(&d_band, entries * sizeof(uint32_t)) != hipSuccess ||
+          hipHostMalloc(&h_band, entries * sizeof(uint32_t), pactae::kHostAllocFlags) != hipSuccess) {
         (void)hipGetLastError();
-        d_pr_full = nullptr;
+        (void)hipFree(d_band);
+        d_band = nullptr;
+        h_band = nullptr;
         return SMP_OK;
       }
-    }
-    if (!h_pr_full) {
-      if (hipHostMalloc(&h_pr_full, entries * sizeof(uint32_t), pactae::kHostAllocFlags) != hipSuccess) {
-        (void)hipGetLastError();
-        h_pr_full = nullptr;
-        return SMP_OK;
-      }
+      band_cap = entries;
     }
     if (d_rank_in) {
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_in, d_pr_full, n, n, d_sc, stream, /*row_major=*/1));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_in, d_band, n, cols, d_sc, stream, /*row_major=*/1, c0));
     } else {
       if (int rc = ensure_rank_matrix()) return rc;
-      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr_full, n, n, d_sc, stream, /*row_major=*/1));
+      SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_band, n, cols, d_sc, stream, /*row_major=*/1, c0));
     }
-    SMP_HIP_TRY(hipMemcpyAsync(h_pr_full, d_pr_full, (size_t)n * n * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               stream));
+    SMP_HIP_TRY(hipMemcpyAsync(h_band, d_band, entries * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     SMP_HIP_TRY(wait_stream(stream));
-    times.tail_full_matrix = 1;
+    if (cols == n) times.tail_full_matrix = 1;
+    ++times.tail_bands;
+    *ok = true;
     return SMP_OK;
   }
 
@@ -1158,11 +1189,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // (the stable matching does not depend on the order of proposals).  A chain
   // that walks past the host's PR prefix (or past its man's extension window)
   // stops and is collected; once every other chain is done, the GPU computes an
-  // extension window for each collected man (the next kWinE entries of his list,
-  // by searching the women's rows, or from the rank matrix when one exists) and
-  // the host continues them, in rounds until every chain has ended.  Only if the
-  // windows run out are the collected chains handed back to the GPU, which
-  // resumes them from the shared partner words with the full rank matrix.
+  // extension window for each collected man (the next entries of his list, by
+  // searching the women's rows, or from the rank matrix when one exists), and
+  // for the men matched near the end of what the host holds for them, and the
+  // host continues the chains, in rounds until every chain has ended.  Only if
+  // the window storage runs out are the collected chains handed back to the
+  // GPU, which resumes them from the shared partner words with the full rank
+  // matrix.
   int host_tail() {
     auto t0 = Clock::now();
     int parked = h_sc->parked_count;
@@ -1192,26 +1225,42 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.handoff_ms = ms_since(t0);
 
     t0 = Clock::now();
+    int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
     hst.resize(n);
+    deep_w.clear();
     for (int w = 0; w < n; ++w) {
       const unsigned long long v = h_partner[w];
-      hst[w] = v == pactae::kEmpty ? kHostFree : hs_pack(pactae::pw_rank(v), pactae::pw_man(v), pactae::pw_pos(v));
+      if (v == pactae::kEmpty) {
+        hst[w] = kHostFree;
+      } else {
+        hst[w] = hs_pack(pactae::pw_rank(v), pactae::pw_man(v), pactae::pw_pos(v));
+        if ((int)pactae::pw_pos(v) + 1 + kWinAhead >= kcap) deep_w.push_back(w);
+      }
     }
-    for (int m : win_men) win_slot[m] = -1, win_len[m] = 0;  // windows of an earlier solve
+    for (int m : win_men) win_off[m] = -1, win_len[m] = 0;  // windows of an earlier solve
     win_men.clear();
+    win_used = 0;
     long long props = 0;
     int bailed = 0;
-    const uint32_t* pr = h_pr;
+    const uint32_t* pr = h_pr;  // the prefix, column-blocked (pr_index)
     uint64_t* st = hst.data();
-    int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
-    bool full = false;  // the host holds every man's whole list (pr = h_pr_full)
+    // A band the host holds: positions kb0 .. kb1 - 1 of every man, row-major
+    // with bw = kb1 - kb0 entries per man (none while kb1 = 0).
+    const uint32_t* band = nullptr;
+    int kb0 = 0, kb1 = 0, bw = 0;
+    int hold = kcap;  // positions below it are held for every man (prefix or band)
     int2* bail = h_parked;  // stopped chains overwrite the consumed parked list (bailed <= taken)
     int cm[kTailSlots], ck[kTailSlots];
     // Walks the `parked` chains of h_parked; the chains that leave the entries
-    // the host holds are collected in bail[0..bailed).
-    auto walk = [&]() -> int {
+    // the host holds are collected in bail[0..bailed).  FULL: the band is the
+    // whole matrix (the loop then has one source of entries).
+    auto walk_impl = [&](auto full_tag) -> int {
+      constexpr bool FULL = decltype(full_tag)::value;
       int act = 0, taken = 0;
       bailed = 0;
+      // What the host holds does not change during a walk: keep it in locals.
+      const uint32_t* const bd = band;
+      const int b0 = kb0, b1 = kb1, bs = bw, kc = kcap, hd = hold;
       // Puts parked chain `taken` into slot s; false if it names a man outside [0,n).
       auto take = [&](int sl) {
         cm[sl] = h_parked[taken].x;
@@ -1227,13 +1276,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
         int m = cm[i], k = ck[i];
         bool ended = false;  // chain reached a free woman, or left the entries the host holds
         for (;;) {
-          // Eight positions of one 32-entry block of the prefix (or of the whole
-          // matrix) at a time, without branches: the women's words are eight
+          // Eight positions of one 32-entry block of the prefix (or of the band)
+          // at a time, without branches: the women's words are eight
           // independent loads, and the first position where the man beats the
           // woman's partner is found from a bit mask.  Positions before it are
           // refusals; the candidate itself goes through the step below.
-          if ((full ? k + 8 <= n : (k + 8 <= kcap && (k & 31) <= 24 && (k >> 5) < blocks_ready))) {
-            const uint32_t* p8 = pr + (full ? (size_t)m * n + k : pactae::pr_index(m, k, n));
+          const bool in_band = FULL ? k + 8 <= n : k >= b0 && k + 8 <= b1;
+          if (in_band || (!FULL && k + 8 <= kc && (k & 31) <= 24 && (k >> 5) < blocks_ready)) {
+            const uint32_t* p8 = FULL      ? bd + (size_t)m * n + k
+                                 : in_band ? bd + (size_t)m * bs + (k - b0)
+                                           : pr + pactae::pr_index(m, k, n);
             uint32_t cand = 0, range = 0;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -1254,25 +1306,28 @@ struct smp_engine : public pactae::AbsSmpEngine {
             k += j0;
           }
           uint32_t e;
-          if (full) {
+          if (FULL) {
             if (k >= n) return tail_error("host tail: a man exhausted his list");
-            e = pr[(size_t)m * n + k];
-          } else if (k < kcap) {
+            e = bd[(size_t)m * n + k];
+          } else if (k >= b0 && k < b1) {
+            e = bd[(size_t)m * bs + (k - b0)];
+          } else if (k < kc) {
             if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
               while (blocks_ready <= (k >> 5)) SMP_HIP_TRY(wait_event(ev_blk[blocks_ready++]));
             }
             e = pr[pactae::pr_index(m, k, n)];
+          } else if (k >= n) {
+            return tail_error("host tail: a man exhausted his list");
           } else {
-            const int sl = win_slot[m];
+            const int wo = win_off[m];
             const int off = k - win_start[m];
-            if (sl < 0 || off < 0 || off >= win_len[m]) {
+            if (wo < 0 || off < 0 || off >= win_len[m]) {
               bail[bailed++] = make_int2(m, k);
               ended = true;
               break;
             }
-            e = h_win[(size_t)sl * kWinE + off];
-            if (e == pactae::kWinInvalid)
-              return tail_error(k >= n ? "host tail: a man exhausted his list" : "host tail: malformed list entry");
+            e = h_win[(size_t)wo + off];
+            if (e == pactae::kWinInvalid) return tail_error("host tail: malformed list entry");
           }
           ++props;
           const uint32_t w = pactae::pr_woman(e), r = pactae::pr_rank(e);
@@ -1280,13 +1335,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
           const uint64_t v = st[w];
           if (r < (uint32_t)(v & 0xFFFFu)) {
             st[w] = hs_pack(r, (uint32_t)m, (uint32_t)k);
+            if (!FULL && k + 1 + kWinAhead >= hd) deep_w.push_back((int)w);  // he may soon leave what the host holds
             if ((v & 0xFFFFu) == kHostFree) {
               ended = true;
               break;
             }
             m = (int)((v >> 16) & 0xFFFFu);
             k = (int)(v >> 32) + 1;
-            if (k < kcap) __builtin_prefetch(pr + (full ? (size_t)m * n + k : pactae::pr_index(m, k, n)));
+            if (FULL) __builtin_prefetch(bd + (size_t)m * n + k);
+            else if (k >= b0 && k < b1) __builtin_prefetch(bd + (size_t)m * bs + (k - b0));
+            else if (k < kc) __builtin_prefetch(pr + pactae::pr_index(m, k, n));
             break;
           }
           ++k;
@@ -1305,57 +1363,106 @@ struct smp_engine : public pactae::AbsSmpEngine {
       }
       return SMP_OK;
     };
-    int rounds = 0;
+    auto walk = [&]() -> int {
+      return kb0 == 0 && kb1 == n ? walk_impl(std::true_type{}) : walk_impl(std::false_type{});
+    };
+    int rounds = 0, band_round = 0, max_stop = 0;
+    long long searched = 0;  // window entries found by searching rows
     for (;;) {
       if (int rc = walk()) return rc;
       if (bailed == 0) break;
-      // Many rounds of windows mean chains that walk deep into long lists (one
-      // GPU round trip per 256 positions of a man): the whole PR matrix is then
-      // gathered on the GPU and copied to the host once, and the host walks
-      // every chain from it.
-      if (rounds >= window_rounds_limit()) {
-        if (int rc = host_full_matrix()) return rc;
-        if (h_pr_full) {
-          pr = h_pr_full;
-          kcap = n;
-          full = true;
-          parked = bailed;
-          continue;
+      for (int b = 0; b < bailed; ++b) max_stop = std::max(max_stop, bail[b].y);
+      // Chains that keep leaving what the host holds, one man at a time, cost a
+      // GPU round trip each: after enough rounds the host takes a band of every
+      // man's list instead, wide enough for the deepest position a chain has
+      // stopped at and wider each time, and finally the whole matrix.
+      if (kb1 < n) {
+        int cols = std::max(bw ? 2 * bw : std::max(kcap, 256), (max_stop + 64 - kcap + 255) / 256 * 256);
+        int c0 = kcap;
+        if (cols > n / 4 || kcap + cols >= n) c0 = 0, cols = n;  // chains that walk this deep: the whole matrix
+        // A partial band against the rounds since the last one, the whole
+        // matrix against every round of this solve.
+        if ((c0 > 0 ? rounds - band_round : rounds) < band_rounds(cols)) c0 = -1;
+        if (c0 >= 0) {
+          bool ok = false;
+          if (int rc = host_band(c0, cols, &ok)) return rc;
+          band = ok ? h_band : nullptr;
+          kb0 = ok ? c0 : 0;
+          kb1 = ok ? c0 + cols : 0;
+          bw = ok ? cols : 0;
+          hold = std::max(kcap, kb1);
+          band_round = rounds;
+          if (ok) {
+            parked = bailed;  // the stopped chains go round again, over the band
+            continue;
+          }
         }
       }
-      // Windows for the stopped chains (a man keeps his slot; his new window
-      // starts where he stopped).  Out of slots: hand them back to the GPU.
-      int need = 0;
-      for (int b = 0; b < bailed; ++b) need += win_slot[bail[b].x] < 0;
-      if ((int)win_men.size() + need > kMaxWindows) break;
-      int span = 0;
-      for (int b = 0; b < bailed; ++b) {
-        const int m = bail[b].x;
-        if (win_slot[m] < 0) {
-          win_slot[m] = (int)win_men.size();
-          win_men.push_back(m);
-        }
-        win_start[m] = bail[b].y;
-        win_len[m] = win_len[m] == 0 ? kWinFirst : std::min(kWinE, 2 * win_len[m]);
-        h_winreq[b] = make_int4(m, bail[b].y, win_slot[m], win_len[m]);
-        span = std::max(span, win_len[m]);
-      }
-      // Search the women's rows while that reads less than building the rank
-      // matrix would (6 n^2 bytes); otherwise build it once and gather.
+      // Windows for the stopped chains, each starting where its chain stopped;
+      // out of window storage, the chains go back to the GPU.
       const bool have_rank = rank_built || d_rank_in;
-      if (!have_rank && (long long)bailed * span * 4 > 6ll * n) {
-        if (int rc = ensure_rank_matrix()) return rc;
+      int count = 0;
+      long long total = 0;
+      auto next_len = [&](int m, int start) {
+        const int len = win_len[m] == 0 ? kWinFirst : std::min(kWinMax, 2 * win_len[m]);
+        return std::max(1, std::min(len, n - start));
+      };
+      auto request = [&](int m, int start, int len) {
+        if (win_len[m] == 0) win_men.push_back(m);
+        h_winreq[count++] = make_int4(m, start, (int)total, len);
+        win_off[m] = (int)(win_used + total);
+        win_start[m] = start;
+        win_len[m] = len;
+        total += len;
+      };
+      {
+        long long need = 0;
+        for (int b = 0; b < bailed; ++b) need += next_len(bail[b].x, bail[b].y);
+        if (win_used + need > window_cap()) break;
+        for (int b = 0; b < bailed; ++b) request(bail[b].x, bail[b].y, next_len(bail[b].x, bail[b].y));
+      }
+      // Men who may soon leave the entries the host holds for them, while the
+      // storage lasts (each man is matched to one woman, and a stopped chain's
+      // man to none, so no man is asked for twice).
+      if (rounds > 0 || have_rank) {
+        ++deep_round;
+        for (int w : deep_w) {
+          if (deep_stamp[w] == deep_round) continue;
+          deep_stamp[w] = deep_round;
+          const uint64_t v = st[w];
+          if ((v & 0xFFFFu) == kHostFree) continue;
+          const int m = (int)((v >> 16) & 0xFFFFu);
+          const int start = std::max((int)(v >> 32) + 1, hold);
+          if (start >= n) continue;
+          if (win_len[m] > 0 && start >= win_start[m] && start + kWinAhead < win_start[m] + win_len[m]) continue;
+          const int len = next_len(m, start);
+          if (win_used + total + len > window_cap()) break;
+          request(m, start, len);
+        }
+        deep_w.clear();
+      }
+      // Search the women's rows (4 n bytes per entry) while the searches of
+      // this solve together read less than building the rank matrix would
+      // (6 n^2 bytes); then build it once and gather.
+      if (!have_rank) {
+        if ((searched + total) * 4 > 6ll * n) {
+          if (int rc = ensure_rank_matrix()) return rc;
+        } else {
+          searched += total;
+        }
       }
       const bool gather = rank_built || d_rank_in;
       const unsigned seq = ++flag_seq;
       SMP_HIP_TRY(pactae::launch_window(d_pref_m, d_pref_w, gather && !d_rank_in ? d_rank_w : nullptr, d_rank_in, n,
-                                        h_winreq_dev, bailed, span, kWinE, h_win_dev, d_sc, h_flags_dev + kFlagWindow,
-                                        seq, stream));
+                                        h_winreq_dev, count, (int)total, h_win_dev + win_used, d_sc,
+                                        h_flags_dev + kFlagWindow, seq, stream));
       SMP_HIP_TRY(wait_flag(kFlagWindow, seq));
+      win_used += (size_t)total;
       ++rounds;
-      times.tail_windows += bailed;
+      times.tail_windows += count;
       parked = bailed;  // the stopped chains, in h_parked[0..bailed), go round again
     }
+    times.tail_rounds += rounds;
     times.host_proposals = props;
     times.bailed = bailed;
     if (bailed == 0) {

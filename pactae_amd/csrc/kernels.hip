@@ -424,7 +424,7 @@ template <typename RankT>
 __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ pref_m,
                                                    const RankT* __restrict__ rank_w,
                                                    uint32_t* __restrict__ pr, int n, int K,
-                                                   int tiles, int per_xcd, Scalars* sc, int row_major) {
+                                                   int tiles, int per_xcd, Scalars* sc, int row_major, int col0) {
   const int b = blockIdx.x;
   const int t = (b & 7) * per_xcd + (b >> 3);
   if (t >= tiles) return;
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(256) void k_gather_pr(const int32_t* __restrict__ p
     const int m = mb * kGatherMen + (i / chunks);
     const int k = ct * kGatherCols + (i % chunks) * 64 + lane;
     if (m >= n || k >= K) continue;
-    const int w = pref_m[(size_t)m * n + k];
+    const int w = pref_m[(size_t)m * n + col0 + k];
     uint32_t e = pr_pack(0, 0xFFFFu);
     if ((unsigned)w < (unsigned)n) {
       const uint32_t r = (uint32_t)rank_w[(size_t)w * n + m];
@@ -822,15 +822,16 @@ __global__ __launch_bounds__(256) void k_export_state(Scalars* sc, const unsigne
   }
 }
 
-// Extension windows for the host tail: for request i = (m, k0, slot, len) and
-// j < len <= E, entry (w, rank) of position k0 + j of man m's list goes to
-// out[slot * E + j]
+// Extension windows for the host tail.  Request i = (m, k0, off, len) asks for
+// positions k0 .. k0 + len - 1 of man m's list; the requests' entries are laid
+// out back to back (off of request i + 1 = off + len of request i, off of
+// request 0 = 0), so entry e < total of the launch belongs to the last request
+// whose off <= e.  Entry (w, rank) of position k0 + j goes to out[off + j]
 // (pinned host memory, written through its device-visible address), or
 // kWinInvalid for a position past the list's end or a malformed entry.  The rank
 // comes from the rank matrix when one exists (rank16, or rank32 given as input),
-// else from a search of woman w's row for m by one workgroup per entry.  The last
-// workgroup to finish releases the writes at system scope and stores seq into
-// *h_flag.
+// else from a search of woman w's row for m.  The last workgroup to finish
+// releases the writes at system scope and stores seq into *h_flag.
 __device__ inline void publish_blocks(Scalars* sc, unsigned* h_flag, unsigned seq) {
   __threadfence_system();
   __syncthreads();
@@ -840,27 +841,37 @@ __device__ inline void publish_blocks(Scalars* sc, unsigned* h_flag, unsigned se
   }
 }
 
+// The request that entry e belongs to (binary search over the ascending offsets).
+__device__ inline int window_request(const int4* __restrict__ req, int count, int e) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (req[mid].z <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Four entries per 1024-thread workgroup, one per 256-thread quarter (fewer
 // workgroups: each one's system-scope release before the flag costs more than
 // its search).
 __global__ __launch_bounds__(1024) void k_window_search(const int32_t* __restrict__ pref_m,
                                                         const int32_t* __restrict__ pref_w, int n,
-                                                        const int4* __restrict__ req, int count, int span, int E,
+                                                        const int4* __restrict__ req, int count, int total,
                                                         uint32_t* __restrict__ out, Scalars* sc, unsigned* h_flag,
                                                         unsigned seq) {
   __shared__ int found[4];
   const int quarter = (int)threadIdx.x >> 8, tq = (int)threadIdx.x & 255;
   const int entry = (int)blockIdx.x * 4 + quarter;
-  const int i = entry / span, j = entry - i * span;
-  int m = -1, k = -1, len = 0, slot = 0;
-  if (i < count) {
-    const int4 q = req[i];
+  int m = -1, k = -1;
+  bool used = false;
+  if (entry < total) {
+    const int4 q = req[window_request(req, count, entry)];
+    const int j = entry - q.z;
+    used = j >= 0 && j < q.w;
     m = q.x;
     k = q.y + j;
-    slot = q.z;
-    len = q.w;  // entries of this window (at most E)
   }
-  const bool used = i < count && j < len;
   if (tq == 0) found[quarter] = -1;
   __syncthreads();
   int w = -1;
@@ -893,7 +904,7 @@ __global__ __launch_bounds__(1024) void k_window_search(const int32_t* __restric
   }
   __syncthreads();
   if (tq == 0 && used) {
-    uint32_t* dst = out + (size_t)slot * E + j;
+    uint32_t* dst = out + entry;
     const int r = found[quarter];
     if ((unsigned)w < (unsigned)n && r >= 0) {
       *dst = pr_pack((uint32_t)w, (uint32_t)r);
@@ -908,24 +919,26 @@ __global__ __launch_bounds__(1024) void k_window_search(const int32_t* __restric
 template <typename RankT>
 __global__ __launch_bounds__(256) void k_window_gather(const int32_t* __restrict__ pref_m,
                                                        const RankT* __restrict__ rank_w, int n,
-                                                       const int4* __restrict__ req, int count, int span, int E,
+                                                       const int4* __restrict__ req, int count, int total,
                                                        uint32_t* __restrict__ out, Scalars* sc, unsigned* h_flag,
                                                        unsigned seq) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t < count * span && t % span < req[t / span].w) {
-    const int i = t / span, j = t - i * span;
-    const int4 q = req[i];
-    const int m = q.x, k = q.y + j;
-    uint32_t e = kWinInvalid;
-    if ((unsigned)m < (unsigned)n && (unsigned)k < (unsigned)n) {
-      const int w = pref_m[(size_t)m * n + k];
-      if ((unsigned)w < (unsigned)n) {
-        const uint32_t r = (uint32_t)rank_w[(size_t)w * n + m];
-        if (r < (uint32_t)n) e = pr_pack((uint32_t)w, r);
+  if (t < total) {
+    const int4 q = req[window_request(req, count, t)];
+    const int j = t - q.z;
+    if (j >= 0 && j < q.w) {
+      const int m = q.x, k = q.y + j;
+      uint32_t e = kWinInvalid;
+      if ((unsigned)m < (unsigned)n && (unsigned)k < (unsigned)n) {
+        const int w = pref_m[(size_t)m * n + k];
+        if ((unsigned)w < (unsigned)n) {
+          const uint32_t r = (uint32_t)rank_w[(size_t)w * n + m];
+          if (r < (uint32_t)n) e = pr_pack((uint32_t)w, r);
+        }
+        if (e == kWinInvalid) atomicOr(&sc->error, kErrEntryRange);
       }
-      if (e == kWinInvalid) atomicOr(&sc->error, kErrEntryRange);
+      out[t] = e;
     }
-    out[(size_t)q.z * E + j] = e;
   }
   publish_blocks(sc, h_flag, seq);
 }
@@ -1771,25 +1784,26 @@ hipError_t set_invert_lds_limit(int bytes) {
 
 template <typename RankT>
 static hipError_t launch_gather_pr_t(const int32_t* pref_m, const RankT* rank_w, uint32_t* pr, int n, int K,
-                                     Scalars* sc, hipStream_t s, int row_major) {
+                                     Scalars* sc, hipStream_t s, int row_major, int col0) {
+  if (col0 < 0 || K < 0 || col0 > n - K || (col0 != 0 && !row_major)) return hipErrorInvalidValue;
   const int men_blocks = (n + kGatherMen - 1) / kGatherMen;
   const int col_tiles = (K + kGatherCols - 1) / kGatherCols;
   const long long tiles = (long long)men_blocks * col_tiles;
   if (tiles > (1ll << 30)) return hipErrorInvalidValue;
   const int per_xcd = (int)((tiles + 7) / 8);
   hipLaunchKernelGGL((k_gather_pr<RankT>), dim3(per_xcd * 8), dim3(256), 0, s, pref_m, rank_w, pr, n, K,
-                     (int)tiles, per_xcd, sc, row_major);
+                     (int)tiles, per_xcd, sc, row_major, col0);
   return hipGetLastError();
 }
 
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s, int row_major) {
-  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s, row_major);
+                            Scalars* sc, hipStream_t s, int row_major, int col0) {
+  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s, row_major, col0);
 }
 
 hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s, int row_major) {
-  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s, row_major);
+                            Scalars* sc, hipStream_t s, int row_major, int col0) {
+  return launch_gather_pr_t(pref_m, rank_w, pr, n, K, sc, s, row_major, col0);
 }
 
 hipError_t launch_export_state(Scalars* sc, const unsigned long long* wstats, int waves,
@@ -1815,18 +1829,18 @@ hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scala
 }
 
 hipError_t launch_window(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank16, const int32_t* rank32,
-                         int n, const int4* req, int count, int span, int E, uint32_t* out, Scalars* sc,
-                         unsigned* h_flag, unsigned seq, hipStream_t s) {
-  if (count <= 0 || span <= 0 || span > E) return hipErrorInvalidValue;
+                         int n, const int4* req, int count, int total, uint32_t* out, Scalars* sc, unsigned* h_flag,
+                         unsigned seq, hipStream_t s) {
+  if (count <= 0 || total <= 0) return hipErrorInvalidValue;
   if (rank32) {
-    hipLaunchKernelGGL(k_window_gather<int32_t>, dim3((count * span + 255) / 256), dim3(256), 0, s, pref_m, rank32, n,
-                       req, count, span, E, out, sc, h_flag, seq);
+    hipLaunchKernelGGL(k_window_gather<int32_t>, dim3((total + 255) / 256), dim3(256), 0, s, pref_m, rank32, n, req,
+                       count, total, out, sc, h_flag, seq);
   } else if (rank16) {
-    hipLaunchKernelGGL(k_window_gather<uint16_t>, dim3((count * span + 255) / 256), dim3(256), 0, s, pref_m, rank16,
-                       n, req, count, span, E, out, sc, h_flag, seq);
+    hipLaunchKernelGGL(k_window_gather<uint16_t>, dim3((total + 255) / 256), dim3(256), 0, s, pref_m, rank16, n, req,
+                       count, total, out, sc, h_flag, seq);
   } else {
-    hipLaunchKernelGGL(k_window_search, dim3((count * span + 3) / 4), dim3(1024), 0, s, pref_m, pref_w, n, req, count,
-                       span, E, out, sc, h_flag, seq);
+    hipLaunchKernelGGL(k_window_search, dim3((total + 3) / 4), dim3(1024), 0, s, pref_m, pref_w, n, req, count, total,
+                       out, sc, h_flag, seq);
   }
   return hipGetLastError();
 }

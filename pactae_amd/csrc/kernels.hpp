@@ -81,12 +81,14 @@ hipError_t launch_first_match(const int32_t* pref_m, int n, int32_t* match, hipS
 hipError_t launch_invert_rows(const int32_t* pref, uint16_t* rank, int n, int chunk, int validate,
                               Scalars* sc, hipStream_t s, int nontemporal);
 hipError_t set_invert_lds_limit(int bytes);
-// row_major = 1: out[m * K + k] instead of the column-blocked pr_index layout.
+// row_major = 1: out[m * K + k] instead of the column-blocked pr_index layout,
+// for list positions col0 + k (a band of K columns from col0; col0 = 0 unless
+// row_major).
 hipError_t launch_gather_pr(const int32_t* pref_m, const uint16_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s, int row_major = 0);
+                            Scalars* sc, hipStream_t s, int row_major = 0, int col0 = 0);
 // The same from an int32 rank matrix given as input.
 hipError_t launch_gather_pr(const int32_t* pref_m, const int32_t* rank_w, uint32_t* pr, int n, int K,
-                            Scalars* sc, hipStream_t s, int row_major = 0);
+                            Scalars* sc, hipStream_t s, int row_major = 0, int col0 = 0);
 // Sparse prefix pass: buckets of (m, k < K) per woman (bins of 256 women with
 // bin_cap entries each and counters kBinCounterStride ints apart in bin_cnt,
 // then per-woman buckets of cap entries and their counts in cnt; bin_cnt and
@@ -111,15 +113,16 @@ hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int 
                               const int* cnt, const uint32_t* bucket, int cap, uint16_t* brank, uint32_t* pr,
                               Scalars* sc, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
-// Extension windows for the host tail: req[i].w (<= span <= E) list entries
-// (woman, rank) from position req[i].y of man req[i].x into out[req[i].z * E ..]
-// (pinned host memory, device address), kWinInvalid past the list's end.  Ranks
-// from rank32 or rank16 when given, else by searching the woman's row (pref_w).
-// The kernel then stores seq into *h_flag (pinned) once every entry is visible
-// to the host.
+// Extension windows for the host tail: req[i].w list entries (woman, rank) from
+// position req[i].y of man req[i].x into out[req[i].z ..] (pinned host memory,
+// device address), kWinInvalid past the list's end.  The requests' entries are
+// back to back: req[0].z = 0, req[i + 1].z = req[i].z + req[i].w, and total is
+// the sum of the lengths.  Ranks from rank32 or rank16 when given, else by
+// searching the woman's row (pref_w).  The kernel then stores seq into *h_flag
+// (pinned) once every entry is visible to the host.
 hipError_t launch_window(const int32_t* pref_m, const int32_t* pref_w, const uint16_t* rank16, const int32_t* rank32,
-                         int n, const int4* req, int count, int span, int E, uint32_t* out, Scalars* sc,
-                         unsigned* h_flag, unsigned seq, hipStream_t s);
+                         int n, const int4* req, int count, int total, uint32_t* out, Scalars* sc, unsigned* h_flag,
+                         unsigned seq, hipStream_t s);
 hipError_t launch_fold_wstats(const unsigned long long* wstats, int waves, Scalars* sc, hipStream_t s);
 // Folds the proposal waves' counters, then writes the control block, the partner
 // words and the parked chains into host memory (device-visible pinned pointers)

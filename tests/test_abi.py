@@ -65,7 +65,8 @@ def test_unknown_options_are_rejected(smp):
     n = 4
     pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=1)
     obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
-    for bad in (dict(init_mode=3), dict(init_mode=-1), dict(schedule=6), dict(schedule=-1)):
+    for bad in (dict(init_mode=3), dict(init_mode=-1), dict(schedule=6), dict(schedule=-1),
+                dict(tail_window_entries=-1)):
         with pytest.raises(smp.SmpError) as ei:
             smp.SmpEngineBamboo(obj, n, **bad)
         assert ei.value.status == 1, bad

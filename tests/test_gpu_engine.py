@@ -387,18 +387,42 @@ def test_tail_extension_windows_20000(gpu, oracle):
         assert prof["rank_matrix"] == 0, cfg
 
 
-def test_tail_windows_run_out_falls_back(gpu, oracle):
-    """More stopped chains than extension windows (a 1-column prefix, 5,000 men
-    handed to the host, 4,096 windows): the chains go back to the GPU, which
-    resumes them with the full rank matrix.  Bit-exact either way."""
+@pytest.mark.parametrize("entries", [4096, 0])
+def test_tail_windows_run_out_falls_back(gpu, oracle, entries):
+    """A 1-column prefix with 5,000 men handed to the host: every chain leaves the
+    prefix at once.  With window storage capped at 4,096 entries (64 per first
+    window) the chains go back to the GPU, which resumes them with the full rank
+    matrix; with the automatic storage the host finishes them from windows.
+    Bit-exact either way."""
     smp = gpu
     n = 6000
     pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=77)
     want, t = oracle.gs_seq(n, pm, pw)
-    got, prof = solve(smp, n, pm, pw, prefix_cols=1, tail_mode=2, tail_threshold=5000, init_mode=2)
+    got, prof = solve(smp, n, pm, pw, prefix_cols=1, tail_mode=2, tail_threshold=5000, init_mode=2,
+                      tail_window_entries=entries)
     assert np.array_equal(got, want)
     assert prof["proposals"] == t["proposals"]
-    assert prof["bailed"] > 0 and prof["rank_matrix"] == 1
+    if entries:
+        assert prof["bailed"] > 0 and prof["rank_matrix"] == 1
+    else:
+        assert prof["bailed"] == 0 and prof["tail_windows"] >= 5000 and prof["tail_rounds"] > 0
+
+
+def test_tail_band_correlated_20000(gpu, oracle):
+    """Correlated lists, n = 20,000, seed 0x5EED0004: 594 men end past position 256
+    (the deepest at 441), and the host tail's chains reach them one at a time.
+    After a few rounds of windows the host takes a band of every man's list
+    beyond the prefix and finishes from it.  Bit-exact against the oracle."""
+    smp = gpu
+    n = 20000
+    pm, pw = smp.generate(smp.Workload.CORRELATED, n, seed=0x5EED0004)
+    want, t = oracle.gs_seq(n, pm, pw)
+    pos = np.argmax(pm.reshape(n, n) == want[:, None], axis=1)
+    assert int((pos >= 256).sum()) > 500
+    got, prof = solve(smp, n, pm, pw, tail_mode=2, tail_threshold=64)
+    assert np.array_equal(got, want)
+    assert prof["proposals"] == t["proposals"]
+    assert prof["tail_bands"] >= 1 and prof["tail_full_matrix"] == 0 and prof["bailed"] == 0, prof
 
 
 @pytest.mark.parametrize("kind,param", [(4, 5), (7, 0)])
