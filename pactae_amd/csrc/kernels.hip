@@ -10,8 +10,11 @@
 //   k_post                            PostProc               src/smp/smp_engine_bamboo.cu:299-303
 //   k_master_fused .. k_post_master   master-list path (every man holds the same list):
 //                                     detection, women-proposing walk, matching
-//   k_mw_cas / k_post_cas             MW-Par-GPU-CAS baseline schedule
-//                                     src/smp/smp_engine_mw_3_f9.cu:37-97
+//   k_walk_thread / k_post_cas        the reference's thread-per-man schedules:
+//                                     MW-Par-GPU-CAS src/smp/smp_engine_mw_3_f9.cu:37-97,
+//                                     src/smp/kernel.cu:5-55, 122-165, 468-512, 515-552
+//   k_frontier_init, k_frontier       frontier schedule (rounds, LDS pre-reduction)
+//   k_window_search, k_window_gather  the host tail's extension windows
 //   k_bin_prefix, k_bin_sort,         sparse prefix pass: F9RankMatrixInitKernel and
 //   k_invert_rows<.., SP=1>           F9PRMatrixInitKernel restricted to the first K list
 //                                     positions, without writing the n x n rank matrix
