@@ -410,7 +410,11 @@ typedef struct smp_server_opts {
   int32_t max_batch;        /* requests per launch; 0 = 64                           */
   int32_t max_n;            /* largest request accepted; 0 = 8192 (int16 ranks)      */
   int32_t block_threads;    /* workgroup size of the serving kernel; 0 = auto        */
-  int64_t hbm_budget_bytes; /* device arena for request state; 0 = 4 GiB             */
+  int64_t hbm_budget_bytes; /* device arena for request state (the lanes' int16 rank
+                               matrices and the list copies of host-submitted
+                               requests); 0 = 1 GiB, room for two host-submitted
+                               n = 8192 requests.  Allocated when the server is
+                               created and freed when it is destroyed            */
   int32_t large_n;          /* batched mode: requests with n >= large_n are solved by
                                the multi-workgroup single-instance pipeline on
                                workers of their own, beside the lanes;

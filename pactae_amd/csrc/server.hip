@@ -184,10 +184,11 @@ struct smp_server {
 
   bool is_large(const Request& r) const { return opts.mode == SMP_SERVE_BATCHED && r.n >= opts.large_n; }
 
-  // Arena bytes of the int16 rank matrix (lane requests only: the large-request
-  // pipeline has its own) and of the list copies (host-submitted requests).
+  // Arena bytes of the int16 rank matrix (lane requests only: the
+  // single-instance pipelines of the large-request workers and of plain mode
+  // have their own) and of the list copies (host-submitted requests).
   size_t rank_bytes(const Request& r) const {
-    return is_large(r) ? 0 : (((size_t)r.n * r.n * 2 + 255) & ~size_t(255));
+    return opts.mode != SMP_SERVE_BATCHED || is_large(r) ? 0 : (((size_t)r.n * r.n * 2 + 255) & ~size_t(255));
   }
   size_t footprint(const Request& r) const {
     size_t b = rank_bytes(r);
@@ -580,7 +581,7 @@ void smp_server_default_opts(smp_server_opts* o) {
   o->max_batch = 64;
   o->max_n = 8192;
   o->block_threads = 0;
-  o->hbm_budget_bytes = 4ll << 30;
+  o->hbm_budget_bytes = 1ll << 30;
   o->large_n = 512;
   o->large_workers = 3;
   o->large_spill_us = 0;
@@ -598,7 +599,7 @@ int smp_server_create(const smp_server_opts* opts, smp_server** out) {
   if (o.lanes <= 0) o.lanes = 3;
   if (o.max_batch <= 0) o.max_batch = 64;
   if (o.max_n <= 0) o.max_n = 8192;
-  if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = 4ll << 30;
+  if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = 1ll << 30;
   if (o.block_threads != 256 && o.block_threads != 512 && o.block_threads != 1024) o.block_threads = 1024;
   if (o.large_n <= 0) o.large_n = 512;
   if (o.large_workers <= 0) o.large_workers = 3;
