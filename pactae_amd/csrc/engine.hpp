@@ -196,11 +196,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // a row.
   static constexpr int kWinFirst = 64;
   static constexpr int kWinMax = 4096;
-  // From the second round of windows on (or from the first when a rank matrix
-  // exists), each round also gives a window to every man matched at most
-  // kWinAhead positions before the end of the entries the host holds for him,
-  // so that a chain that displaces him continues without another round trip.
-  // deep_w collects the women whose partner may be such a man.
+  // When a rank matrix exists (a window entry is then one gather), each round
+  // also gives a window to every man matched at most kWinAhead positions before
+  // the end of the entries the host holds for him, so that a chain that
+  // displaces him continues without another round trip.  deep_w collects the
+  // women whose partner may be such a man.
   static constexpr int kWinAhead = 8;
   size_t window_cap() const {
     return opts.tail_window_entries > 0 ? std::min(win_cap, (size_t)opts.tail_window_entries) : win_cap;
@@ -1252,10 +1252,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
     int2* bail = h_parked;  // stopped chains overwrite the consumed parked list (bailed <= taken)
     int cm[kTailSlots], ck[kTailSlots];
     // Walks the `parked` chains of h_parked; the chains that leave the entries
-    // the host holds are collected in bail[0..bailed).  FULL: the band is the
-    // whole matrix (the loop then has one source of entries).
-    auto walk_impl = [&](auto full_tag) -> int {
-      constexpr bool FULL = decltype(full_tag)::value;
+    // the host holds are collected in bail[0..bailed).  MODE 0: the prefix and
+    // windows only; 1: a band beside them; 2: the band is the whole matrix (the
+    // loop then has one source of entries).
+    auto walk_impl = [&](auto mode_tag) -> int {
+      constexpr int MODE = decltype(mode_tag)::value;
+      constexpr bool FULL = MODE == 2, BAND = MODE == 1;
       int act = 0, taken = 0;
       bailed = 0;
       // What the host holds does not change during a walk: keep it in locals.
@@ -1281,7 +1283,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
           // independent loads, and the first position where the man beats the
           // woman's partner is found from a bit mask.  Positions before it are
           // refusals; the candidate itself goes through the step below.
-          const bool in_band = FULL ? k + 8 <= n : k >= b0 && k + 8 <= b1;
+          const bool in_band = FULL ? k + 8 <= n : BAND && k >= b0 && k + 8 <= b1;
           if (in_band || (!FULL && k + 8 <= kc && (k & 31) <= 24 && (k >> 5) < blocks_ready)) {
             const uint32_t* p8 = FULL      ? bd + (size_t)m * n + k
                                  : in_band ? bd + (size_t)m * bs + (k - b0)
@@ -1309,7 +1311,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
           if (FULL) {
             if (k >= n) return tail_error("host tail: a man exhausted his list");
             e = bd[(size_t)m * n + k];
-          } else if (k >= b0 && k < b1) {
+          } else if (BAND && k >= b0 && k < b1) {
             e = bd[(size_t)m * bs + (k - b0)];
           } else if (k < kc) {
             if ((k >> 5) >= blocks_ready) {  // a block the warm-up did not reach
@@ -1343,7 +1345,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
             m = (int)((v >> 16) & 0xFFFFu);
             k = (int)(v >> 32) + 1;
             if (FULL) __builtin_prefetch(bd + (size_t)m * n + k);
-            else if (k >= b0 && k < b1) __builtin_prefetch(bd + (size_t)m * bs + (k - b0));
+            else if (BAND && k >= b0 && k < b1) __builtin_prefetch(bd + (size_t)m * bs + (k - b0));
             else if (k < kc) __builtin_prefetch(pr + pactae::pr_index(m, k, n));
             break;
           }
@@ -1364,7 +1366,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
       return SMP_OK;
     };
     auto walk = [&]() -> int {
-      return kb0 == 0 && kb1 == n ? walk_impl(std::true_type{}) : walk_impl(std::false_type{});
+      if (kb1 == 0) return walk_impl(std::integral_constant<int, 0>{});
+      if (kb0 == 0 && kb1 == n) return walk_impl(std::integral_constant<int, 2>{});
+      return walk_impl(std::integral_constant<int, 1>{});
     };
     int rounds = 0, band_round = 0, max_stop = 0;
     long long searched = 0;  // window entries found by searching rows
@@ -1423,8 +1427,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
       }
       // Men who may soon leave the entries the host holds for them, while the
       // storage lasts (each man is matched to one woman, and a stopped chain's
-      // man to none, so no man is asked for twice).
-      if (rounds > 0 || have_rank) {
+      // man to none, so no man is asked for twice); only with a rank matrix,
+      // where a window entry is one gather rather than a search of a row.
+      if (have_rank) {
         ++deep_round;
         for (int w : deep_w) {
           if (deep_stamp[w] == deep_round) continue;
