@@ -119,6 +119,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int K = 0;  // PR prefix columns
   int T = 0;  // hand-off threshold (0 = GPU finishes)
   int K_alloc = 0;               // PR columns the buffers hold
+  size_t pr_cap = 0;             // entries d_pr holds (grown for the whole matrix)
+  // Lists whose chains walked far past the prefix on the GPU take the whole PR
+  // matrix from their next solve on (see deep_lists()).
+  bool full_pr_next = false;
   int tuned_K = 0, tuned_T = -1;  // measured choice for AUTO mode (smp_tune)
   bool owns_lists = false;
   bool have_result = false;
@@ -356,7 +360,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     for (auto& e : ev) SMP_HIP_TRY(hipEventCreate(&e));
     for (auto& e : ev_blk) SMP_HIP_TRY(hipEventCreate(&e));
     if (hipMalloc(&d_rank_w, nn * sizeof(uint16_t)) != hipSuccess ||
-        hipMalloc(&d_pr, pactae::pr_entries(n, K_alloc) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_pr, (pr_cap = pactae::pr_entries(n, K_alloc)) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&d_partner, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&d_match, (size_t)n * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&d_seen, (size_t)n * sizeof(int)) != hipSuccess ||
@@ -528,6 +532,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   void configure(int ni) {
     n = ni;
     K = prefix_for(ni);
+    full_pr_next = false;
     // Default from the measured sweep at n = 20,000 (uniform): the host takes
     // over at 64 free men; smp_tune replaces this with a measurement.
     if (opts.tail_mode == SMP_TAIL_GPU)
@@ -758,6 +763,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     span_bucket = span_rank = span_gather = span_propose = Span{};
     have_result = false;
     master = false;
+    if (full_pr_next) {
+      full_pr_next = false;
+      if (grow_pr(n)) K = n;
+    }
     matrices_valid = false;
     sparse = false;
     rank_built = false;
@@ -1546,6 +1555,37 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return SMP_OK;
   }
 
+  // With the automatic width, lists on which the proposal kernel made more
+  // than n^2 / 8 proposals (men walking on average more than n / 8 positions:
+  // lists that share their top women, such as the reference's RANDOM with
+  // groups of 5) take the whole PR matrix, the reference's layout, from their
+  // next solve on.  Past the prefix each step of a chain otherwise gathers a
+  // rank from the rank matrix after loading the woman from the list, two
+  // dependent loads instead of one; the gather of the whole matrix (4 n^2
+  // bytes written) is one parallel pass.  The reference's RANDOM, n = 20,000:
+  // 108 -> 83 ms per solve; n = 4,000: 9.9 -> 7.8 ms.  Up to n = 32,768 (4 GiB
+  // of PR matrix).
+  bool deep_lists() const {
+    if (opts.prefix_cols > 0 || opts.schedule != SMP_SCHEDULE_BAMBOO || master || K >= n || n > 32768)
+      return false;
+    const long long gpu_props = (long long)times.proposals - (long long)times.host_proposals;
+    return gpu_props * 8 > (long long)n * n;
+  }
+  // d_pr room for `cols` columns; false (the old buffer kept) if it cannot be had.
+  bool grow_pr(int cols) {
+    const size_t need = pactae::pr_entries(n_alloc, cols);
+    if (need <= pr_cap) return true;
+    uint32_t* p = nullptr;
+    if (hipMalloc(&p, need * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    (void)hipFree(d_pr);
+    d_pr = p;
+    pr_cap = need;
+    return true;
+  }
+
   // The prefix copies must run beside the proposal kernel.  Streams share the
   // device's few hardware queues; when a copy stream lands on the compute
   // stream's queue, its copies wait for the kernel instead.  Seen from the
@@ -1644,6 +1684,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     times.total_ms = precheck_time_ + init_time_ + core_time_ + post_time_;
     times.prefix_cols = K;
     times.tail_threshold = T;
+    full_pr_next = deep_lists();
     if (!times.is_perfect && T > 0 && blocks_copied > 0 && opts.prefix_copy == 0) check_copy_overlap();
     times.copy_requeues = requeues;
     take_span(span_bucket, &times.bucket_kernel_ms);
