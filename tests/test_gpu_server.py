@@ -72,13 +72,29 @@ def test_large_request_worker(gpu, oracle, large_n):
 
 
 def test_plain_mode_matches(gpu, oracle):
+    """Plain mode, host-submitted and device-resident requests.  Its pipeline
+    has buffers of its own, so the arena holds only the host requests' list
+    copies (no rank block): device-resident requests take no arena room."""
     smp = gpu
     reqs = requests(smp, 12, seed=9)
     with smp.SmpServer(mode=smp.SmpServer.PLAIN) as srv:
         tickets = [srv.submit(pm, pw, n) for n, (pm, pw) in reqs]
         got = [srv.wait(t, timeout_ms=60000)[0] for t in tickets]
+        peak_host = srv.stats()["peak_arena_bytes"]
     for (n, (pm, pw)), g in zip(reqs, got):
         assert np.array_equal(g, oracle.gs_seq(n, pm, pw)[0])
+    assert peak_host <= sum(2 * (((n * n * 4) + 255) // 256 * 256) for n, _ in reqs)
+    dev = [(n, smp.DeviceArray(pm), smp.DeviceArray(pw)) for n, (pm, pw) in reqs]
+    with smp.SmpServer(mode=smp.SmpServer.PLAIN) as srv:
+        tickets = [srv.submit(dm, dw, n) for n, dm, dw in dev]
+        got = [srv.wait(t, timeout_ms=60000)[0] for t in tickets]
+        st = srv.stats()
+    for (n, (pm, pw)), g in zip(reqs, got):
+        assert np.array_equal(g, oracle.gs_seq(n, pm, pw)[0])
+    assert st["peak_arena_bytes"] <= 256 * len(reqs), st["peak_arena_bytes"]
+    for _, dm, dw in dev:
+        dm.free()
+        dw.free()
 
 
 def test_invalid_request_fails_alone(gpu, oracle):
