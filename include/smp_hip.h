@@ -230,7 +230,9 @@ typedef struct smp_times {
   int32_t tail_bands;      /* bands of columns of every man's list (the whole
                               matrix included) the host tail took because chains
                               kept leaving what it held                           */
-  int32_t reserved[1];
+  int32_t deep_restart;    /* 1 if the proposal kernel's chains walked so far past
+                              the prefix that every chain was parked, the whole
+                              PR matrix gathered and the chains resumed over it   */
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;

@@ -88,7 +88,7 @@ class SmpTimes(ctypes.Structure):
         ("tail_full_matrix", i32),
         ("tail_rounds", i32),
         ("tail_bands", i32),
-        ("reserved", i32 * 1),
+        ("deep_restart", i32),
     ]
 
     def as_dict(self) -> dict:

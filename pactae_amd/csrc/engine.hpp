@@ -123,6 +123,12 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // Lists whose chains walked far past the prefix on the GPU take the whole PR
   // matrix from their next solve on (see deep_lists()).
   bool full_pr_next = false;
+  // A proposal launch that resumes parked chains (a restart over the whole PR
+  // matrix) instead of starting every man, and the deep-step limit of the
+  // launch (0 = off).
+  const int2* resume_starts = nullptr;
+  int resume_n = 0;
+  long long deep_limit = 0;
   int tuned_K = 0, tuned_T = -1;  // measured choice for AUTO mode (smp_tune)
   bool owns_lists = false;
   bool have_result = false;
@@ -962,6 +968,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.starts = starts;
     a.nstarts = nstarts;
     a.rank_sparse = rank_built || d_rank_in ? 0 : 1;
+    a.deep_limit = deep_limit;
     return a;
   }
 
@@ -1010,8 +1017,43 @@ struct smp_engine : public pactae::AbsSmpEngine {
       times.handoff_free = 0;
       return SMP_OK;
     }
+    // Deep chains: with the automatic width and K < n, a launch whose waves
+    // make more than n^2 / 128 steps past the prefix parks every chain; the
+    // host then gathers the whole PR matrix and resumes them from the shared
+    // partner words (lists that share their top women, such as the
+    // reference's RANDOM with groups of 5: its first solve at n = 20,000 takes
+    // the whole matrix a few percent into the walk).
+    resume_starts = nullptr;
+    resume_n = 0;
+    deep_limit = opts.prefix_cols <= 0 && opts.schedule == SMP_SCHEDULE_BAMBOO && K < n && n <= 32768
+                     ? std::max<long long>(1, (long long)n * n / 128)
+                     : 0;
     for (int run = 0;; ++run) {
       if (int st = propose_phase(grid)) return st;
+      if (h_sc->restart && resume_starts == nullptr) {
+        const int parked = h_sc->parked_count;
+        times.deep_restart = 1;
+        deep_limit = 0;
+        SMP_HIP_TRY(hipMemcpyAsync(d_starts, d_parked, (size_t)parked * sizeof(int2), hipMemcpyDeviceToDevice,
+                                   stream));
+        if (grow_pr(n)) K = n;
+        if (!d_rank_in) {
+          if (int rc = ensure_rank_matrix()) return rc;
+        }
+        SMP_HIP_TRY(hipEventRecord(ev[1], stream));
+        if (d_rank_in)
+          SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_in, d_pr, n, K, d_sc, stream));
+        else
+          SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
+        span_gather = {&ev[1], &ev[2]};
+        SMP_HIP_TRY(pactae::launch_fill_i32(&d_sc->parked_count, 1, 0, stream));
+        SMP_HIP_TRY(pactae::launch_fill_i32(&d_sc->restart, 1, 0, stream));
+        SMP_HIP_TRY(pactae::launch_fill_i32(&d_sc->active, 1, std::min(parked, grid * 4), stream));
+        if (int st = end_init()) return st;
+        resume_starts = d_starts;
+        resume_n = parked;
+        if (parked > 0) continue;
+      }
       if (run == 0 && sparse && !rank_built && h_sc->prefix_overflow != 0) {
         // A bucket overflowed: the sparse pass wrote the rank matrix instead of
         // the prefix and the proposal kernel did nothing.  Gather the prefix
@@ -1049,7 +1091,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // the control block (and, with a hand-off, the partner words and parked
   // chains).  ev[5] marks the end of the phase on the device.
   int propose_phase(int grid) {
-    SMP_HIP_TRY(pactae::launch_propose(propose_args(T, nullptr, 0), grid, stream));
+    SMP_HIP_TRY(pactae::launch_propose(propose_args(T, resume_starts, resume_n), grid, stream));
     span_propose = {&ev[2], &ev[5]};
     state_copied = false;
     if (T == 0) {

@@ -63,6 +63,8 @@ __global__ void k_reset_scalars(Scalars* sc, int active, int* z0, size_t n0, int
     sc->first_differs = 0;
     sc->not_master = 0;
     sc->export_blocks = 0;
+    sc->restart = 0;
+    sc->deep_steps = 0;
     sc->atomics = 0;
     sc->steps = 0;
     sc->proposals = 0;
@@ -622,6 +624,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   int spec_m = -1, spec_k = -1;
   uint32_t spec_e = 0;
   unsigned long long n_atomics = 0, n_steps = 0;
+  unsigned long long deep_local = 0;  // steps past the prefix not yet flushed
   int since_poll = 0;
   // Steps between polls: a.poll while many waves are busy, then every step once
   // the count is near the threshold, so that the last waves see the hand-off
@@ -669,6 +672,36 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     if (poll && lane == 0)
       act = __hip_atomic_load(&a.sc->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ++n_steps;
+    // Deep chains: every 256 steps a wave adds its steps past the prefix to the
+    // shared count; past deep_limit the first wave to see it sets `restart`,
+    // and every wave that sees the flag parks its chain and its unstarted men
+    // (as for the hand-off) so that the host gathers the whole PR matrix and
+    // resumes them.
+    bool restart = false;
+    if (a.deep_limit > 0) {
+      deep_local += (unsigned long long)(k >= K);
+      if ((n_steps & 255) == 0) {
+        int rs = 0;
+        if (lane == 0) {
+          if (deep_local) {
+            const unsigned long long tot = atomicAdd(&a.sc->deep_steps, deep_local) + deep_local;
+            if (tot > (unsigned long long)a.deep_limit) atomicOr(&a.sc->restart, 1);
+          }
+          rs = __hip_atomic_load(&a.sc->restart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        deep_local = 0;
+        restart = __shfl(rs, 0) != 0;
+      }
+    }
+    if (restart) {
+      if (lane == 0) {
+        a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(m, k);
+        for (; next_init < total; next_init += waves)
+          a.parked[atomicAdd(&a.sc->parked_count, 1)] =
+              a.starts ? a.starts[next_init] : make_int2(next_init, 0);
+      }
+      break;
+    }
     const int kk = k + lane;
     bool valid = kk < n && (have_rank || kk < K);
     uint32_t w = 0, r = 0;

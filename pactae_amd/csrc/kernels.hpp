@@ -25,6 +25,10 @@ struct ProposeArgs {
                                //   a chain reaching position K is parked at K for the
                                //   host, which builds rank_w.  If the pass overflowed
                                //   (sc->prefix_overflow), the kernel does nothing.
+  long long deep_limit;        // > 0: once the waves have made this many steps past
+                               //   position K, every chain is parked and sc->restart
+                               //   set (the host then resumes them over the whole PR
+                               //   matrix); 0 = off
 };
 
 // Control block of the frontier schedule (device memory).

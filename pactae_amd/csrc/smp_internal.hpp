@@ -74,6 +74,11 @@ struct Scalars {
   int first_differs; // precheck: 1 if two men have different first choices
   int not_master;    // precheck: 1 if some man's list differs from man 0's
   int export_blocks; // k_export_state: blocks finished (the last one resets it)
+  int restart;       // proposal kernel: deep chains seen, every chain parked for a
+                     // restart over the whole PR matrix
+  int pad0;
+  unsigned long long deep_steps; // proposal kernel: steps past the PR prefix so far
+                                 // (flushed by the waves every 256 steps)
   unsigned long long atomics;    // atomicMin attempts
   unsigned long long steps;      // segment scans
   unsigned long long proposals;  // sum over women of (partner's list position + 1)

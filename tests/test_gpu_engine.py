@@ -443,31 +443,35 @@ def test_fig9_workloads_4000(gpu, oracle, kind, param):
         assert prof["tail_full_matrix"] == 1 and prof["bailed"] == 0
 
 
-def test_deep_lists_take_the_whole_pr_matrix(gpu, oracle):
-    """Lists on which the proposal kernel makes more than n^2 / 8 proposals (the
-    reference's RANDOM with groups of 5, about n^2 / 2) take the whole PR matrix
-    from their next solve on; uniform lists keep the automatic prefix.  Every
-    solve bit-exact, and new lists start again from the automatic width."""
+@pytest.mark.parametrize("tail", [dict(), dict(tail_mode=1)])
+def test_deep_lists_take_the_whole_pr_matrix(gpu, oracle, tail):
+    """The reference's RANDOM with groups of 5 (about n^2 / 2 proposals): the
+    first solve's chains walk far past the prefix, so the proposal kernel parks
+    them, the whole PR matrix is gathered and they resume over it; later solves
+    of the same lists start with the whole matrix.  Uniform and correlated lists
+    keep the automatic prefix.  Every solve bit-exact, with and without the
+    host tail."""
     smp = gpu
     n = 4000
     pm, pw = smp.generate(smp.Workload.REF_RANDOM, n, seed=0x5EED0009, param=5)
     want, t = oracle.gs_seq(n, pm, pw)
     dm, dw = smp.DeviceArray(pm), smp.DeviceArray(pw)
-    with smp.SmpEngineBamboo.from_device(dm, dw, n) as e:
-        widths = []
+    with smp.SmpEngineBamboo.from_device(dm, dw, n, **tail) as e:
+        seen = []
         for _ in range(3):
             e.exec()
             p = e.profile()
             assert np.array_equal(e.result(), want)
             assert p["proposals"] == t["proposals"]
-            widths.append(p["prefix_cols"])
-        assert widths[0] < n and widths[1] == n and widths[2] == n, widths
-    pmu, pwu = smp.generate(smp.Workload.UNIFORM, n, seed=3)
-    wantu, _ = oracle.gs_seq(n, pmu, pwu)
-    got, prof = solve(smp, n, pmu, pwu)
-    assert np.array_equal(got, wantu) and prof["prefix_cols"] < n
+            seen.append((p["prefix_cols"], p["deep_restart"]))
+        assert seen == [(n, 1), (n, 0), (n, 0)], seen
     dm.free()
     dw.free()
+    for kind in (smp.Workload.UNIFORM, smp.Workload.CORRELATED):
+        pmu, pwu = smp.generate(kind, n, seed=3)
+        wantu, _ = oracle.gs_seq(n, pmu, pwu)
+        got, prof = solve(smp, n, pmu, pwu, **tail)
+        assert np.array_equal(got, wantu) and prof["prefix_cols"] < n and prof["deep_restart"] == 0, kind
 
 
 def test_config3_congested_and_shifted_20000(gpu, oracle):
