@@ -1614,12 +1614,20 @@ struct smp_engine : public pactae::AbsSmpEngine {
     return gpu_props * 8 > (long long)n * n;
   }
   // d_pr room for `cols` columns; false (the old buffer kept) if it cannot be had.
+  // The old buffer is freed only after the engine's streams are idle: prefix
+  // copies the host tail did not wait for may still read it.
   bool grow_pr(int cols) {
     const size_t need = pactae::pr_entries(n_alloc, cols);
     if (need <= pr_cap) return true;
     uint32_t* p = nullptr;
     if (hipMalloc(&p, need * sizeof(uint32_t)) != hipSuccess) {
       (void)hipGetLastError();
+      return false;
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess || hipStreamSynchronize(copy_stream) != hipSuccess ||
+        hipStreamSynchronize(copy_stream2) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(p);
       return false;
     }
     (void)hipFree(d_pr);
