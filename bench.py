@@ -253,8 +253,9 @@ def cpu_baseline_single(args, n, pm, pw, gpu_match, inst):
     ok = True
     table = {}
     t_cpu = time.perf_counter()
-    for variant, rk in (("ref", None), ("tuned", None), ("ref", rank)):
+    for variant, rk in (("ref", None), ("tuned", None), ("cxx", None), ("ref", rank)):
         key = {("ref", False): "reference_flags", ("tuned", False): "march_x86_64_v3",
+               ("cxx", False): "cxx_release_flags",
                ("ref", True): "reference_flags_ranks_given"}[(variant, rk is not None)]
         table[key] = {}
         for name, (fn, par) in engines.items():
@@ -281,8 +282,12 @@ def cpu_baseline_single(args, n, pm, pw, gpu_match, inst):
     return {"value": ref[best]["median_ms"], "unit": "ms/solve", "cores": threads if best.endswith("Par") else 1,
             "kind": "port", "engine": best, "engines": table, "threads_par": threads,
             "cpu_model": cpu_model(), "host_cores": host_cores(),
-            "flags": {"reference_flags": "-O3 -DNDEBUG -fno-strict-aliasing -fwrapv (the reference's host flags)",
-                      "march_x86_64_v3": "-O3 -DNDEBUG -march=x86-64-v3"},
+            "flags": {"reference_flags": "-O3 -DNDEBUG -fno-strict-aliasing -fwrapv (the host flags of the "
+                                         "reference's .cu CPU engines)",
+                      "march_x86_64_v3": "-O3 -DNDEBUG -march=x86-64-v3",
+                      "cxx_release_flags": "-O3 -DNDEBUG -fno-tree-vectorize -fno-tree-slp-vectorize "
+                                           "-fno-aggressive-loop-optimizations -fno-unroll-loops (the reference's "
+                                           "C++ Release flags, g++ spelling)"},
             "note": "GS-Par and MW-Par time their single-threaded n^2 rank inversion in init, as the "
                     "reference does, so with the lists as input they are no faster than the sequential engines",
             "sample": f"the same n={n} instance (the tuning seed), 1 warm-up + {runs} timed solves per engine and "

@@ -43,12 +43,12 @@ class OracleTimes(ctypes.Structure):
 
 
 def build() -> None:
-    """Compile both oracle variants (g++ only; no GPU needed)."""
+    """Compile the oracle variants (g++ only; no GPU needed)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
 def _lib(variant: str = "ref") -> ctypes.CDLL:
-    name = {"ref": "liboracle.so", "tuned": "liboracle_tuned.so"}[variant]
+    name = {"ref": "liboracle.so", "tuned": "liboracle_tuned.so", "cxx": "liboracle_cxx.so"}[variant]
     if variant in _LIBS:
         return _LIBS[variant]
     path = os.path.join(_HERE, "_build", name)

@@ -215,8 +215,11 @@ def test_engines_agree_and_stable(oracle, smp, kind, n):
         assert np.array_equal(m, ref), name
     assert oracle.count_blocking_pairs(n, pm, pw, ref) == 0
     assert t_gs["proposals"] == t_mw["proposals"]
-    tuned, _ = oracle.gs_seq(n, pm, pw, variant="tuned")
-    assert np.array_equal(tuned, ref)
+    for variant in ("tuned", "cxx"):  # the other two builds of the same source
+        other, _ = oracle.gs_seq(n, pm, pw, variant=variant)
+        assert np.array_equal(other, ref), variant
+        other, _ = oracle.mw_par(n, pm, pw, variant=variant, threads=4)
+        assert np.array_equal(other, ref), variant
 
 
 # ------------------------------------------------------------ planted mistakes
