@@ -135,7 +135,13 @@ typedef struct smp_opts {
                              list entries: 0 = auto (512 n, between 65,536 and
                              4,194,304); a smaller value caps it.  Chains that
                              find no storage left go back to the GPU            */
-  int32_t reserved[2];
+  int32_t start_order;    /* order in which the proposal kernel starts the men:
+                             0 = the order of woman 0's list (default: when the
+                             women's lists share a common order, the men they
+                             prefer propose first and displace fewer men; the
+                             reference's RANDOM lists, n = 20,000: 83 -> 64 ms),
+                             1 = by index.  The matching is the same          */
+  int32_t reserved[1];
 } smp_opts;
 
 enum smp_init_mode {

@@ -49,7 +49,8 @@ class SmpOpts(ctypes.Structure):
         ("schedule", i32),
         ("init_mode", i32),
         ("tail_window_entries", i32),
-        ("reserved", i32 * 2),
+        ("start_order", i32),
+        ("reserved", i32 * 1),
     ]
 
 

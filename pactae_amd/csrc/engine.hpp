@@ -969,6 +969,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
     a.nstarts = nstarts;
     a.rank_sparse = rank_built || d_rank_in ? 0 : 1;
     a.deep_limit = deep_limit;
+    // Men start in the order of woman 0's list (opts.start_order 0): when the
+    // women's lists share a common order (the reference's RANDOM lists share
+    // one in groups of 5), the men the women prefer then propose first and
+    // displace fewer men; for independent lists the order is as good as any.
+    // The row is a permutation (the init validates it), and the matching does
+    // not depend on the order.
+    a.start_order = opts.start_order == 0 && !d_rank_in ? d_pref_w : nullptr;
     return a;
   }
 

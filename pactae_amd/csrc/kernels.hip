@@ -653,7 +653,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         m = st.x;
         k = st.y;
       } else {
-        m = next_init;
+        m = a.start_order ? a.start_order[next_init] : next_init;
         k = 0;
       }
       next_init += waves;
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(m, k);
         for (; next_init < total; next_init += waves)
           a.parked[atomicAdd(&a.sc->parked_count, 1)] =
-              a.starts ? a.starts[next_init] : make_int2(next_init, 0);
+              a.starts ? a.starts[next_init] : make_int2(a.start_order ? a.start_order[next_init] : next_init, 0);
       }
       break;
     }
@@ -779,7 +779,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
           if (m >= 0) a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(m, k);
           for (; next_init < total; next_init += waves)
             a.parked[atomicAdd(&a.sc->parked_count, 1)] =
-                a.starts ? a.starts[next_init] : make_int2(next_init, 0);
+                a.starts ? a.starts[next_init] : make_int2(a.start_order ? a.start_order[next_init] : next_init, 0);
         }
         break;
       }

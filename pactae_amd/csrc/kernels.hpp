@@ -25,6 +25,8 @@ struct ProposeArgs {
                                //   a chain reaching position K is parked at K for the
                                //   host, which builds rank_w.  If the pass overflowed
                                //   (sc->prefix_overflow), the kernel does nothing.
+  const int32_t* start_order;  // not null: work item i is man start_order[i] (a
+                               //   permutation of 0..n-1), else man i
   long long deep_limit;        // > 0: once the waves have made this many steps past
                                //   position K, every chain is parked and sc->restart
                                //   set (the host then resumes them over the whole PR

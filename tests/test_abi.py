@@ -66,11 +66,11 @@ def test_unknown_options_are_rejected(smp):
     pm, pw = smp.generate(smp.Workload.UNIFORM, n, seed=1)
     obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
     for bad in (dict(init_mode=3), dict(init_mode=-1), dict(schedule=6), dict(schedule=-1),
-                dict(tail_window_entries=-1)):
+                dict(tail_window_entries=-1), dict(start_order=2), dict(start_order=-1)):
         with pytest.raises(smp.SmpError) as ei:
             smp.SmpEngineBamboo(obj, n, **bad)
         assert ei.value.status == 1, bad
-        assert "init_mode or schedule" in str(ei.value)
+        assert "init_mode, schedule or start_order" in str(ei.value)
     assert [m.value for m in smp.InitMode] == [0, 1, 2]
 
 

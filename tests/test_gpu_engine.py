@@ -474,6 +474,21 @@ def test_deep_lists_take_the_whole_pr_matrix(gpu, oracle, tail):
         assert np.array_equal(got, wantu) and prof["prefix_cols"] < n and prof["deep_restart"] == 0, kind
 
 
+@pytest.mark.parametrize("kind", [0, 1, 3, 4, 6, 7])
+def test_start_order_either_way(gpu, oracle, kind):
+    """Men started in the order of woman 0's list (the default) or by index:
+    the same matching and proposal count, with and without the host tail."""
+    smp = gpu
+    for n in (64, 700, 2500):
+        pm, pw = smp.generate(kind, n, seed=4100 + n, param=5 if kind == 4 else 0)
+        want, t = oracle.gs_seq(n, pm, pw)
+        for order in (0, 1):
+            for tail in (dict(tail_mode=1), dict(tail_mode=2, tail_threshold=32)):
+                got, prof = solve(smp, n, pm, pw, precheck=0, start_order=order, **tail)
+                assert np.array_equal(got, want), (kind, n, order, tail)
+                assert prof["proposals"] == t["proposals"], (kind, n, order, tail)
+
+
 def test_config3_congested_and_shifted_20000(gpu, oracle):
     smp = gpu
     n = 20000
