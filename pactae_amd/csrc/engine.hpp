@@ -270,6 +270,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // in 16-31, list position in 32-47 (all < n <= 65535).  One word keeps each
   // proposal on one cache line.
   std::vector<uint64_t> hst;
+  // The rank field of hst alone (2 bytes per woman: 40 KB at n = 20,000, inside
+  // a core's L1), read by the walk's eight-position scans; kept equal to hst's.
+  std::vector<uint16_t> hrank;
   static constexpr uint32_t kHostFree = 0xFFFFu;
   static constexpr int kTailSlots = 4;  // chains the host walks interleaved
   static uint64_t hs_pack(uint32_t r, uint32_t m, uint32_t p) {
@@ -1285,13 +1288,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
     t0 = Clock::now();
     int kcap = std::min(std::min(K, n), blocks_copied * pactae::kPrBlock);  // host-side prefix
     hst.resize(n);
+    hrank.resize(n);
     deep_w.clear();
     for (int w = 0; w < n; ++w) {
       const unsigned long long v = h_partner[w];
       if (v == pactae::kEmpty) {
         hst[w] = kHostFree;
+        hrank[w] = (uint16_t)kHostFree;
       } else {
         hst[w] = hs_pack(pactae::pw_rank(v), pactae::pw_man(v), pactae::pw_pos(v));
+        hrank[w] = (uint16_t)pactae::pw_rank(v);
         if ((int)pactae::pw_pos(v) + 1 + kWinAhead >= kcap) deep_w.push_back(w);
       }
     }
@@ -1302,6 +1308,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     int bailed = 0;
     const uint32_t* pr = h_pr;  // the prefix, column-blocked (pr_index)
     uint64_t* st = hst.data();
+    uint16_t* sr = hrank.data();
     // A band the host holds: positions kb0 .. kb1 - 1 of every man, row-major
     // with bw = kb1 - kb0 entries per man (none while kb1 = 0).
     const uint32_t* band = nullptr;
@@ -1353,7 +1360,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
               const uint32_t w8 = pactae::pr_woman(e8);
               range |= (uint32_t)(w8 >= (uint32_t)n) << j;
               const uint32_t wc = w8 < (uint32_t)n ? w8 : 0u;
-              cand |= (uint32_t)(pactae::pr_rank(e8) < (uint32_t)(st[wc] & 0xFFFFu)) << j;
+              cand |= (uint32_t)(pactae::pr_rank(e8) < (uint32_t)sr[wc]) << j;
             }
             const uint32_t stop = cand | range;  // a malformed entry goes to the step below
             if (stop == 0) {
@@ -1395,6 +1402,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
           const uint64_t v = st[w];
           if (r < (uint32_t)(v & 0xFFFFu)) {
             st[w] = hs_pack(r, (uint32_t)m, (uint32_t)k);
+            sr[w] = (uint16_t)r;
             if (!FULL && k + 1 + kWinAhead >= hd) deep_w.push_back((int)w);  // he may soon leave what the host holds
             if ((v & 0xFFFFu) == kHostFree) {
               ended = true;
