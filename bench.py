@@ -221,15 +221,14 @@ def ranked_leg(smp, args, n, pm, pw, dm, match):
     return res
 
 
-def cpu_check_fig9(legs, keep):
-    """GS-Seq on the Fig 9 legs' instances: the bit-exact check of their GPU
-    matchings, and one CPU solve each (the reference's host flags) beside them."""
+def cpu_check_fig9(res, n, pm, pw, match):
+    """GS-Seq on a Fig 9 leg's instance: the bit-exact check of its GPU matching,
+    and one CPU solve (the reference's host flags) beside it."""
     from oracle import oracle
-    for res, (n, pm, pw, match) in zip(legs, keep):
-        want, t = oracle.gs_seq(n, pm, pw)
-        res["bit_exact"] = bool(np.array_equal(want, match))
-        res["cpu_gs_seq_ms"] = round(t["total_ms"], 1)
-        log(f"cpu GS-Seq on {res['workload']}: {t['total_ms']:.0f} ms, bit-exact {res['bit_exact']}")
+    want, t = oracle.gs_seq(n, pm, pw)
+    res["bit_exact"] = bool(np.array_equal(want, match))
+    res["cpu_gs_seq_ms"] = round(t["total_ms"], 1)
+    log(f"cpu GS-Seq on {res['workload']}: {t['total_ms']:.0f} ms, bit-exact {res['bit_exact']}")
 
 
 def cpu_baseline_single(args, n, pm, pw, gpu_match, inst):
@@ -296,17 +295,24 @@ def cpu_baseline_single(args, n, pm, pw, gpu_match, inst):
 
 
 def fig9_legs(smp, args):
-    """The reference's own Fig 9 inputs (benchmark/figure_9.cu:40-57) at n = 20,000,
-    beside the headline: RANDOM with groups of 5 (about n^2 / 2 proposals under
-    heavy contention) and SOLO (one displacement chain of n^2 - n + 1 proposals,
-    walked by the host tail from the whole PR matrix).  Default engine options;
-    one warm-up solve (it allocates the whole-matrix buffers, and on RANDOM it
-    shows the chains walking far past the prefix, so that the engine takes the
-    whole PR matrix from the next solve on), then timed solves.  The warm-up
-    solve's time is reported beside them."""
-    out, keep = [], []
-    for name, kind, param, runs in (("REF_RANDOM(group 5)", smp.Workload.REF_RANDOM, 5, 3),
-                                    ("REF_SOLO", smp.Workload.REF_SOLO, 0, 1)):
+    """The reference's own Fig 9 (benchmark/figure_9.cu:22-57) at its n = 30,000,
+    beside the headline: the reference's CONGESTED (men share one list, women
+    share another), PERFECT (distinct first choices), RANDOM with groups of 5
+    (about n^2 / 2 proposals under heavy contention) and SOLO (one displacement
+    chain of n^2 - n + 1 proposals, walked by the host tail from the whole PR
+    matrix).  Default engine options; one first solve (it allocates the
+    whole-matrix buffers, and on RANDOM it switches to the whole PR matrix during
+    the walk), then timed solves; the first solve's time is reported beside
+    them.  Fig 9's GPU baseline, MW-Par-GPU-CAS (src/smp/smp_engine_mw_3_f9.cu,
+    the schedule of this engine), is timed once on every leg but SOLO, which the
+    reference leaves out for it (figure_9.cu:53-54).  With the CPU legs on, each
+    leg is checked against GS-Seq (bit-exact) and GS-Seq's time is recorded."""
+    out = []
+    legs = (("REF_CONGESTED", smp.Workload.REF_CONGESTED, 0, 3, True),
+            ("REF_PERFECT", smp.Workload.REF_PERFECT, 0, 3, True),
+            ("REF_RANDOM(group 5)", smp.Workload.REF_RANDOM, 5, 3, True),
+            ("REF_SOLO", smp.Workload.REF_SOLO, 0, 1, False))
+    for name, kind, param, runs, with_cas in legs:
         n = args.fig9_n
         pm, pw = smp.generate(kind, n, seed=args.seed + 0x90, param=param, threads=16)
         dm, dw = smp.DeviceArray(pm), smp.DeviceArray(pw)
@@ -316,19 +322,28 @@ def fig9_legs(smp, args):
             walls = _timed(e, runs)
             p = e.profile()
             match = e.result()
+        cas = None
+        if with_cas:
+            with smp.SmpEngineBamboo.from_device(dm, dw, n, schedule=smp.Schedule.MW_CAS) as e:
+                cas_ms = _timed(e, 1)[0]
+                cas = {"ms_per_solve": round(cas_ms, 3), "bit_exact": bool(np.array_equal(e.result(), match))}
         dm.free()
         dw.free()
         res = {"workload": name, "n": n, "seed": args.seed + 0x90, "ms_per_solve": round(statistics.median(walls), 3),
                "runs": runs, "first_solve_ms": round(first, 3), "first_solve_prefix_cols": first_cols,
-               "prefix_cols": p["prefix_cols"], "proposals": p["proposals"],
+               "prefix_cols": p["prefix_cols"], "proposals": p["proposals"], "is_perfect": p["is_perfect"],
+               "master_list": p["master_list"], "master_both": p["master_both"],
                "propose_kernel_ms": round(p["propose_kernel_ms"], 3), "gather_kernel_ms": round(p["gather_kernel_ms"], 3),
                "host_tail_ms": round(p["host_tail_ms"], 3), "host_proposals": p["host_proposals"],
-               "tail_windows": p["tail_windows"], "tail_rounds": p["tail_rounds"], "tail_bands": p["tail_bands"], "tail_full_matrix": p["tail_full_matrix"],
-               "resume_ms": round(p["resume_ms"], 3)}
+               "tail_windows": p["tail_windows"], "tail_rounds": p["tail_rounds"], "tail_bands": p["tail_bands"],
+               "tail_full_matrix": p["tail_full_matrix"], "deep_restart": p["deep_restart"],
+               "resume_ms": round(p["resume_ms"], 3), "gpu_baseline_mw_cas": cas}
+        if not args.no_cpu:
+            cpu_check_fig9(res, n, pm, pw, match)
         log(f"fig9 {name}: {res}")
         out.append(res)
-        keep.append((n, pm, pw, match))
-    return out, keep
+        del pm, pw
+    return out
 
 
 def batched(smp, args):
@@ -527,8 +542,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the headline and CPU legs (for profiling the batched or serving leg alone)")
-    ap.add_argument("--no-fig9", action="store_true", help="skip the reference's RANDOM(5) and SOLO legs")
-    ap.add_argument("--fig9-n", type=int, default=20000)
+    ap.add_argument("--no-fig9", action="store_true", help="skip the legs on the reference's Fig 9 workloads")
+    ap.add_argument("--fig9-n", type=int, default=30000, help="n of the Fig 9 legs (the reference's Fig 9: 30,000)")
     ap.add_argument("--no-ranked", action="store_true",
                     help="skip the leg with the women's rank matrix given as input")
     args = ap.parse_args()
@@ -553,6 +568,8 @@ def main() -> int:
     if args.no_single:  # the other legs alone (profiling runs)
         out = {"metric": "ms/solve at n=20k (bit-exact); batched solves/s vs looped plain launches", "value": None,
                "unit": "ms/solve", "n_gpus": 1, "note": "--no-single: the headline leg was not run"}
+        if not args.no_fig9:
+            out["fig9"] = fig9_legs(smp, args)
         if not args.no_batch:
             out["batched"], _ = batched(smp, args)
         if not args.no_serve:
@@ -626,11 +643,7 @@ def main() -> int:
         out["bit_exact_timing_seeds"] = seeds_ok
         out["speedup_vs_cpu"] = round(cpu["value"] / value, 2)
     if not args.no_fig9:
-        legs, keep = fig9_legs(smp, args)
-        if not args.no_cpu:
-            cpu_check_fig9(legs, keep)
-        del keep
-        out["fig9"] = legs
+        out["fig9"] = fig9_legs(smp, args)
     if not args.no_batch:
         bres, _ = batched(smp, args)
         out["batched"] = bres

@@ -239,6 +239,11 @@ typedef struct smp_times {
   int32_t deep_restart;    /* 1 if the proposal kernel's chains walked so far past
                               the prefix that every chain was parked, the whole
                               PR matrix gathered and the chains resumed over it   */
+  int32_t master_both;     /* 1 if, besides every man holding the same list, every
+                              woman held the same list: the matching then is the
+                              serial dictatorship match[P[k]] = L[k] (the
+                              reference's CONGESTED lists)                        */
+  int32_t reserved[1];
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;

@@ -10,10 +10,10 @@ __all__ = ["build", "smp"]
 
 
 def __getattr__(name):
-    if name == "smp":
-        from . import smp
-        return smp
-    if name == "build":
-        from .build import build
-        return build
+    # Import the submodule by its full name: ``from . import smp`` would look the
+    # attribute up on this package first and land here again.  The import binds
+    # the submodule on the package, so this runs once per name.
+    if name in ("smp", "build"):
+        import importlib
+        return importlib.import_module(f"{__name__}.{name}")
     raise AttributeError(name)

@@ -90,6 +90,8 @@ class SmpTimes(ctypes.Structure):
         ("tail_rounds", i32),
         ("tail_bands", i32),
         ("deep_restart", i32),
+        ("master_both", i32),
+        ("reserved", i32 * 1),
     ]
 
     def as_dict(self) -> dict:

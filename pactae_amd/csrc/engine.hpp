@@ -123,6 +123,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // Lists whose chains walked far past the prefix on the GPU take the whole PR
   // matrix from their next solve on (see deep_lists()).
   bool full_pr_next = false;
+  // Master-list path: the women share woman 0's list too (closed form).
+  bool master_both = false;
   // A proposal launch that resumes parked chains (a restart over the whole PR
   // matrix) instead of starting every man, and the deep-step limit of the
   // launch (0 = off).
@@ -772,6 +774,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     span_bucket = span_rank = span_gather = span_propose = Span{};
     have_result = false;
     master = false;
+    master_both = false;
     if (full_pr_next) {
       full_pr_next = false;
       if (grow_pr(n)) K = n;
@@ -826,6 +829,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       SMP_HIP_TRY(hipEventRecord(ev[2], stream));
       if (int st = read_scalars()) return st;
       master = h_sc->not_master == 0;
+      master_both = master && h_sc->both_master != 0;
       if (master) {
         span_propose = {&ev[0], &ev[2]};
       } else {
@@ -837,6 +841,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       }
     }
     times.master_list = master ? 1 : 0;
+    times.master_both = master_both ? 1 : 0;
     return SMP_OK;
   }
 
@@ -1592,6 +1597,8 @@ struct smp_engine : public pactae::AbsSmpEngine {
       SMP_HIP_TRY(pactae::launch_first_match(d_pref_m, n, d_match, stream));
     } else if (thread_walk()) {
       SMP_HIP_TRY(pactae::launch_post_cas(d_pref_w, d_seen, d_cas_next, n, d_match, d_sc, stream));
+    } else if (master_both) {
+      SMP_HIP_TRY(pactae::launch_master_both(d_pref_m, d_pref_w, n, d_match, d_sc, stream));
     } else if (master) {
       SMP_HIP_TRY(pactae::launch_post_master(d_partner, d_seen, n, d_match, d_sc, stream));
     } else {

@@ -64,6 +64,9 @@ __global__ void k_reset_scalars(Scalars* sc, int active, int* z0, size_t n0, int
     sc->not_master = 0;
     sc->export_blocks = 0;
     sc->restart = 0;
+    sc->women_differ = 0;
+    sc->scans_done = 0;
+    sc->both_master = 0;
     sc->deep_steps = 0;
     sc->atomics = 0;
     sc->steps = 0;
@@ -1092,38 +1095,125 @@ __global__ void k_master_inverse(const int32_t* __restrict__ L, int n, int32_t* 
   }
 }
 
-// Permutation check of every row (one workgroup per row, an n-bit LDS set), by
-// workgroups blk = 0..nblk-1 of a launch.  With stop_unless_master the check
-// stops once another workgroup has found that the men's lists differ (it then ran
-// ahead of a decision that went the other way; the general path checks the rows
-// itself).
-__device__ void check_rows_blocks(const int32_t* __restrict__ pref, int n, Scalars* sc, int blk, int nblk,
-                                  uint32_t* bits, bool stop_unless_master) {
-  const int words = (n + 31) >> 5;
-  int bad = 0;
-  for (int row = blk; row < n; row += nblk) {
-    if (stop_unless_master && __hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      break;
-    for (int i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0;
-    __syncthreads();
-    const int32_t* src = pref + (size_t)row * n;
-    for (int r0 = threadIdx.x; r0 < n; r0 += 256 * 8) {
-      int v[8];
+// Marks entry v of a row in the row's n-bit LDS set; a value out of range or
+// seen before is an error.
+__device__ __forceinline__ void perm_mark(int v, int n, uint32_t* bits, int& bad) {
+  if ((unsigned)v >= (unsigned)n) {
+    bad |= kErrEntryRange;
+  } else {
+    const uint32_t bit = 1u << (v & 31);
+    if (atomicOr(&bits[v >> 5], bit) & bit) bad |= kErrNotPerm;
+  }
+}
+
+// Whether row `src` equals row 0 (`row0`), decided by the whole workgroup (the
+// result is the same in every thread).  4096 entries a round; a round that
+// differs ends the comparison.
+__device__ bool row_equals_row0(const int32_t* __restrict__ src, const int32_t* __restrict__ row0, int n) {
+  if ((n & 3) == 0) {
+    const int4* s4 = reinterpret_cast<const int4*>(src);
+    const int4* l4 = reinterpret_cast<const int4*>(row0);
+    const int nq = n >> 2;
+    for (int q0 = 0; q0 < nq; q0 += 256 * 4) {
+      bool diff = false;
+      int4 a[4];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int r = r0 + 256 * j;
-        v[j] = r < n ? __builtin_nontemporal_load(src + r) : 0;
+      for (int j = 0; j < 4; ++j) {
+        const int q = min(q0 + 256 * j + (int)threadIdx.x, nq - 1);
+        a[j].x = __builtin_nontemporal_load(&s4[q].x);
+        a[j].y = __builtin_nontemporal_load(&s4[q].y);
+        a[j].z = __builtin_nontemporal_load(&s4[q].z);
+        a[j].w = __builtin_nontemporal_load(&s4[q].w);
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if (r0 + 256 * j < n) {
-          if ((unsigned)v[j] >= (unsigned)n) {
-            bad |= kErrEntryRange;
-          } else {
-            const uint32_t bit = 1u << (v[j] & 31);
-            if (atomicOr(&bits[v[j] >> 5], bit) & bit) bad |= kErrNotPerm;
+      for (int j = 0; j < 4; ++j) {
+        const int q = min(q0 + 256 * j + (int)threadIdx.x, nq - 1);
+        const int4 b = l4[q];
+        diff |= a[j].x != b.x || a[j].y != b.y || a[j].z != b.z || a[j].w != b.w;
+      }
+      if (__syncthreads_or(diff)) return false;
+    }
+    return true;
+  }
+  for (int r0 = 0; r0 < n; r0 += 256 * 8) {
+    bool diff = false;
+    int v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int r = min(r0 + 256 * j + (int)threadIdx.x, n - 1);
+      v[j] = __builtin_nontemporal_load(src + r);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) diff |= v[j] != row0[min(r0 + 256 * j + (int)threadIdx.x, n - 1)];
+    if (__syncthreads_or(diff)) return false;
+  }
+  return true;
+}
+
+// The women's rows of the master-list launch, by workgroups blk = 0..nblk-1.
+// Every row is compared with row 0 until some row differs (that row sets
+// sc->women_differ); a row equal to row 0 is a permutation exactly when row 0
+// is, so only row 0 and the rows that differ from it get the permutation check
+// (an n-bit LDS set per row).  The scan stops once another workgroup has found
+// that the men's lists differ (it then ran ahead of a decision that went the
+// other way; the general path checks the rows itself).
+__device__ void check_rows_blocks(const int32_t* __restrict__ pref, int n, Scalars* sc, int blk, int nblk,
+                                  uint32_t* bits) {
+  const int words = (n + 31) >> 5;
+  int bad = 0;
+  bool compare = true;
+  for (int row = blk; row < n; row += nblk) {
+    // Thread 0 reads the flags and the workgroup follows it (the barrier of
+    // __syncthreads_or hands its value to every thread), so every thread leaves
+    // the loop, or stops comparing, together.
+    const bool lead = threadIdx.x == 0;
+    if (__syncthreads_or(lead && __hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      break;
+    if (compare &&
+        __syncthreads_or(lead && __hip_atomic_load(&sc->women_differ, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      compare = false;
+    const int32_t* src = pref + (size_t)row * n;
+    if (compare && row != 0) {
+      if (row_equals_row0(src, pref, n)) continue;
+      if (lead) atomicOr(&sc->women_differ, 1);
+      compare = false;
+    }
+    for (int i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0;
+    __syncthreads();
+    if ((n & 3) == 0) {
+      const int4* s4 = reinterpret_cast<const int4*>(src);
+      const int nq = n >> 2;
+      for (int q0 = threadIdx.x; q0 < nq; q0 += 256 * 4) {
+        int4 a[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int q = min(q0 + 256 * j, nq - 1);
+          a[j].x = __builtin_nontemporal_load(&s4[q].x);
+          a[j].y = __builtin_nontemporal_load(&s4[q].y);
+          a[j].z = __builtin_nontemporal_load(&s4[q].z);
+          a[j].w = __builtin_nontemporal_load(&s4[q].w);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (q0 + 256 * j < nq) {
+            perm_mark(a[j].x, n, bits, bad);
+            perm_mark(a[j].y, n, bits, bad);
+            perm_mark(a[j].z, n, bits, bad);
+            perm_mark(a[j].w, n, bits, bad);
           }
         }
+      }
+    } else {
+      for (int r0 = threadIdx.x; r0 < n; r0 += 256 * 8) {
+        int v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int r = r0 + 256 * j;
+          v[j] = r < n ? __builtin_nontemporal_load(src + r) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (r0 + 256 * j < n) perm_mark(v[j], n, bits, bad);
       }
     }
     __syncthreads();
@@ -1141,8 +1231,15 @@ __device__ void check_rows_blocks(const int32_t* __restrict__ pref, int n, Scala
 // scan that decides whether the lists are a master list, it is speculative: it
 // stops once the scan has found two men's lists that differ (sc->not_master),
 // checked every 16 steps, and its result is then thrown away.
+// With wait_scans != 0 the launch also compares every woman's list with woman
+// 0's (wait_scans scan workgroups, each counted in sc->scans_done when it ends):
+// the walk starts only once a woman's list has differed (sc->women_differ) or
+// every scan has ended, since when all women share a list the walk is not needed
+// (sc->both_master) and, on such lists a serial chain of n^2 / 2 proposals, would
+// only take bandwidth from the scans.  The scans wait on nothing, so the wait ends.
 __device__ void master_walk(const int32_t* __restrict__ pref_w, const int32_t* __restrict__ linv,
-                            unsigned long long* partner_m, int n, Scalars* sc, int wave, int waves) {
+                            unsigned long long* partner_m, int n, Scalars* sc, int wave, int waves,
+                            int wait_scans) {
   const int lane = threadIdx.x & 63;
   int next_init = wave;
   int w = -1, k = 0;
@@ -1150,7 +1247,21 @@ __device__ void master_walk(const int32_t* __restrict__ pref_w, const int32_t* _
   // would displace (by the hint) is loaded too; (spec_w, spec_k) says whose.
   int spec_w = -1, spec_k = -1, spec_m = 0;
   unsigned long long n_atomics = 0, n_steps = 0;
-  if (__hip_atomic_load(&sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) next_init = n;
+  if (wait_scans) {
+    for (;;) {
+      int go = 0;
+      if (lane == 0)
+        go = __hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+             __hip_atomic_load(&sc->women_differ, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+             (__hip_atomic_load(&sc->scans_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= wait_scans);
+      if (__shfl(go, 0)) break;
+      __builtin_amdgcn_s_sleep(16);
+    }
+  }
+  if (__hip_atomic_load(&sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+      __hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+      __hip_atomic_load(&sc->both_master, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    next_init = n;
   for (;;) {
     if (w < 0) {
       if (next_init >= n) break;
@@ -1163,7 +1274,8 @@ __device__ void master_walk(const int32_t* __restrict__ pref_w, const int32_t* _
       break;
     }
     if ((n_steps & 15) == 15 &&
-        __hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        (__hip_atomic_load(&sc->not_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+         __hip_atomic_load(&sc->both_master, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0))
       break;
     const uint32_t rw = (uint32_t)linv[w];
     const int kk = k + lane;
@@ -1234,15 +1346,55 @@ __global__ __launch_bounds__(256) void k_master_fused(const int32_t* __restrict_
   extern __shared__ uint32_t bits[];
   const int b = (int)blockIdx.x;
   if (b < g_walk) {
-    master_walk(pref_w, linv, partner_m, n, sc, b * 4 + (int)(threadIdx.x >> 6), g_walk * 4);
+    master_walk(pref_w, linv, partner_m, n, sc, b * 4 + (int)(threadIdx.x >> 6), g_walk * 4,
+                check ? (int)gridDim.x - g_walk : 0);
     return;
   }
   const int r = b - g_walk;
   if (r < g_rows) {
     rows_equal_blocks(pref_m, n, sc, r, g_rows);
-    return;
+  } else if (check) {
+    check_rows_blocks(pref_w, n, sc, r - g_rows, (int)gridDim.x - g_walk - g_rows, bits);
   }
-  if (check) check_rows_blocks(pref_w, n, sc, r - g_rows, (int)gridDim.x - g_walk - g_rows, bits, true);
+  // With the women's rows checked too: the last scan workgroup to finish, if
+  // no man's list differs from man 0's and no woman's from woman 0's, sets
+  // both_master, which stops the walk (the matching then has a closed form).
+  if (check) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const int scans = (int)gridDim.x - g_walk;
+      const int done = __hip_atomic_fetch_add(&sc->scans_done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+      if (done == scans && __hip_atomic_load(&sc->not_master, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+          __hip_atomic_load(&sc->women_differ, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __hip_atomic_store(&sc->both_master, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Every man holds man 0's list L and every woman woman 0's list P: the stable
+// matching is the serial dictatorship match[P[k]] = L[k] (the man all women put
+// at position k gets the woman all men put there), and the man-proposing count
+// is sum over k of (k + 1).  Both rows were checked to be permutations.
+__global__ __launch_bounds__(256) void k_master_both(const int32_t* __restrict__ pref_m,
+                                                     const int32_t* __restrict__ pref_w, int n,
+                                                     int32_t* __restrict__ match, Scalars* sc) {
+  __shared__ unsigned long long red[4];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long props = 0;
+  if (k < n) {
+    const int m = pref_w[k], w = pref_m[k];
+    if ((unsigned)m < (unsigned)n && (unsigned)w < (unsigned)n) {
+      match[m] = w;
+      props = (unsigned long long)k + 1ull;
+    } else {
+      atomicOr(&sc->error, kErrEntryRange);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) props += __shfl_down(props, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = props;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&sc->proposals, red[0] + red[1] + red[2] + red[3]);
 }
 
 // match[m] = the woman in man m's partner word; proposals (the man-proposing
@@ -1690,6 +1842,12 @@ hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars
   hipError_t e = launch_fill_i32(linv, n, -1, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_master_inverse, dim3((n + 255) / 256), dim3(256), 0, s, L, n, linv, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_master_both(const int32_t* pref_m, const int32_t* pref_w, int n, int32_t* match, Scalars* sc,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_master_both, dim3((n + 255) / 256), dim3(256), 0, s, pref_m, pref_w, n, match, sc);
   return hipGetLastError();
 }
 

@@ -73,6 +73,10 @@ hipError_t launch_first_choices(const int32_t* pref_m, int n, int* seen, Scalars
 hipError_t launch_master_fused(const int32_t* pref_m, const int32_t* pref_w, const int32_t* linv,
                                unsigned long long* partner_m, int n, int check, Scalars* sc, int cus, hipStream_t s);
 hipError_t launch_master_inverse(const int32_t* L, int n, int32_t* linv, Scalars* sc, hipStream_t s);
+// Both sides hold master lists (every man man 0's list, every woman woman 0's):
+// match[pref_w[k]] = pref_m[k], and sc->proposals += sum of (k + 1).
+hipError_t launch_master_both(const int32_t* pref_m, const int32_t* pref_w, int n, int32_t* match, Scalars* sc,
+                              hipStream_t s);
 hipError_t launch_post_master(const unsigned long long* partner_m, const int32_t* linv, int n, int32_t* match,
                               Scalars* sc, hipStream_t s);
 // The reference's thread-per-man walks (schedule SMP_SCHEDULE_MW_CAS, _LA_CAS,

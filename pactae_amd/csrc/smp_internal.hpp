@@ -76,6 +76,10 @@ struct Scalars {
   int export_blocks; // k_export_state: blocks finished (the last one resets it)
   int restart;       // proposal kernel: deep chains seen, every chain parked for a
                      // restart over the whole PR matrix
+  int women_differ;  // master-list path: some woman's list differs from woman 0's
+  int scans_done;    // master-list path: scan workgroups finished
+  int both_master;   // master-list path: every scan done, every man's list equal to
+                     // man 0's and every woman's to woman 0's
   int pad0;
   unsigned long long deep_steps; // proposal kernel: steps past the PR prefix so far
                                  // (flushed by the waves every 256 steps)

@@ -29,6 +29,15 @@ def test_library_exports_every_declared_symbol(smp):
     assert set(names) == bound, set(names) ^ bound
 
 
+def test_package_exposes_its_modules(smp):
+    """Both import forms reach the same module (the package imports them lazily)."""
+    import pactae_amd
+    from pactae_amd import smp as a
+    assert pactae_amd.smp is a
+    from pactae_amd import build as b
+    assert callable(b.build)
+
+
 def test_struct_layouts_match_header(tmp_path):
     from pactae_amd import _abi
     src = tmp_path / "sz.c"

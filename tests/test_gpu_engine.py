@@ -495,7 +495,7 @@ def test_config3_congested_and_shifted_20000(gpu, oracle):
     pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=0x5EED0003)
     want = oracle.serial_dictatorship(n, pm[:n], pw)
     got, prof = solve(smp, n, pm, pw, tail_mode=1)
-    assert prof["master_list"] == 1
+    assert prof["master_list"] == 1 and prof["master_both"] == 0
     assert np.array_equal(got, want)
     assert prof["proposals"] == n * (n + 1) // 2
     # the man-proposing walk on the same input (precheck off: no fast paths)
@@ -507,6 +507,26 @@ def test_config3_congested_and_shifted_20000(gpu, oracle):
     got, prof = solve(smp, n, pm, pw)
     assert prof["is_perfect"] == 1
     assert np.array_equal(got, pm.reshape(n, n)[:, 0])
+
+
+def test_reference_congested_30000(gpu):
+    """The reference's CONGESTED lists at n = 30,000 (men share L, women share
+    P): the stable matching is match[P[k]] = L[k] with n(n+1)/2 man-proposing
+    proposals, answered without a walk."""
+    smp = gpu
+    n = 30000
+    pm, pw = smp.generate(smp.Workload.REF_CONGESTED, n, seed=0x5EED0003)
+    L, P = pm[:n], pw[:n]
+    want = np.empty(n, np.int32)
+    want[P] = L
+    obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
+    with smp.SmpEngineBamboo(obj, n) as e:
+        for _ in range(2):  # the second solve starts from the first one's state
+            got = e.find_stable_matching()
+            prof = e.profile()
+            assert prof["master_list"] == 1 and prof["master_both"] == 1
+            assert np.array_equal(got, want)
+            assert prof["proposals"] == n * (n + 1) // 2
 
 
 @pytest.mark.parametrize("n", [2, 3, 64, 65, 1000, 2048])
@@ -521,9 +541,26 @@ def test_master_list_path(gpu, oracle, n):
         assert t["proposals"] == n * (n + 1) // 2
         got, prof = solve(smp, n, pm, pw)
         assert prof["master_list"] == 1 and prof["is_perfect"] == 0
+        # the reference's CONGESTED lists: the women share a list too (for
+        # tiny n the random women's lists of CONGESTED can coincide as well)
+        women_share = bool((pw.reshape(n, n) == pw[:n]).all())
+        assert women_share or kind == smp.Workload.CONGESTED
+        assert prof["master_both"] == women_share, kind
         assert np.array_equal(got, want), kind
         assert prof["proposals"] == t["proposals"]
         assert oracle.count_blocking_pairs(n, pm, pw, got) == 0
+        if kind == smp.Workload.REF_CONGESTED:
+            # one woman's list differs from woman 0's (her last two entries,
+            # or for n = 2 woman 0's own): the women-proposing walk answers
+            for row in sorted({0, n - 1}):
+                wnear = pw.copy().reshape(n, n)
+                wnear[row, -2], wnear[row, -1] = wnear[row, -1], wnear[row, -2]
+                wnear = wnear.reshape(-1)
+                want3, t3 = oracle.gs_seq(n, pm, wnear)
+                got3, prof3 = solve(smp, n, pm, wnear)
+                assert prof3["master_list"] == 1 and prof3["master_both"] == 0, row
+                assert np.array_equal(got3, want3), row
+                assert prof3["proposals"] == t3["proposals"], row
         near = pm.copy().reshape(n, n)
         row = n - 1
         near[row, -2], near[row, -1] = near[row, -1], near[row, -2]
