@@ -433,13 +433,20 @@ typedef struct smp_server_opts {
                                workers of their own, beside the lanes;
                                0 = 512; above max_n = never                        */
   int32_t large_workers;    /* threads (each with its own pipeline) serving the large
-                               requests; 0 = 3                                     */
+                               requests; 0 = 2                                     */
   int32_t large_spill_us;   /* > 0: worker i > 0 takes the oldest waiting large
                                request only when more than i are waiting or it has
                                waited this long (so that at low load a request runs
                                alone on the GPU); 0 = any idle worker at once
                                (default: measured best for p50 and p99)            */
-  int32_t reserved[5];
+  int32_t small_chain_lanes; /* lanes per proposal chain in the one-kernel solve of
+                               lane launches whose requests all have n <= 512
+                               (2, 4, 8 or 16); 0 = auto                          */
+  int32_t small_to_idle_worker; /* batched mode: 1 = a request below large_n goes to
+                               a large-request worker when one is idle and none
+                               waits for the workers, else to the lanes;
+                               0 = always to the lanes; -1 = the default          */
+  int32_t reserved[3];
 } smp_server_opts;
 
 typedef struct smp_request_info {

@@ -177,7 +177,9 @@ class SmpServerOpts(ctypes.Structure):
         ("large_n", i32),
         ("large_workers", i32),
         ("large_spill_us", i32),
-        ("reserved", i32 * 5),
+        ("small_chain_lanes", i32),
+        ("small_to_idle_worker", i32),
+        ("reserved", i32 * 3),
     ]
 
 

@@ -554,6 +554,9 @@ struct smp_engine : public pactae::AbsSmpEngine {
       if (opts.prefix_cols <= 0) K = tuned_K;
       if (opts.tail_threshold <= 0) T = tuned_T;
     }
+    // The prefix buffers (d_pr, h_pr) hold K_alloc columns; a width set after
+    // they were sized is cut to them (the whole matrix goes through grow_pr).
+    if (K_alloc > 0) K = std::min(K, K_alloc);
   }
 
   // Measured choice of the hand-off threshold T and the prefix width K: every

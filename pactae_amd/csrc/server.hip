@@ -4,21 +4,26 @@
 //
 // Requests are admitted in arrival order when the HBM arena has room for their
 // state (int16 rank matrix, plus a copy of the lists for host-submitted
-// requests).  Admitted requests are launched together on whichever launch lane
-// (stream) is free; with several lanes, a request that arrives while others are on
-// the GPU starts as soon as a lane and memory free up, and its kernels overlap
-// with the ones already running.  One launch is two kernels:
-//   k_rank_rows16   all women's rank rows of the launch over a flat row index,
-//                   one wave per row (so a large request is spread over the
-//                   whole GPU), staged in LDS, written as int16;
-//   k_serve_chains  one workgroup per request: proposal chains, 16 lanes per
-//                   chain, with the partner words in LDS, then the matching,
-//                   written straight into pinned host memory.
-// Requests with n >= large_n skip the lanes: large_workers threads, each with a
-// pipeline of its own, solve them one at a time each with the multi-workgroup
-// single-instance pipeline
-// (engine.hpp), whose chains run over the whole GPU instead of one workgroup, so a
-// large congested request does not hold a lane for its whole serial chain.
+// requests).  Where each goes:
+//   - n >= large_n: to large_workers threads, each with a single-instance
+//     pipeline of its own (engine.hpp: chains over the whole GPU, then the host
+//     tail), so a large congested request does not hold a lane for its whole
+//     serial chain;
+//   - smaller ones, with small_to_idle_worker, to a worker that is idle while no
+//     request waits for the workers: at low load a small request then gets the
+//     same pipeline as in plain mode, whose host tail walks the last chains
+//     faster than one workgroup can;
+//   - the rest to the launch lanes (streams): admitted requests are launched
+//     together on whichever lane is free, so a request that arrives while
+//     others are on the GPU starts as soon as a lane and memory free up, and its
+//     kernels overlap with the ones already running.  A launch whose requests
+//     all have n <= kSmallN is one kernel, k_serve_small (one workgroup per
+//     request: rank rows, chains, matching); a larger one is k_serve_prep, a
+//     flat k_rank_rows16 pass over all its women's rows (one wave per row, so a
+//     large request is spread over the whole GPU) and k_serve_chains (one
+//     workgroup per request).  Results, proposal counts, error bits and a
+//     completion flag go straight into pinned host memory; the server loop
+//     polls the flag.
 // The plain mode is the baseline: one request at a time through the plain
 // single-instance pipeline.
 
@@ -43,20 +48,179 @@ struct ServeReq {
   int row0;        // first flat row index of this request in the launch
 };
 
+// The launch's request descriptors, written by the host into pinned memory, are
+// copied to device memory (the rank pass looks a row's request up by binary
+// search over them), and the error words are cleared.  One small kernel in place
+// of a copy and a fill: each copy on a lane's stream is a switch between the
+// compute queue and a copy engine.
+__global__ __launch_bounds__(256) void k_serve_prep(const ServeReq* __restrict__ host_reqs, ServeReq* reqs,
+                                                    int count, int* errs) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    reqs[i] = host_reqs[i];
+    errs[i] = 0;
+  }
+}
+
+// One workgroup per request.  The matching goes straight into pinned host memory
+// (q.match), and so do the proposal count and the error bits (the rank pass's,
+// from errs, with the chains').  Each workgroup releases its host writes at system
+// scope; the last one to finish (counted in *done, which it resets) then writes
+// seq into the host's flag word, so a host that sees seq sees every result.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK, 8) void k_serve_chains(const ServeReq* __restrict__ reqs, int count,
-                                                        unsigned long long* props, int* errs) {
+                                                        const int* __restrict__ errs,
+                                                        unsigned long long* h_props, int* h_errs, int* done,
+                                                        unsigned* h_flag, unsigned seq) {
   extern __shared__ unsigned long long partner[];
-  __shared__ int cursor;
+  __shared__ int cursor, sbad;
   __shared__ unsigned long long red[BLOCK / 64];
   const int i = blockIdx.x;
-  if (i >= count) return;
+  if (threadIdx.x == 0) sbad = 0;
   const ServeReq q = reqs[i];
   InstanceRef ref{q.pm, q.pw, q.rank, q.match, q.n};
   unsigned long long p = 0;
   const int bad = wg_chains_g<BLOCK, 16, 8, false>(ref, partner, &cursor, red, &p);
-  if (bad) atomicOr(&errs[i], bad);
-  if (threadIdx.x == 0) props[i] = p;
+  if (bad) atomicOr(&sbad, bad);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    h_props[i] = p;
+    h_errs[i] = sbad | __hip_atomic_load(&errs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(done, 1) == count - 1) {
+    *done = 0;
+    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Small requests (n <= kSmallN): the whole solve in the request's own workgroup,
+// one launch per lane launch.  Split over a flat rank kernel and a chain kernel,
+// a small request paid two dependent HBM misses per chain step (its list entry,
+// then its rank, which the flat pass had left in other XCDs' L2s): ~180 us at
+// n < 512.  Here the workgroup inverts its own women's rows (rank[w][pref_w[w][r]]
+// = r, the F9RankMatrixInitKernel step, src/smp/kernel.cu:672-686) and writes the
+// int16 ranks itself, so they sit in its own XCD's L2 when its chains read them.
+constexpr int kSmallN = 512;
+constexpr int kSmallPer = 16;  // entries each thread loads per round of the rank pass
+// LDS prefix: the first kSmallP positions of every man's list with their ranks,
+// filled from the LDS rank rows during the rank pass.  Nearly all chain steps of
+// uniform lists fall inside it and then read no global memory at all.
+constexpr int kSmallP = 32;
+
+// The instance's women's rank rows, as many whole rows per round as
+// BLOCK * kSmallPer entries hold.  Each thread issues all of its loads of a
+// round before the LDS staging is cleared, so a round costs one memory round
+// trip.  `buf` holds BLOCK * kSmallPer int16 of LDS.  Returns this thread's
+// error bits.
+// `lpfx` (n * kSmallP words of LDS) must hold (w << 16) | 0xFFFF for each
+// prefix entry (kLdsPfxBad outside the list or for an entry outside [0, n)); the
+// rank of every entry whose woman's row passes through `buf` is filled in.
+template <int BLOCK>
+__device__ int wg_rank_small(const int32_t* __restrict__ pw, int16_t* __restrict__ rank, int n, int16_t* buf,
+                             uint32_t* lpfx) {
+  constexpr int kRound = BLOCK * kSmallPer;
+  const int rows_per = kRound / n;  // n <= kSmallN < kRound
+  const int npfx = n * kSmallP;
+  int bad = 0;
+  for (int r0 = 0; r0 < n; r0 += rows_per) {
+    const int cnt = min(rows_per, n - r0) * n;
+    const int32_t* src = pw + (size_t)r0 * n;
+    int v[kSmallPer];
+#pragma unroll
+    for (int j = 0; j < kSmallPer; ++j) {
+      const int idx = (int)threadIdx.x + j * BLOCK;
+      v[j] = idx < cnt ? src[idx] : -1;
+    }
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) buf[i] = -1;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSmallPer; ++j) {
+      const int idx = (int)threadIdx.x + j * BLOCK;
+      if (idx < cnt) {
+        const int row = idx / n, col = idx - row * n;
+        if ((unsigned)v[j] < (unsigned)n) buf[row * n + v[j]] = (int16_t)col;
+        else bad |= kErrEntryRange;
+      }
+    }
+    __syncthreads();
+    const int nr = cnt / n;
+    for (int i = threadIdx.x; i < npfx; i += BLOCK) {
+      const uint32_t e = lpfx[i];
+      if (e == kLdsPfxBad) continue;
+      const int w = (int)(e >> 16);
+      if (w >= r0 && w < r0 + nr) {
+        const int16_t x = buf[(w - r0) * n + i / kSmallP];
+        if (x >= 0) lpfx[i] = (e & 0xFFFF0000u) | (uint32_t)(uint16_t)x;
+      }
+    }
+    int16_t* dst = rank + (size_t)r0 * n;
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+      const int16_t x = buf[i];
+      if (x < 0) bad |= kErrNotPerm;
+      dst[i] = x;
+    }
+    __syncthreads();  // buf is cleared again by the next round
+  }
+  return bad;
+}
+
+// One workgroup per request of at most kSmallN men: descriptor from pinned host
+// memory, rank rows, chains (16 lanes per chain), matching, proposal count and
+// error bits into pinned host memory, then the completion flag as in
+// k_serve_chains.
+template <int BLOCK, int G>
+__global__ __launch_bounds__(BLOCK, 2) void k_serve_small(const ServeReq* __restrict__ host_reqs, int count,
+                                                          unsigned long long* h_props, int* h_errs, int* done,
+                                                          unsigned* h_flag, unsigned seq) {
+  static_assert(sizeof(ServeReq) % 4 == 0 && sizeof(ServeReq) / 4 <= 64, "descriptor words");
+  constexpr int kDescWords = (int)(sizeof(ServeReq) / 4);
+  __shared__ unsigned long long partner[kSmallN];
+  __shared__ int16_t buf[BLOCK * kSmallPer];
+  __shared__ uint32_t lpfx[kSmallN * kSmallP];
+  __shared__ int desc[kDescWords];
+  __shared__ int cursor, sbad;
+  __shared__ unsigned long long red[BLOCK / 64];
+  const int i = blockIdx.x;
+  if (threadIdx.x < kDescWords) desc[threadIdx.x] = reinterpret_cast<const int*>(host_reqs + i)[threadIdx.x];
+  if (threadIdx.x == 0) sbad = 0;
+  __syncthreads();
+  const ServeReq q = *reinterpret_cast<const ServeReq*>(desc);
+  int bad = 0;
+  if (q.n <= kSmallN && q.n > 0) {
+    const int n = q.n;
+    // The men's list prefixes (women only; ranks come with the rank rows).
+    constexpr int kPer = kSmallN * kSmallP / BLOCK;
+    int wv[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int idx = (int)threadIdx.x + j * BLOCK, m = idx / kSmallP, k = idx % kSmallP;
+      wv[j] = m < n && k < n ? q.pm[(size_t)m * n + k] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int idx = (int)threadIdx.x + j * BLOCK;
+      lpfx[idx] = (unsigned)wv[j] < (unsigned)n ? ((uint32_t)wv[j] << 16) | 0xFFFFu : kLdsPfxBad;
+    }
+    __syncthreads();
+    bad = wg_rank_small<BLOCK>(q.pw, q.rank, n, buf, lpfx);
+    InstanceRef ref{q.pm, q.pw, q.rank, q.match, n};
+    unsigned long long p = 0;
+    bad |= wg_chains_g<BLOCK, G, 8, false, kSmallP>(ref, partner, &cursor, red, &p, lpfx);
+    if (threadIdx.x == 0) h_props[i] = p;
+  } else if (threadIdx.x == 0) {
+    h_props[i] = 0;
+    bad = kErrEntryRange;  // the host sends only 1 <= n <= kSmallN here
+  }
+  if (bad) atomicOr(&sbad, bad);
+  __syncthreads();
+  if (threadIdx.x == 0) h_errs[i] = sbad;
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(done, 1) == count - 1) {
+    *done = 0;
+    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // First-fit arena over one device allocation, 256-byte granules.
@@ -124,6 +288,7 @@ struct smp_server {
     const int32_t* d_pw = nullptr;
     size_t off = 0, bytes = 0;
     bool has_block = false;
+    bool to_worker = false;  // solved by a worker's single-instance pipeline
     State state = kPending;
     int status = SMP_OK;
     std::string error;
@@ -140,10 +305,18 @@ struct smp_server {
     pactae::ServeReq* h_reqs = nullptr;  // pinned
     pactae::ServeReq* d_reqs = nullptr;
     int32_t* h_match = nullptr;          // pinned results, max_batch * max_n
-    unsigned long long* d_props = nullptr;
-    int* d_errs = nullptr;
-    unsigned long long* h_props = nullptr;
-    int* h_errs = nullptr;
+    int* d_errs = nullptr;               // the rank pass's error bits per request
+    unsigned long long* h_props = nullptr;  // pinned, written by k_serve_chains
+    int* h_errs = nullptr;                  // pinned, written by k_serve_chains
+    unsigned* h_flag = nullptr;             // pinned: seq of the last finished launch
+    int* d_done = nullptr;                  // workgroups of the launch finished
+    // Device-visible addresses of the pinned buffers.
+    pactae::ServeReq* h_reqs_dev = nullptr;
+    int32_t* h_match_dev = nullptr;
+    unsigned long long* h_props_dev = nullptr;
+    int* h_errs_dev = nullptr;
+    unsigned* h_flag_dev = nullptr;
+    unsigned seq = 0;  // sequence number of the launch in flight
   };
 
   smp_server_opts opts{};
@@ -169,6 +342,7 @@ struct smp_server {
   std::thread worker;
   smp_server_stats stats{};
   int inflight = 0;
+  unsigned lane_polls = 0;  // flag checks of in-flight lanes (every 256th also queries the event)
   // Work counters read without the lock by threads that spin briefly for new
   // work before they block (a wake-up through a condition variable costs tens
   // of microseconds, a visible share of a small request's latency).
@@ -183,12 +357,26 @@ struct smp_server {
   }
 
   bool is_large(const Request& r) const { return opts.mode == SMP_SERVE_BATCHED && r.n >= opts.large_n; }
+  int busy_workers = 0;  // workers solving a request (with mu held)
+  // Whether a request goes to the workers (with mu held): every large request,
+  // and with small_to_idle_worker a small one while a worker is idle and no
+  // request waits for the workers, so that at low load it gets the whole GPU's
+  // pipeline (whose host tail walks the last chains far faster than one
+  // workgroup does) and under load the lanes take it.
+  bool route_to_worker(const Request& r) const {
+    if (opts.mode != SMP_SERVE_BATCHED || large_workers.empty()) return false;
+    if (is_large(r)) return true;
+    return opts.small_to_idle_worker > 0 &&
+           (int)large_workers.size() - busy_workers > (int)ready_large.size();
+  }
 
   // Arena bytes of the int16 rank matrix (lane requests only: the
   // single-instance pipelines of the large-request workers and of plain mode
   // have their own) and of the list copies (host-submitted requests).
   size_t rank_bytes(const Request& r) const {
-    return opts.mode != SMP_SERVE_BATCHED || is_large(r) ? 0 : (((size_t)r.n * r.n * 2 + 255) & ~size_t(255));
+    return opts.mode != SMP_SERVE_BATCHED || is_large(r) || r.to_worker
+               ? 0
+               : (((size_t)r.n * r.n * 2 + 255) & ~size_t(255));
   }
   size_t footprint(const Request& r) const {
     size_t b = rank_bytes(r);
@@ -227,10 +415,19 @@ struct smp_server {
         SMP_HIP_TRY(hipHostMalloc(&L.h_match, mb * opts.max_n * sizeof(int32_t), hipHostMallocDefault));
         SMP_HIP_TRY(hipHostMalloc(&L.h_props, mb * sizeof(unsigned long long), hipHostMallocDefault));
         SMP_HIP_TRY(hipHostMalloc(&L.h_errs, mb * sizeof(int), hipHostMallocDefault));
+        SMP_HIP_TRY(hipHostMalloc(&L.h_flag, sizeof(unsigned), hipHostMallocDefault));
+        *L.h_flag = 0;
         SMP_HIP_TRY(hipMalloc(&L.d_reqs, mb * sizeof(pactae::ServeReq)));
-        SMP_HIP_TRY(hipMalloc(&L.d_props, mb * sizeof(unsigned long long)));
         SMP_HIP_TRY(hipMalloc(&L.d_errs, mb * sizeof(int)));
+        SMP_HIP_TRY(hipMalloc(&L.d_done, sizeof(int)));
+        SMP_HIP_TRY(hipMemset(L.d_done, 0, sizeof(int)));
+        SMP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&L.h_reqs_dev), L.h_reqs, 0));
+        SMP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&L.h_match_dev), L.h_match, 0));
+        SMP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&L.h_props_dev), L.h_props, 0));
+        SMP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&L.h_errs_dev), L.h_errs, 0));
+        SMP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&L.h_flag_dev), L.h_flag, 0));
       }
+      SMP_HIP_TRY(hipDeviceSynchronize());  // the d_done clears
       const int lds = (int)std::min<size_t>((size_t)opts.max_n * 8, 160 * 1024);
       SMP_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&pactae::k_serve_chains<256>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
@@ -306,7 +503,9 @@ struct smp_server {
       Request* r = ready_large.front();
       ready_large.pop_front();
       large_queued.fetch_sub(1, std::memory_order_relaxed);
+      ++busy_workers;
       solve_plain(lk, r, -1, eng);
+      --busy_workers;
       ++stats.large_solved;
       cv_work.notify_one();  // its arena block is free for admission
     }
@@ -332,9 +531,10 @@ struct smp_server {
       (void)hipHostFree(L.h_match);
       (void)hipHostFree(L.h_props);
       (void)hipHostFree(L.h_errs);
+      (void)hipHostFree(L.h_flag);
       (void)hipFree(L.d_reqs);
-      (void)hipFree(L.d_props);
       (void)hipFree(L.d_errs);
+      (void)hipFree(L.d_done);
       if (L.done) (void)hipEventDestroy(L.done);
       if (L.stream) (void)hipStreamDestroy(L.stream);
     }
@@ -377,6 +577,7 @@ struct smp_server {
   hipError_t admit(std::unique_lock<std::mutex>& lk, bool* progressed) {
     while (!pending.empty()) {
       Request* r = pending.front();
+      r->to_worker = route_to_worker(*r);
       r->bytes = footprint(*r);
       if (!arena.alloc(r->bytes, &r->off)) break;
       r->has_block = true;
@@ -403,7 +604,7 @@ struct smp_server {
         std::vector<int32_t>().swap(r->host_pw);
       }
       r->state = kReady;
-      if (is_large(*r) && !large_workers.empty()) {
+      if (r->to_worker) {
         push_large(r);
       } else {
         ready.push_back(r);
@@ -418,13 +619,12 @@ struct smp_server {
     cv_large.notify_all();  // the workers decide which of them takes it
   }
 
-  // A large request with its lists in HBM needs no arena room of its own (the
-  // worker's pipeline has its buffers), so when nothing waits ahead of it in
-  // arrival order it goes straight to the workers from the submitting thread.
+  // A request for the workers with its lists in HBM needs no arena room of its
+  // own (the worker's pipeline has its buffers), so when nothing waits ahead of
+  // it in arrival order it goes straight to them from the submitting thread.
   bool admit_large_now(Request* r) {  // with mu held
-    if (opts.mode != SMP_SERVE_BATCHED || !r->on_device || !is_large(*r) || large_workers.empty() ||
-        !pending.empty())
-      return false;
+    if (!r->on_device || !pending.empty() || !route_to_worker(*r)) return false;
+    r->to_worker = true;
     r->bytes = footprint(*r);
     if (!arena.alloc(r->bytes, &r->off)) return false;
     r->has_block = true;
@@ -447,46 +647,59 @@ struct smp_server {
       r->t_launch = now;
       r->batch_size = count;
       r->lane = lane_id;
-      int32_t* hm = L.h_match + (size_t)i * opts.max_n;
-      int32_t* dm = nullptr;
-      hipError_t e = hipHostGetDevicePointer(reinterpret_cast<void**>(&dm), hm, 0);
-      if (e != hipSuccess) return e;
+      int32_t* dm = L.h_match_dev + (size_t)i * opts.max_n;
       L.h_reqs[i] = pactae::ServeReq{r->d_pm, r->d_pw, reinterpret_cast<int16_t*>(d_arena + r->off), dm,
                                      r->n, rows};
       rows += r->n;
       bmax = std::max(bmax, r->n);
     }
-    hipError_t e = hipMemcpyAsync(L.d_reqs, L.h_reqs, count * sizeof(pactae::ServeReq),
-                                  hipMemcpyHostToDevice, L.stream);
-    if (e != hipSuccess) return e;
-    if ((e = pactae::launch_fill_i32(L.d_errs, count, 0, L.stream)) != hipSuccess) return e;
+    ++L.seq;
+    hipError_t e = hipSuccess;
+    if (bmax <= pactae::kSmallN) {
+      // Small requests only: one fused kernel, one workgroup each.
+      auto kern = pactae::k_serve_small<1024, 16>;
+      switch (opts.small_chain_lanes) {
+        case 2: kern = pactae::k_serve_small<1024, 2>; break;
+        case 4: kern = pactae::k_serve_small<1024, 4>; break;
+        case 8: kern = pactae::k_serve_small<1024, 8>; break;
+        default: break;
+      }
+      hipLaunchKernelGGL(kern, dim3(count), dim3(1024), 0, L.stream, L.h_reqs_dev, count, L.h_props_dev,
+                         L.h_errs_dev, L.d_done, L.h_flag_dev, L.seq);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = hipEventRecord(L.done, L.stream)) != hipSuccess) return e;
+      L.busy = true;
+      ++stats.launches;
+      return hipSuccess;
+    }
+    hipLaunchKernelGGL(pactae::k_serve_prep, dim3(1), dim3(256), 0, L.stream, L.h_reqs_dev, L.d_reqs, count,
+                       L.d_errs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = pactae::launch_rank_rows16(L.stream, L.d_reqs, count, static_cast<const int*>(nullptr), rows,
                                         bmax, L.d_errs, 1, dev.cus, dev.max_lds)) != hipSuccess)
       return e;
     const size_t lds = (size_t)bmax * 8;
     if (opts.block_threads == 256)
       hipLaunchKernelGGL(pactae::k_serve_chains<256>, dim3(count), dim3(256), lds, L.stream, L.d_reqs,
-                         count, L.d_props, L.d_errs);
+                         count, L.d_errs, L.h_props_dev, L.h_errs_dev, L.d_done, L.h_flag_dev, L.seq);
     else if (opts.block_threads == 512)
       hipLaunchKernelGGL(pactae::k_serve_chains<512>, dim3(count), dim3(512), lds, L.stream, L.d_reqs,
-                         count, L.d_props, L.d_errs);
+                         count, L.d_errs, L.h_props_dev, L.h_errs_dev, L.d_done, L.h_flag_dev, L.seq);
     else
       hipLaunchKernelGGL(pactae::k_serve_chains<1024>, dim3(count), dim3(1024), lds, L.stream, L.d_reqs,
-                         count, L.d_props, L.d_errs);
+                         count, L.d_errs, L.h_props_dev, L.h_errs_dev, L.d_done, L.h_flag_dev, L.seq);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(L.h_props, L.d_props, count * sizeof(unsigned long long),
-                            hipMemcpyDeviceToHost, L.stream)) != hipSuccess)
-      return e;
-    if ((e = hipMemcpyAsync(L.h_errs, L.d_errs, count * sizeof(int), hipMemcpyDeviceToHost, L.stream)) !=
-        hipSuccess)
-      return e;
+    // The event only backs up the flag: it tells a failed launch or a device
+    // error (or a stream that drained without the flag) from one still running.
     if ((e = hipEventRecord(L.done, L.stream)) != hipSuccess) return e;
     L.busy = true;
     ++stats.launches;
     return hipSuccess;
   }
 
-  void complete(Lane& L) {  // with mu held; the lane's event has fired
+  bool lane_flag_seen(const Lane& L) const { return __atomic_load_n(L.h_flag, __ATOMIC_ACQUIRE) == L.seq; }
+
+  void complete(Lane& L) {  // with mu held; the lane's flag holds its seq
     for (size_t i = 0; i < L.reqs.size(); ++i) {
       Request* r = L.reqs[i];
       const int32_t* hm = L.h_match + i * opts.max_n;
@@ -521,14 +734,24 @@ struct smp_server {
         int busy = 0;
         for (auto& L : lanes) {
           if (L.busy) {
-            hipError_t q = hipEventQuery(L.done);
-            if (q == hipSuccess) {
+            if (lane_flag_seen(L)) {
               complete(L);
               progressed = true;
-            } else if (q != hipErrorNotReady) {
-              fail_all(std::string("server: ") + hipGetErrorString(q));
-              break;
+              continue;
             }
+            // Not finished by the flag: ask the runtime now and then, so that a
+            // failed launch or a device error ends the wait.
+            if ((++lane_polls & 255) != 0) continue;
+            hipError_t q = hipEventQuery(L.done);
+            if (q == hipErrorNotReady) continue;
+            if (q == hipSuccess && lane_flag_seen(L)) {
+              complete(L);
+              progressed = true;
+              continue;
+            }
+            fail_all(std::string("server: lane launch ended without its results (") +
+                     (q == hipSuccess ? "no completion flag" : hipGetErrorString(q)) + ")");
+            break;
           }
         }
         if (hipError_t e = admit(lk, &progressed); e != hipSuccess) {
@@ -583,8 +806,10 @@ void smp_server_default_opts(smp_server_opts* o) {
   o->block_threads = 0;
   o->hbm_budget_bytes = 1ll << 30;
   o->large_n = 512;
-  o->large_workers = 3;
+  o->large_workers = 2;
   o->large_spill_us = 0;
+  o->small_chain_lanes = 0;
+  o->small_to_idle_worker = -1;
 }
 
 int smp_server_create(const smp_server_opts* opts, smp_server** out) {
@@ -602,7 +827,11 @@ int smp_server_create(const smp_server_opts* opts, smp_server** out) {
   if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = 1ll << 30;
   if (o.block_threads != 256 && o.block_threads != 512 && o.block_threads != 1024) o.block_threads = 1024;
   if (o.large_n <= 0) o.large_n = 512;
-  if (o.large_workers <= 0) o.large_workers = 3;
+  if (o.small_to_idle_worker < 0) o.small_to_idle_worker = 1;
+  if (o.small_chain_lanes != 2 && o.small_chain_lanes != 4 && o.small_chain_lanes != 8 &&
+      o.small_chain_lanes != 16)
+    o.small_chain_lanes = 16;
+  if (o.large_workers <= 0) o.large_workers = 2;
   if (o.large_workers > 8) {
     pactae::set_error("server: large_workers must be <= 8");
     return SMP_ERR_INVALID;

@@ -95,9 +95,15 @@ __device__ inline uint32_t rank_of(const int16_t* rk, int n, uint32_t w, uint32_
 // `partner` holds n words of LDS, `cursor` one int of LDS, `red` BLOCK/64 words.
 // The rank matrix must be complete and visible to the workgroup.  Thread 0
 // receives the proposal count in *props.  Returns this thread's error bits.
-template <int BLOCK, int G, int JMAX, bool kTransposed>
+// With P > 0, `lpfx` is an LDS prefix of every man's list: lpfx[m * P + k] =
+// (w << 16) | rank of m in w's list for positions k < P (kLdsPfxBad marks an
+// entry outside [0, n)); those positions are then read from LDS instead of from
+// the list and the rank matrix.
+constexpr uint32_t kLdsPfxBad = 0xFFFFFFFFu;
+template <int BLOCK, int G, int JMAX, bool kTransposed, int P = 0>
 __device__ int wg_chains_g(const InstanceRef& d, unsigned long long* partner, int* cursor,
-                           unsigned long long* red, unsigned long long* props) {
+                           unsigned long long* red, unsigned long long* props,
+                           const uint32_t* lpfx = nullptr) {
   static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "group size");
   static_assert(JMAX >= 1 && JMAX <= 8, "window");
   constexpr int NW = BLOCK / 64;
@@ -139,22 +145,32 @@ __device__ int wg_chains_g(const InstanceRef& d, unsigned long long* partner, in
     unsigned long long mf = 0;
     if (!done) {
       int wv[JMAX];
+      uint32_t rv[JMAX];
+      bool lds_rank[JMAX];
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) {
         const int pos = k + gl + G * j;
         wv[j] = -1;
+        rv[j] = 0;
+        lds_rank[j] = false;
         if (j < J && pos < n) {
-          wv[j] = pm[(size_t)m * n + pos];
+          if (P > 0 && pos < P) {
+            const uint32_t e = lpfx[m * P + pos];
+            wv[j] = e == kLdsPfxBad ? n : (int)(e >> 16);
+            rv[j] = e & 0xFFFFu;
+            lds_rank[j] = true;
+          } else {
+            wv[j] = pm[(size_t)m * n + pos];
+          }
           if ((unsigned)wv[j] >= (unsigned)n) {
             bad |= kErrEntryRange;
             wv[j] = -1;
           }
         }
       }
-      uint32_t rv[JMAX];
 #pragma unroll
       for (int j = 0; j < JMAX; ++j)
-        rv[j] = wv[j] >= 0 ? rank_of<kTransposed>(rk, n, (uint32_t)wv[j], (uint32_t)m) : 0u;
+        if (wv[j] >= 0 && !lds_rank[j]) rv[j] = rank_of<kTransposed>(rk, n, (uint32_t)wv[j], (uint32_t)m);
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) {
         if (wv[j] >= 0 && jf == 8) {

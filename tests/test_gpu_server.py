@@ -55,20 +55,37 @@ def test_device_resident_requests_and_admission(gpu, oracle):
 
 
 @pytest.mark.parametrize("large_n", [300, 100000])
-def test_large_request_worker(gpu, oracle, large_n):
-    """Requests at or above large_n go to the single-instance pipeline worker
-    (lane -1); the rest to the lanes; every result matches the oracle."""
+@pytest.mark.parametrize("idle", [0, 1])
+def test_large_request_worker(gpu, oracle, large_n, idle):
+    """Requests at or above large_n go to the single-instance pipeline workers
+    (lane -1).  The rest go to the lanes, or with small_to_idle_worker to a
+    worker that is idle while none waits for the workers.  A large_n above
+    max_n means no workers at all.  Every result matches the oracle."""
     smp = gpu
     reqs = requests(smp, 30, seed=13, n_min=64, n_max=900)
     want = [oracle.gs_seq(n, pm, pw)[0] for n, (pm, pw) in reqs]
-    with smp.SmpServer(lanes=2, max_batch=8, max_n=1024, large_n=large_n) as srv:
+    dev = [(smp.DeviceArray(pm), smp.DeviceArray(pw)) for _, (pm, pw) in reqs]
+    with smp.SmpServer(lanes=2, max_batch=8, max_n=1024, large_n=large_n, small_to_idle_worker=idle) as srv:
         tickets = [srv.submit(pm, pw, n) for n, (pm, pw) in reqs]
         out = [srv.wait(t, timeout_ms=60000) for t in tickets]
         st = srv.stats()
-    for (n, _), (got, info), w in zip(reqs, out, want):
+        # One at a time, lists in HBM: with small_to_idle_worker every request
+        # finds an idle worker; without it only the large ones go there.
+        alone = [srv.wait(srv.submit(dm, dw, n), timeout_ms=60000) for (n, _), (dm, dw) in zip(reqs, dev)]
+    for (n, _), (got, info), (got1, info1), w in zip(reqs, out, alone, want):
         assert np.array_equal(got, w), n
-        assert (info["lane"] == -1) == (n >= large_n), (n, info["lane"])
-    assert st["large_solved"] == sum(n >= large_n for n, _ in reqs)
+        assert np.array_equal(got1, w), n
+        if n >= large_n:
+            assert info["lane"] == -1 and info1["lane"] == -1, (n, info["lane"])
+        elif not idle or large_n > 1024:
+            assert info["lane"] >= 0 and info1["lane"] >= 0, (n, info["lane"])
+        else:
+            assert info1["lane"] == -1, (n, info1["lane"])
+    if not idle or large_n > 1024:
+        assert st["large_solved"] == sum(n >= large_n for n, _ in reqs)
+    for dm, dw in dev:
+        dm.free()
+        dw.free()
 
 
 def test_plain_mode_matches(gpu, oracle):
@@ -110,3 +127,22 @@ def test_invalid_request_fails_alone(gpu, oracle):
             srv.wait(t_bad, timeout_ms=60000)
         got, _ = srv.wait(t_ok, timeout_ms=60000)
     assert np.array_equal(got, oracle.gs_seq(n, pm, pw)[0])
+
+
+@pytest.mark.parametrize("max_batch", [1, 8])
+def test_lane_kernels_at_the_small_request_bound(gpu, oracle, max_batch):
+    """Lane launches whose requests all have n <= 512 run the fused one-kernel
+    solve; a launch holding a larger request runs the rank pass and the chain
+    kernel.  Sizes on both sides of the bound, every kind, all on the lanes."""
+    smp = gpu
+    sizes = [1, 2, 3, 64, 511, 512, 513, 700, 512, 40]
+    kinds = [0, 1, 3, 0, 3, 1, 0, 3, 0, 1]
+    reqs = [(n, smp.generate(k, n, seed=4242 + i)) for i, (n, k) in enumerate(zip(sizes, kinds))]
+    with smp.SmpServer(lanes=2, max_batch=max_batch, max_n=1024, large_n=100000, small_to_idle_worker=0) as srv:
+        tickets = [srv.submit(pm, pw, n) for n, (pm, pw) in reqs]
+        out = [srv.wait(t, timeout_ms=60000) for t in tickets]
+    for (n, (pm, pw)), (got, info) in zip(reqs, out):
+        want, t = oracle.gs_seq(n, pm, pw)
+        assert np.array_equal(got, want), n
+        assert info["proposals"] == t["proposals"], n
+        assert info["lane"] >= 0

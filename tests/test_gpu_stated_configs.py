@@ -108,7 +108,9 @@ def test_serving_poisson_stream_to_8192(gpu, oracle, split):
     picks = np.concatenate([np.arange(len(pool)), rng.integers(0, len(pool), count - len(pool))])
     rng.shuffle(picks)
     arrivals = np.cumsum(rng.exponential(1.0 / 1000.0, size=count))
-    opts = {} if split == "default" else {"large_n": 100000}
+    # all_lanes: no request goes to the single-instance workers, so the lane
+    # kernels see every size up to 8,192
+    opts = {} if split == "default" else {"large_n": 100000, "small_to_idle_worker": 0}
     print(f"serving {count} requests, split {split}", flush=True)
     got, infos, st = _serve_stream(smp, pool, picks, arrivals, **opts)
     for i, p in enumerate(picks):
@@ -119,7 +121,8 @@ def test_serving_poisson_stream_to_8192(gpu, oracle, split):
         assert st["large_solved"] == 0
         assert all(x["lane"] >= 0 for x in big)
     else:
-        assert st["large_solved"] == sum(int(sizes[p]) >= 512 for p in picks)
+        # every large request, and small ones that found a worker idle
+        assert st["large_solved"] >= sum(int(sizes[p]) >= 512 for p in picks)
         assert all(x["lane"] == -1 for x in big)
     assert st["completed"] == count and st["arena_bytes"] == 0
     for _, dm, dw in pool:
