@@ -267,8 +267,29 @@ __device__ int wg_chains(const InstanceRef& d, unsigned long long* partner, int*
 struct PoolSlot {
   int state, n, cursor, active;
   long long off, moff;  // element offset of the lists / of the matching
-  int master, pad;      // 1: every man holds the same list; the women propose
+  int master;           // 1: every man holds the same list; the women propose
+  int reach16;          // running mean, x16, of how far into a step's window its
+                        // candidate lay (a step without one counts its whole
+                        // window): sets the window a chain starts a step with
 };
+// The window (list positions) a pool chain scans at the start of a man's walk
+// and after each claim: about twice the instance's running mean reach, 16 to 64.
+// Independent lists find a candidate within a few positions, and scanning 64
+// then fetched 4x the rank lines it used (each lookup is a line of its own);
+// nearly shared lists refuse dozens of positions in a row and keep 64.
+__device__ inline int pool_window0(int reach16) {
+  const int want = (2 * reach16 + 15) >> 4;  // 2 x mean reach, rounded up
+  if (want <= 16) return 16;
+  if (want <= 32) return 32;
+  return 64;
+}
+__device__ inline void pool_reach_note(PoolSlot& sl, int reach) {
+  const int r = __hip_atomic_load(&sl.reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  // r += (16 * reach - r) / 8; racing groups may lose an update, which only
+  // delays the mean.
+  __hip_atomic_store(&sl.reach16, r + ((16 * reach - r) >> 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+constexpr int kReach16Init = 8 * 16;  // a first window of 16 positions
 enum { kSlotFree = 0, kSlotOpening = 1, kSlotOpen = 2, kSlotFinal = 3 };
 
 __device__ inline int lds_ld(const int* p) {
@@ -381,6 +402,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
         sl.off = d.off;
         sl.moff = d.moff;
         sl.master = master_flags && d.n >= 2 && !master_flags[d.idx];
+        sl.reach16 = kReach16Init;
         sl.state = kSlotOpen;
       } else {
         sl.n = 0;
@@ -448,13 +470,14 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       sl.off = d.off;
       sl.moff = d.moff;
       sl.master = master_flags && d.n >= 2 && !master_flags[d.idx];
+      sl.reach16 = kReach16Init;
       lds_st(&sl.cursor, 0);
       lds_st(&sl.state, kSlotOpen);
     }
   };
 
   const int start = (wave * (64 / G) + lane / G) & (NS - 1);
-  int s = 0, n = 0, m = -1, k = 0, J = 1;
+  int s = 0, n = 0, m = -1, k = 0, W = 16;  // W: positions scanned by the next step
   long long off = 0, moff = 0;
   bool exited = false, mst = false;
   uint32_t rprop = 0;  // master-list slot: the proposing woman's rank, Linv[m]
@@ -474,7 +497,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       s = a_s;
       m = a_v;
       k = 0;
-      J = 1;
+      W = pool_window0(__hip_atomic_load(&slots[s].reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
       n = slots[s].n;
       off = slots[s].off;
       moff = slots[s].moff;
@@ -502,7 +525,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       for (int j = 0; j < JMAX; ++j) {
         const int pos = k + gl + G * j;
         wv[j] = -1;
-        if (j < J && pos < n) {
+        if (gl + G * j < W && pos < n) {
           wv[j] = pm[(size_t)m * n + pos];
           if ((unsigned)wv[j] >= (unsigned)n) {
             bad |= kErrEntryRange;
@@ -538,9 +561,10 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
     old = __shfl(old, src);
     const int won = __shfl((int)(owner && old > mf), src);
     if (scan) {
+      if (gl == 0) pool_reach_note(slots[s], key >= 8 * G ? W : key + 1);
       if (key >= 8 * G) {
-        k += G * J;
-        J = min(2 * J, JMAX);
+        k += W;
+        W = min(2 * W, G * JMAX);
       } else if (won) {
         if (old == kEmpty) {
           ended = true;
@@ -549,10 +573,10 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
           k = (int)pw_pos(old) + 1;
           if (mst) rprop = (uint32_t)linv_all[moff + m];
         }
-        J = 1;
+        W = pool_window0(__hip_atomic_load(&slots[s].reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
       } else {
         k += key + 1;
-        J = 1;
+        W = pool_window0(__hip_atomic_load(&slots[s].reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
       }
     }
     if (ended) {
