@@ -76,7 +76,11 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch_pool(const BatchDesc* __rest
   extern __shared__ unsigned long long partner[];
   __shared__ PoolSlot slots[NS];
   __shared__ int qdone;
-  const int bad = wg_pool_chains<BLOCK, G, 8, NS, TR>(desc, count, next_inst, pm_all, rank_all, match_all,
+  // Windows of up to 128 positions (8 for G <= 16): a longer window was used
+  // only by walks that refuse hundreds of positions in a row, and each extra
+  // window slot costs every step its unrolled instructions.
+  constexpr int kJmax = G >= 64 ? 2 : G == 32 ? 4 : 8;
+  const int bad = wg_pool_chains<BLOCK, G, kJmax, NS, TR>(desc, count, next_inst, pm_all, rank_all, match_all,
                                                        proposals, max_n, partner, slots, &qdone, pw_all,
                                                        master_flags, linv_all);
   if (bad) atomicOr(error, bad);

@@ -484,9 +484,17 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
   for (;;) {
     int act = kActNone, a_s = 0, a_v = 0;
     if (!exited && m < 0 && gl == 0) act = pool_acquire<NS>(slots, qdone, next_inst, count, start, &a_s, &a_v);
-    act = __shfl(act, gbase);
-    a_s = __shfl(a_s, gbase);
-    a_v = __shfl(a_v, gbase);
+    if constexpr (G == 64) {
+      // One chain per wave: the leader's values are wave-uniform; read them into
+      // scalar registers instead of an exchange through the LDS crossbar.
+      act = __builtin_amdgcn_readfirstlane(act);
+      a_s = __builtin_amdgcn_readfirstlane(a_s);
+      a_v = __builtin_amdgcn_readfirstlane(a_v);
+    } else {
+      act = __shfl(act, gbase);
+      a_s = __shfl(a_s, gbase);
+      a_v = __shfl(a_v, gbase);
+    }
     if (act == kActInit) {
       open(a_s, a_v);
     } else if (act == kActFinal) {
@@ -514,6 +522,10 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
     }
     const bool scan = live && !ended;
     unsigned long long* pw_ = partner + (size_t)s * max_n;
+    // The instance's running mean reach, read now so that the window after this
+    // step's claim does not wait for an LDS load.
+    const int reach16 =
+        scan ? __hip_atomic_load(&slots[s].reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
     int jf = 8;
     uint32_t wf = 0;
     unsigned long long mf = 0;
@@ -552,14 +564,39 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       }
     }
     int key = jf * G + gl;
+    if constexpr (G == 64) {
+      // One chain per wave: the first candidate in list order from ballots, one
+      // per window slot, instead of six rounds of lane exchanges.
+      key = 8 * G;
 #pragma unroll
-    for (int o = 1; o < G; o <<= 1) key = min(key, __shfl_xor(key, o));
+      for (int j = 0; j < JMAX; ++j) {
+        const unsigned long long b = __ballot(jf == j);
+        if (b) {
+          key = j * G + __ffsll((long long)b) - 1;
+          break;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int o = 1; o < G; o <<= 1) key = min(key, __shfl_xor(key, o));
+    }
     const bool owner = scan && jf < 8 && key == jf * G + gl;
     unsigned long long old = 0;
     if (owner) old = atomicMin(&pw_[wf], mf);
     const int src = gbase + (key & (G - 1));
-    old = __shfl(old, src);
-    const int won = __shfl((int)(owner && old > mf), src);
+    int won;
+    if constexpr (G == 64) {
+      // key came from ballots, so src is wave-uniform: read the owner's words
+      // with readlane.
+      const unsigned long long mo = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(mf >> 32), src) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)mf, src);
+      old = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(old >> 32), src) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane((int)old, src);
+      won = key < 8 * G && old > mo;
+    } else {
+      old = __shfl(old, src);
+      won = __shfl((int)(owner && old > mf), src);
+    }
     if (scan) {
       if (gl == 0) pool_reach_note(slots[s], key >= 8 * G ? W : key + 1);
       if (key >= 8 * G) {
@@ -573,17 +610,18 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
           k = (int)pw_pos(old) + 1;
           if (mst) rprop = (uint32_t)linv_all[moff + m];
         }
-        W = pool_window0(__hip_atomic_load(&slots[s].reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        W = pool_window0(reach16);
       } else {
         k += key + 1;
-        W = pool_window0(__hip_atomic_load(&slots[s].reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        W = pool_window0(reach16);
       }
     }
     if (ended) {
       m = -1;
       int fin = 0;
       if (gl == 0) fin = pool_leave(slots[s]) ? 1 : 0;
-      fin = __shfl(fin, gbase);
+      if constexpr (G == 64) fin = __builtin_amdgcn_readfirstlane(fin);
+      else fin = __shfl(fin, gbase);
       if (fin) finalize(s);
     }
   }
