@@ -283,11 +283,10 @@ __device__ inline int pool_window0(int reach16) {
   if (want <= 32) return 32;
   return 64;
 }
-__device__ inline void pool_reach_note(PoolSlot& sl, int reach) {
-  const int r = __hip_atomic_load(&sl.reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  // r += (16 * reach - r) / 8; racing groups may lose an update, which only
-  // delays the mean.
-  __hip_atomic_store(&sl.reach16, r + ((16 * reach - r) >> 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// r16 is the mean the caller read at the start of its step: r16 += (16 * reach -
+// r16) / 8.  Racing groups may lose an update, which only delays the mean.
+__device__ inline void pool_reach_note(PoolSlot& sl, int r16, int reach) {
+  __hip_atomic_store(&sl.reach16, r16 + ((16 * reach - r16) >> 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 constexpr int kReach16Init = 8 * 16;  // a first window of 16 positions
 enum { kSlotFree = 0, kSlotOpening = 1, kSlotOpen = 2, kSlotFinal = 3 };
@@ -598,7 +597,7 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
       won = __shfl((int)(owner && old > mf), src);
     }
     if (scan) {
-      if (gl == 0) pool_reach_note(slots[s], key >= 8 * G ? W : key + 1);
+      if (gl == 0) pool_reach_note(slots[s], reach16, key >= 8 * G ? W : key + 1);
       if (key >= 8 * G) {
         k += W;
         W = min(2 * W, G * JMAX);
