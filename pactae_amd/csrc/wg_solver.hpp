@@ -1,16 +1,19 @@
-// wg_solver.hpp — one workgroup solves one whole instance (device code).
+// wg_solver.hpp — device code shared by the batched and the serving paths.
 //
-// Used by the batched kernel and by the serving kernel.  Steps:
-//   1. women's rank rows (rank[w][pref_w[w][r]] = r, the F9RankMatrixInitKernel
-//      step, src/smp/kernel.cu:672-686) as int16, `rows` rows in flight, each row
-//      assembled in LDS by a team of BLOCK/rows threads, then written coalesced;
-//   2. proposal chains (the La3F9CoreKernel walk, src/smp/kernel.cu:633-670): each
-//      wave owns one free man, scans 64 list positions per step, ballots the lanes
-//      where he beats the current partner and claims with a 64-bit LDS atomicMin on
-//      the packed (rank, man, position) word; displaced men continue in the wave;
-//   3. the matching (PostProc, src/smp/smp_engine_bamboo.cu:299-303) and the
-//      proposal count (sum over women of position + 1).
-// LDS: partner words (8 * max_n) + row buffers (2 * rows * max_n) + 2 ints.
+//   wg_chains_g     one workgroup walks one instance's proposal chains (the
+//                   La3F9CoreKernel walk, src/smp/kernel.cu:633-670) over partner
+//                   words in LDS, claimed with a 64-bit LDS atomicMin on the
+//                   packed (rank, man, position) word; displaced men continue in
+//                   the same group of lanes; then the matching (PostProc,
+//                   src/smp/smp_engine_bamboo.cu:299-303) and the proposal count
+//                   (the serving lanes);
+//   wg_pool_chains  the same walk over a pool of instances open in LDS at once
+//                   (the batched pool mode), with a window per step that follows
+//                   each instance's running mean reach;
+//   k_rank_rows16   the women's rank rows (rank[w][pref_w[w][r]] = r, the
+//                   F9RankMatrixInitKernel step, src/smp/kernel.cu:672-686) of many
+//                   instances in one flat pass, one wave per row;
+//   k_rankT_tiles   the transposed rank matrix (men's rows) of many instances.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,47 +32,6 @@ struct InstanceRef {
   int32_t* match;
   int n;
 };
-
-inline size_t wg_lds_bytes(int rows, int max_n) {
-  return (size_t)max_n * 8 + (size_t)rows * max_n * 2 + 16;
-}
-
-// Step 1: the instance's women's rank rows, `rows` rows in flight.  All threads
-// of the workgroup call it together.  Returns this thread's error bits.
-template <int BLOCK>
-__device__ int wg_rank_rows(const InstanceRef& d, int16_t* rowbuf, int rows, int max_n) {
-  const int n = d.n;
-  int bad = 0;
-  const int team_size = BLOCK / rows;
-  const int team = threadIdx.x / team_size, tl = threadIdx.x - team * team_size;
-  int16_t* buf = rowbuf + (size_t)team * max_n;
-  for (int base = 0; base < n; base += rows) {
-    const int row = base + team;
-    if (row < n) {
-      for (int c = tl; c < n; c += team_size) buf[c] = -1;
-    }
-    __syncthreads();
-    if (row < n) {
-      const int32_t* src = d.pw + (size_t)row * n;
-      for (int r = tl; r < n; r += team_size) {
-        const int m = src[r];
-        if ((unsigned)m < (unsigned)n) buf[m] = (int16_t)r;
-        else bad |= kErrEntryRange;
-      }
-    }
-    __syncthreads();
-    if (row < n) {
-      int16_t* dst = d.rank + (size_t)row * n;
-      for (int c = tl; c < n; c += team_size) {
-        const int16_t v = buf[c];
-        if (v < 0) bad |= kErrNotPerm;
-        dst[c] = v;
-      }
-    }
-    __syncthreads();  // a team may span several waves: finish reading before the next reset
-  }
-  return bad;
-}
 
 // Rank of man m in woman w's list: rank[w][m] in the row-major (women's rows)
 // layout, rankT[m][w] in the transposed (men's rows) layout, where a chain's
@@ -235,13 +197,6 @@ __device__ int wg_chains_g(const InstanceRef& d, unsigned long long* partner, in
     *props = s;
   }
   return bad;
-}
-
-// The chain walk over the row-major rank matrix, one chain per wave.
-template <int BLOCK>
-__device__ int wg_chains(const InstanceRef& d, unsigned long long* partner, int* cursor,
-                         unsigned long long* red, unsigned long long* props) {
-  return wg_chains_g<BLOCK, 64, 1, false>(d, partner, cursor, red, props);
 }
 
 // ---------------------------------------------------------------------------
@@ -976,18 +931,6 @@ inline void rankT_tiles(const int* sizes, int count, int max_n, std::vector<int2
   out->clear();
   for (int i : order)
     for (int w0 = 0; w0 < sizes[i]; w0 += T) out->push_back(make_int2(i, w0));
-}
-
-// Whole instance in one workgroup: rank rows, then chains.  `ctl` points to two
-// ints of LDS (ctl[1] is the chain cursor).
-template <int BLOCK>
-__device__ int wg_solve(const InstanceRef& d, unsigned char* smem, int rows, int max_n, int* ctl,
-                        unsigned long long* red, unsigned long long* props) {
-  unsigned long long* partner = reinterpret_cast<unsigned long long*>(smem);
-  int16_t* rowbuf = reinterpret_cast<int16_t*>(partner + max_n);
-  int bad = wg_rank_rows<BLOCK>(d, rowbuf, rows, max_n);
-  bad |= wg_chains<BLOCK>(d, partner, &ctl[1], red, props);
-  return bad;
 }
 
 }  // namespace pactae

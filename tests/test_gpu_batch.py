@@ -84,13 +84,16 @@ def test_batch_pool_large_instances(gpu, oracle):
                 assert np.array_equal(got, want[i]), (lay, i, sizes[i], kinds[i])
 
 
-@pytest.mark.parametrize("layout", ["women_rows", "men_rows"])
-def test_batch_pool_master_list_instances(gpu, oracle, layout):
-    """All-congested batch (every man of an instance holds the same list): the pool
-    solves these from the women's side; results and the men's proposal counts
-    n(n+1)/2 must equal the oracle's, beside uniform instances in the same batch."""
+@pytest.mark.parametrize("config", ["fused", "pool_women_rows", "pool_men_rows"])
+def test_batch_master_list_instances(gpu, oracle, config):
+    """Half-congested batch (every man of an instance holds the same list): both
+    modes solve these from the women's side; results and the men's proposal
+    counts n(n+1)/2 must equal the oracle's, beside uniform instances in the same
+    batch."""
     smp = gpu
-    lay = {"women_rows": smp.SmpBatch.WOMEN_ROWS, "men_rows": smp.SmpBatch.MEN_ROWS}[layout]
+    opts = {"fused": {"mode": smp.SmpBatch.FUSED},
+            "pool_women_rows": {"mode": smp.SmpBatch.POOL, "rank_layout": smp.SmpBatch.WOMEN_ROWS},
+            "pool_men_rows": {"mode": smp.SmpBatch.POOL, "rank_layout": smp.SmpBatch.MEN_ROWS}}[config]
     sizes, kinds = smp.batch_plan(40, 1, 600, seed=21, mix=(0.5, 0.5, 0.0))
     pms, pws, want, props = [], [], [], 0
     for i, (n, k) in enumerate(zip(sizes, kinds)):
@@ -100,7 +103,7 @@ def test_batch_pool_master_list_instances(gpu, oracle, layout):
         pws.append(pw)
         want.append(m)
         props += t["proposals"]
-    with smp.SmpBatch(sizes, pms, pws, mode=smp.SmpBatch.POOL, rank_layout=lay) as b:
+    with smp.SmpBatch(sizes, pms, pws, **opts) as b:
         for _ in range(2):
             b.exec()
             for i, got in enumerate(b.result()):
@@ -116,5 +119,21 @@ def test_batch_invalid_lists_raise(gpu, mode):
     n = int(sizes[5])
     bad[5][2 * n] = bad[5][2 * n + 1]  # a woman's row that is not a permutation
     with smp.SmpBatch(sizes, pms, bad, mode=mode, use_graph=0) as b:
+        with pytest.raises(smp.SmpError):
+            b.exec()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_invalid_shared_mens_list_raises(gpu, mode):
+    """Every man of one instance holds the same list, but that list repeats a
+    woman: the master-list path must reject it (its Linv finds the repeat)."""
+    smp = gpu
+    sizes, kinds, pms, pws = make_batch(smp, 12, 16, 300, seed=6)
+    n = int(sizes[3])
+    row = np.arange(n, dtype=np.int32)
+    row[1] = row[0]
+    pms = [pm.copy() for pm in pms]
+    pms[3] = np.tile(row, n)
+    with smp.SmpBatch(sizes, pms, pws, mode=mode, use_graph=0) as b:
         with pytest.raises(smp.SmpError):
             b.exec()
