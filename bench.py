@@ -387,18 +387,38 @@ def batched(smp, args):
             "batch_mode": {1: "fused", 2: "pool"}.get(best["mode"], best["mode"]),
             "group_lanes": best["group_lanes"], "pool_slots": best["pool_slots"],
             "rank_layout": {0: None, 1: "women_rows", 2: "men_rows"}.get(best["rank_layout"]),
-            "rank_pass_ms": round(best["rank_ms"], 3),
+            "rank_pass_ms": round(best["rank_ms"], 3) if best["mode"] == 2 else None,
             "roofline": batched_roofline(best, total),
             "bit_exact_batched_vs_looped": bool(same)}, (sizes, kinds, pm_all, pw_all, res_batched)
 
 
 def batched_roofline(prof, sum_n2):
-    """HBM roofline of the batched launch.  Pool mode: the flat rank pass reads
-    4 B per women's list entry and writes a 2-byte rank (6 B per element of every
-    instance), timed from the launch to its end (the master-list detection runs
-    beside it on a second stream).  The whole launch is also set against the
+    """HBM roofline of the batched launch.  The algorithmic bytes are the rank
+    rows': 4 B read per women's list entry and a 2-byte rank written (6 B per
+    element of every instance).  Pool mode: the flat rank pass alone, timed from
+    the launch to its end (the master-list detection runs beside it on a second
+    stream).  Fused mode: the one kernel, whose walk runs beside other
+    workgroups' rank rows; its measured HBM traffic comes from the committed
+    rocprofv3 counters of the same batch (FETCH_SIZE and WRITE_SIZE, scaled by
+    the copy-kernel calibration).  The whole launch is also set against the
     read floor of both lists (8 B per element) at the HBM peak."""
     out = {"bound": "hbm", "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    if prof["mode"] == 1 and prof["kernel_ms"] > 0:
+        b = 6 * sum_n2
+        a = b / (prof["kernel_ms"] * 1e-3) / 1e9
+        out.update({"kernel": f"k_batch_solve<{prof['block_threads']}> (per-workgroup rank rows and walk)",
+                    "algorithmic_bytes": b, "kernel_ms": round(prof["kernel_ms"], 4),
+                    "achieved": round(a, 1), "frac": round(a / PEAK_HBM_GBS, 4)})
+        path = os.path.join(ROOT, "profiles", "counters_k_batch_solve_mix4096.json")
+        if os.path.exists(path):
+            with open(path) as f:
+                c = json.load(f)
+            if c.get("sum_n2") == sum_n2 and c.get("kernel", "").startswith(f"k_batch_solve<{prof['block_threads']}>"):
+                t = int(c["fetch_size_kib_per_dispatch"] * 1024 * c["fetch_factor"]
+                        + c["write_size_kib_per_dispatch"] * 1024 * c["write_factor"])
+                ta = t / (prof["kernel_ms"] * 1e-3) / 1e9
+                out.update({"traffic": t, "traffic_achieved": round(ta, 1),
+                            "traffic_frac": round(ta / PEAK_HBM_GBS, 4)})
     if prof["mode"] == 2 and prof["rank_ms"] > 0:
         b = 6 * sum_n2
         a = b / (prof["rank_ms"] * 1e-3) / 1e9
