@@ -486,6 +486,14 @@ constexpr int kBinPad = kBinCounterStride;
 // gated: queued right behind the precheck, before the host has read it; the
 // kernels do nothing unless the precheck left the general path open (two men
 // share a first choice, and not all first choices are the same).
+// A 64-bit value of one lane, the lane index being wave-uniform (a ballot's
+// first set bit): two readlanes into scalar registers, where __shfl would go
+// through the LDS crossbar and sit on the chain's dependent path.
+__device__ inline unsigned long long lane_u64(unsigned long long v, int lane) {
+  return ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), lane) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
 __device__ inline bool general_path_open(const Scalars* sc) {
   return __hip_atomic_load(&sc->dup_first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 &&
          __hip_atomic_load(&sc->first_differs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
@@ -693,7 +701,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
           rs = __hip_atomic_load(&a.sc->restart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         deep_local = 0;
-        restart = __shfl(rs, 0) != 0;
+        restart = __builtin_amdgcn_readfirstlane(rs) != 0;
       }
     }
     if (restart) {
@@ -742,15 +750,15 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       if (lane == j) old = atomicMin(&a.partner[w], mine);
       // The man this attempt would displace, by the hint; his next segment is
       // loaded while the atomic is in flight.
-      const unsigned long long hj = __shfl(hint, j);
+      const unsigned long long hj = lane_u64(hint, j);
       int sm = -1, sk = -1;
       if (hj != kEmpty) {
         sm = (int)pw_man(hj);
         sk = (int)pw_pos(hj) + 1;
         if (sk + lane < K && sk + lane < n) spec_e = a.pr[pr_index(sm, sk + lane, n)];
       }
-      old = __shfl(old, j);
-      const unsigned long long mj = __shfl(mine, j);
+      old = lane_u64(old, j);
+      const unsigned long long mj = lane_u64(mine, j);
       ++n_atomics;
       if (old > mj) {
         if (old == kEmpty) {
@@ -776,7 +784,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     }
     if (poll) {
       since_poll = 0;
-      act = __shfl(act, 0);
+      act = __builtin_amdgcn_readfirstlane(act);
       if (act <= T) {
         if (lane == 0) {
           if (m >= 0) a.parked[atomicAdd(&a.sc->parked_count, 1)] = make_int2(m, k);
@@ -1300,16 +1308,16 @@ __device__ void master_walk(const int32_t* __restrict__ pref_w, const int32_t* _
       const int j = __ffsll((long long)cand) - 1;
       unsigned long long old = 0;
       if (lane == j) old = atomicMin(&partner_m[m], mine);
-      const unsigned long long hj = __shfl(hint, j);
+      const unsigned long long hj = lane_u64(hint, j);
       int sw = -1, sk = -1;
       if (hj != kEmpty) {
         sw = (int)pw_man(hj);
         sk = (int)pw_pos(hj) + 1;
         if ((unsigned)sw < (unsigned)n && sk + lane < n) spec_m = pref_w[(size_t)sw * n + sk + lane];
       }
-      old = __shfl(old, j);
+      old = lane_u64(old, j);
       ++n_atomics;
-      if (old > __shfl(mine, j)) {
+      if (old > lane_u64(mine, j)) {
         if (old == kEmpty) {
           w = -1;
         } else {
