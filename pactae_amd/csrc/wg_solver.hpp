@@ -583,6 +583,254 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
 }
 
 // ---------------------------------------------------------------------------
+// One workgroup solves one instance from its lists in global memory (the batched
+// fused mode and the serving lanes):
+//   1. master-list check: rows 1..4 against row 0 on their first 64 entries,
+//      then, for a candidate, every row (the workgroup's waves, stopping at the
+//      first difference);
+//   2. the women's rank rows (rank[w][pref_w[w][r]] = r, the F9RankMatrixInitKernel
+//      step, src/smp/kernel.cu:672-686): rounds of whole rows filling the LDS
+//      staging, every load of a round issued before the staging is cleared;
+//      written as int16 with plain stores, so the walk finds them in this XCD's
+//      L2 (a master-list instance only validates its rows);
+//   3. for a master-list instance, Linv[L[k]] = k in LDS;
+//   4. the walk (the La3F9CoreKernel proposal, src/smp/kernel.cu:633-670): one
+//      chain per wave, the pool walk's step (a window of about twice the
+//      instance's running mean reach, first candidate from ballots, claim with a
+//      64-bit LDS atomicMin, wave-uniform values through readlane); a master-list
+//      instance is walked from the women's side with the rank Linv[w] every man
+//      gives woman w;
+//   5. the matching (PostProc, src/smp/smp_engine_bamboo.cu:299-303) written to
+//      `match`, and this thread's share of the proposal count (the men's sum of
+//      list position + 1) in *props.
+// LDS: `partner` and `linv` hold n words each, `stage` BLOCK * fused_per(BLOCK)
+// entries (which must be >= n), `ctl` three ints.  All threads of the workgroup
+// call it together.  Returns this thread's error bits.
+__host__ __device__ constexpr int fused_per(int) { return 8; }  // 64 VGPRs with no spill
+inline size_t fused_lds(int block, int max_n) {
+  return (size_t)max_n * 12 + (size_t)block * fused_per(block) * 2;
+}
+
+template <int BLOCK>
+__device__ int wg_solve_instance(const int32_t* __restrict__ pm, const int32_t* __restrict__ pw,
+                                 int16_t* __restrict__ rank, int32_t* __restrict__ match, int n,
+                                 unsigned long long* partner, int32_t* linv, uint16_t* stage, int* ctl,
+                                 unsigned long long* props) {
+  constexpr int NW = BLOCK / 64;
+  constexpr int kPer = fused_per(BLOCK);
+  constexpr int kStage = BLOCK * kPer;
+  constexpr int G = 64, JMAX = 2;
+  int& ctl_cursor = ctl[0];
+  int& ctl_differs = ctl[1];
+  int& ctl_reach16 = ctl[2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int bad = 0;
+  for (int w = threadIdx.x; w < n; w += BLOCK) partner[w] = kEmpty;
+  if (threadIdx.x == 0) {
+    ctl_cursor = 0;
+    ctl_differs = n < 2 ? 1 : 0;
+    ctl_reach16 = kReach16Init;
+  }
+  __syncthreads();
+  // 1. Master-list check.
+  if (wave == 0 && n >= 2) {
+    bool diff = false;
+    for (int m = 1; m < min(n, 5) && !diff; ++m)
+      diff = __ballot(lane < n && pm[(size_t)m * n + lane] != pm[lane]) != 0;
+    if (diff && lane == 0) ctl_differs = 1;
+  }
+  __syncthreads();
+  if (!ctl_differs) {
+    for (int m = 1 + wave; m < n; m += NW) {
+      if (__hip_atomic_load(&ctl_differs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+      const int32_t* row = pm + (size_t)m * n;
+      bool diff = false;
+      for (int k0 = 0; k0 < n && !diff; k0 += 512) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = k0 + 64 * j + lane;
+          if (k < n && row[k] != pm[k]) diff = true;
+        }
+        diff = __ballot(diff) != 0;
+      }
+      if (diff) {
+        if (lane == 0) atomicOr(&ctl_differs, 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  const bool mst = __builtin_amdgcn_readfirstlane(ctl_differs) == 0;
+  // 2. Rank rows (validated always, written unless the instance is a master list).
+  const int rows_per = kStage / n;  // >= 1: the host keeps kStage >= max_n
+  // Entry e = threadIdx.x + j * BLOCK of a round lies in row e / n, column
+  // e % n; from one step to the next both advance by BLOCK / n and BLOCK % n.
+  const int step_r = BLOCK / n, step_c = BLOCK % n;
+  const int row0 = (int)threadIdx.x / n, col0 = (int)threadIdx.x - row0 * n;
+  for (int r0 = 0; r0 < n; r0 += rows_per) {
+    const int cnt = min(rows_per, n - r0) * n;
+    const int32_t* src = pw + (size_t)r0 * n;
+    int v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int e = (int)threadIdx.x + j * BLOCK;
+      v[j] = e < cnt ? src[e] : 0;
+    }
+    for (int e = threadIdx.x; e < cnt; e += BLOCK) stage[e] = 0xFFFFu;
+    __syncthreads();
+    int row = row0, col = col0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int e = (int)threadIdx.x + j * BLOCK;
+      if (e < cnt) {
+        if ((unsigned)v[j] < (unsigned)n) stage[row * n + v[j]] = (uint16_t)col;
+        else bad |= kErrEntryRange;
+      }
+      row += step_r;
+      col += step_c;
+      if (col >= n) col -= n, ++row;
+    }
+    __syncthreads();
+    int16_t* dst = rank + (size_t)r0 * n;
+    for (int e = threadIdx.x; e < cnt; e += BLOCK) {
+      const uint16_t x = stage[e];
+      if (x == 0xFFFFu) bad |= kErrNotPerm;
+      if (!mst) dst[e] = (int16_t)x;
+    }
+    __syncthreads();
+  }
+  // 3. Linv of the shared men's list L (row 0).
+  if (mst) {
+    for (int k = threadIdx.x; k < n; k += BLOCK) linv[k] = -1;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += BLOCK) {
+      const int w = pm[k];
+      if ((unsigned)w >= (unsigned)n) bad |= kErrEntryRange;
+      else if (atomicCAS(&linv[w], -1, k) != -1) bad |= kErrNotPerm;
+    }
+    __syncthreads();
+  }
+  // 4. The walk, one chain per wave.
+  {
+    const int32_t* P = mst ? pw : pm;  // the proposers' lists
+    int m = -1, k = 0, W = 16;
+    uint32_t rprop = 0;
+    for (;;) {
+      if (m < 0) {
+        int got = 0;
+        if (lane == 0) got = atomicAdd(&ctl_cursor, 1);
+        got = __builtin_amdgcn_readfirstlane(got);
+        if (got >= n) break;
+        m = got;
+        k = 0;
+        W = pool_window0(__hip_atomic_load(&ctl_reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (mst) rprop = (uint32_t)max(linv[m], 0);
+      }
+      if (k >= n) {  // rows that are not permutations
+        bad |= kErrListEnd;
+        m = -1;
+        continue;
+      }
+      const int reach16 = __hip_atomic_load(&ctl_reach16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      int wv[JMAX];
+      uint32_t rv[JMAX];
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j) {
+        const int pos = k + lane + G * j;
+        wv[j] = -1;
+        if (lane + G * j < W && pos < n) {
+          wv[j] = P[(size_t)m * n + pos];
+          if ((unsigned)wv[j] >= (unsigned)n) {
+            bad |= kErrEntryRange;
+            wv[j] = -1;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j)
+        rv[j] = wv[j] < 0 ? 0u : mst ? rprop : (uint32_t)(uint16_t)rank[(size_t)wv[j] * n + m];
+      int jf = 8;
+      uint32_t wf = 0;
+      unsigned long long mf = 0;
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j) {
+        if (wv[j] >= 0 && jf == 8) {
+          const unsigned long long mine = pw_pack(rv[j], (uint32_t)m, (uint32_t)(k + lane + G * j));
+          const unsigned long long hint =
+              __hip_atomic_load(&partner[wv[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (mine < hint) {
+            jf = j;
+            wf = (uint32_t)wv[j];
+            mf = mine;
+          }
+        }
+      }
+      int key = 8 * G;
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j) {
+        const unsigned long long b = __ballot(jf == j);
+        if (b) {
+          key = j * G + __ffsll((long long)b) - 1;
+          break;
+        }
+      }
+      const bool owner = jf < 8 && key == jf * G + lane;
+      unsigned long long old = 0;
+      if (owner) old = atomicMin(&partner[wf], mf);
+      const int src = key & (G - 1);
+      const unsigned long long mo =
+          ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(mf >> 32), src) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)mf, src);
+      old = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(old >> 32), src) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane((int)old, src);
+      const bool won = key < 8 * G && old > mo;
+      if (lane == 0) {
+        const int reach = key >= 8 * G ? W : key + 1;
+        __hip_atomic_store(&ctl_reach16, reach16 + ((16 * reach - reach16) >> 3), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      if (key >= 8 * G) {
+        k += W;
+        W = min(2 * W, G * JMAX);
+      } else if (won) {
+        if (old == kEmpty) {
+          m = -1;
+        } else {
+          m = (int)pw_man(old);
+          k = (int)pw_pos(old) + 1;
+          if (mst) rprop = (uint32_t)max(linv[m], 0);
+        }
+        W = pool_window0(reach16);
+      } else {
+        k += key + 1;
+        W = pool_window0(reach16);
+      }
+    }
+  }
+  __syncthreads();
+  // 5. The matching and the proposal count (the men's sum of list position + 1).
+  unsigned long long p = 0;
+  for (int a = threadIdx.x; a < n; a += BLOCK) {
+    const unsigned long long v = partner[a];
+    if (v == kEmpty) {
+      bad |= kErrUnmatched;
+    } else if (mst) {  // the words belong to the men
+      const uint32_t w = pw_man(v);
+      if (w < (uint32_t)n) {
+        match[a] = (int32_t)w;
+        p += (unsigned long long)max(linv[w], 0) + 1ull;
+      }
+    } else {
+      const uint32_t man = pw_man(v);
+      if (man < (uint32_t)n) match[man] = a;
+      p += pw_pos(v) + 1ull;
+    }
+  }
+  *props = p;
+  return bad;
+}
+
+// ---------------------------------------------------------------------------
 // Flat rank rows for many instances at once.  Row g of the launch is row
 // g - row0 of the task t with row0 <= g < row0 + n (t = row_task[g] when the
 // map is given, else a binary search over the tasks' row0).  Each wave owns a
