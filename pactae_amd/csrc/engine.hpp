@@ -118,6 +118,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   pactae::DeviceInfo dev;
   int K = 0;  // PR prefix columns
   int T = 0;  // hand-off threshold (0 = GPU finishes)
+  int T1 = 0;  // chains left when k_propose hands over to k_late (0 = no late phase)
   int K_alloc = 0;               // PR columns the buffers hold
   size_t pr_cap = 0;             // entries d_pr holds (grown for the whole matrix)
   // Lists whose chains walked far past the prefix on the GPU take the whole PR
@@ -132,6 +133,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int resume_n = 0;
   long long deep_limit = 0;
   int tuned_K = 0, tuned_T = -1;  // measured choice for AUTO mode (smp_tune)
+  int tuned_T1 = -1;                // measured late-phase hand-over (smp_tune)
   bool owns_lists = false;
   bool have_result = false;
 
@@ -277,6 +279,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   std::vector<uint16_t> hrank;
   static constexpr uint32_t kHostFree = 0xFFFFu;
   static constexpr int kTailSlots = 4;  // chains the host walks interleaved
+  static constexpr int kLateLanes = 16;  // k_late lanes per chain (64 chains at once)
   static uint64_t hs_pack(uint32_t r, uint32_t m, uint32_t p) {
     return (uint64_t)r | ((uint64_t)m << 16) | ((uint64_t)p << 32);
   }
@@ -398,6 +401,10 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     for (int i = 0; i < 16; ++i) __atomic_store_n(&h_flags[i], 0u, __ATOMIC_RELAXED);
     if (hipError_t e = pactae::set_invert_lds_limit(((invert_chunk() + 7) & ~7) * 2); e != hipSuccess) {
+      pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
+      return SMP_ERR_HIP;
+    }
+    if (hipError_t e = pactae::set_late_lds_limit(); e != hipSuccess) {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
@@ -557,6 +564,14 @@ struct smp_engine : public pactae::AbsSmpEngine {
     // The prefix buffers (d_pr, h_pr) hold K_alloc columns; a width set after
     // they were sized is cut to them (the whole matrix goes through grow_pr).
     if (K_alloc > 0) K = std::min(K, K_alloc);
+    // The late phase runs between k_propose and the host tail, for n whose
+    // women's state fits one workgroup's LDS.
+    T1 = 0;
+    if (T > 0 && opts.schedule == SMP_SCHEDULE_BAMBOO && pactae::late_queue_cap(ni) > 0) {
+      int t1 = opts.late_threshold;
+      if (t1 < 0) t1 = ni == n_alloc && opts.tail_mode == SMP_TAIL_AUTO && tuned_T1 >= 0 ? tuned_T1 : 0;
+      if (t1 > T) T1 = std::min(t1, ni);
+    }
   }
 
   // Measured choice of the hand-off threshold T and the prefix width K: every
@@ -1098,6 +1113,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     const int handoff = h_sc->parked_count;
     times.handoff_free = handoff;
+    times.late_chains = h_sc->late_taken;
     if (handoff > 0) {
       if (int st = host_tail()) return st;
     }
@@ -1109,7 +1125,18 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // the control block (and, with a hand-off, the partner words and parked
   // chains).  ev[5] marks the end of the phase on the device.
   int propose_phase(int grid) {
-    SMP_HIP_TRY(pactae::launch_propose(propose_args(T, resume_starts, resume_n), grid, stream));
+    SMP_HIP_TRY(pactae::launch_propose(propose_args(T1 > 0 ? T1 : T, resume_starts, resume_n), grid, stream));
+    if (T1 > 0) {
+      pactae::LateArgs la{};
+      la.pr = d_pr;
+      la.partner = d_partner;
+      la.parked = d_parked;
+      la.sc = d_sc;
+      la.n = n;
+      la.K = K;
+      la.T = T;
+      SMP_HIP_TRY(pactae::launch_late(la, kLateLanes, stream));
+    }
     span_propose = {&ev[2], &ev[5]};
     state_copied = false;
     if (T == 0) {
@@ -1577,7 +1604,11 @@ struct smp_engine : public pactae::AbsSmpEngine {
                                stream));
     const int grid = std::max(1, std::min(propose_grid(), (bailed + 3) / 4));
     SMP_HIP_TRY(pactae::launch_fill_i32(&d_sc->active, 1, std::min(bailed, grid * 4), stream));
-    SMP_HIP_TRY(pactae::launch_propose(propose_args(0, d_starts, bailed), grid, stream));
+    // The resumed chains must all finish here: no hand-off and no deep-list
+    // restart (a restart would park chains that nothing resumes).
+    pactae::ProposeArgs ra = propose_args(0, d_starts, bailed);
+    ra.deep_limit = 0;
+    SMP_HIP_TRY(pactae::launch_propose(ra, grid, stream));
     SMP_HIP_TRY(pactae::launch_fold_wstats(d_wstats, grid * 4, d_sc, stream));
     SMP_HIP_TRY(wait_stream(stream));
     times.resume_ms = ms_since(t0);
