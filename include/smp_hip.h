@@ -141,13 +141,7 @@ typedef struct smp_opts {
                              prefer propose first and displace fewer men; the
                              reference's RANDOM lists, n = 20,000: 83 -> 64 ms),
                              1 = by index.  The matching is the same          */
-  int32_t late_threshold; /* with a host tail: chains left (running or not yet
-                             started) at which the proposal kernel hands its
-                             chains to the one-workgroup late phase (women's
-                             state in LDS) until tail_threshold are left; 0 =
-                             off, -1 = auto (smp_tune's choice, else off).  Only
-                             for n up to ~26,000 (its LDS) and above
-                             tail_threshold                                    */
+  int32_t reserved[1];
 } smp_opts;
 
 enum smp_init_mode {
@@ -249,8 +243,7 @@ typedef struct smp_times {
                               woman held the same list: the matching then is the
                               serial dictatorship match[P[k]] = L[k] (the
                               reference's CONGESTED lists)                        */
-  int32_t late_chains;     /* chains the one-workgroup late phase took over from
-                              the proposal kernel (0 = it did not run)            */
+  int32_t reserved[1];
 } smp_times;
 
 /* Init-only run (AbsSmpInitEngine::Init, include/smp/smp_init_abs.h:16-28;
@@ -313,8 +306,7 @@ typedef struct smp_tune_result {
   int32_t candidates;  /* threshold / prefix pairs measured               */
   int32_t schedule;    /* enum smp_schedule chosen                        */
   int32_t waves_per_cu; /* proposal-kernel occupancy chosen (0 = default) */
-  int32_t late_threshold; /* late-phase hand-over chosen (0 = off)         */
-  int32_t reserved[4];
+  int32_t reserved[5];
 } smp_tune_result;
 int smp_tune(smp_engine* e, int32_t trials, smp_tune_result* out);
 /* Copies of the engine's device matrices after smp_exec or smp_init_exec (the

@@ -50,7 +50,7 @@ class SmpOpts(ctypes.Structure):
         ("init_mode", i32),
         ("tail_window_entries", i32),
         ("start_order", i32),
-        ("late_threshold", i32),
+        ("reserved", i32 * 1),
     ]
 
 
@@ -91,7 +91,7 @@ class SmpTimes(ctypes.Structure):
         ("tail_bands", i32),
         ("deep_restart", i32),
         ("master_both", i32),
-        ("late_chains", i32),
+        ("reserved", i32 * 1),
     ]
 
     def as_dict(self) -> dict:
@@ -123,8 +123,7 @@ class SmpTuneResult(ctypes.Structure):
         ("candidates", i32),
         ("schedule", i32),
         ("waves_per_cu", i32),
-        ("late_threshold", i32),
-        ("reserved", i32 * 4),
+        ("reserved", i32 * 5),
     ]
 
     def as_dict(self) -> dict:

@@ -55,9 +55,8 @@ static int make_engine(int32_t n, const smp_opts* opts, smp_engine** out) {
     smp_default_opts(&e->opts);
   if (e->opts.init_mode < SMP_INIT_AUTO || e->opts.init_mode > SMP_INIT_SPARSE ||
       e->opts.schedule < SMP_SCHEDULE_BAMBOO || e->opts.schedule >= SMP_SCHEDULE_COUNT ||
-      e->opts.tail_window_entries < 0 || e->opts.start_order < 0 || e->opts.start_order > 1 ||
-      e->opts.late_threshold < -1) {
-    pactae::set_error("opts: unknown init_mode, schedule or start_order, a negative tail_window_entries or a late_threshold below -1");
+      e->opts.tail_window_entries < 0 || e->opts.start_order < 0 || e->opts.start_order > 1) {
+    pactae::set_error("opts: unknown init_mode, schedule or start_order, or a negative tail_window_entries");
     return SMP_ERR_INVALID;
   }
   if (int st = e->setup()) return st;

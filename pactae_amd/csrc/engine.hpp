@@ -118,7 +118,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
   pactae::DeviceInfo dev;
   int K = 0;  // PR prefix columns
   int T = 0;  // hand-off threshold (0 = GPU finishes)
-  int T1 = 0;  // chains left when k_propose hands over to k_late (0 = no late phase)
   int K_alloc = 0;               // PR columns the buffers hold
   size_t pr_cap = 0;             // entries d_pr holds (grown for the whole matrix)
   // Lists whose chains walked far past the prefix on the GPU take the whole PR
@@ -133,7 +132,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
   int resume_n = 0;
   long long deep_limit = 0;
   int tuned_K = 0, tuned_T = -1;  // measured choice for AUTO mode (smp_tune)
-  int tuned_T1 = -1;                // measured late-phase hand-over (smp_tune)
   bool owns_lists = false;
   bool have_result = false;
 
@@ -279,7 +277,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
   std::vector<uint16_t> hrank;
   static constexpr uint32_t kHostFree = 0xFFFFu;
   static constexpr int kTailSlots = 4;  // chains the host walks interleaved
-  static constexpr int kLateLanes = 16;  // k_late lanes per chain (64 chains at once)
   static uint64_t hs_pack(uint32_t r, uint32_t m, uint32_t p) {
     return (uint64_t)r | ((uint64_t)m << 16) | ((uint64_t)p << 32);
   }
@@ -401,10 +398,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     for (int i = 0; i < 16; ++i) __atomic_store_n(&h_flags[i], 0u, __ATOMIC_RELAXED);
     if (hipError_t e = pactae::set_invert_lds_limit(((invert_chunk() + 7) & ~7) * 2); e != hipSuccess) {
-      pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
-      return SMP_ERR_HIP;
-    }
-    if (hipError_t e = pactae::set_late_lds_limit(); e != hipSuccess) {
       pactae::set_error(std::string("hipFuncSetAttribute(LDS) failed: ") + hipGetErrorString(e));
       return SMP_ERR_HIP;
     }
@@ -564,14 +557,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
     // The prefix buffers (d_pr, h_pr) hold K_alloc columns; a width set after
     // they were sized is cut to them (the whole matrix goes through grow_pr).
     if (K_alloc > 0) K = std::min(K, K_alloc);
-    // The late phase runs between k_propose and the host tail, for n whose
-    // women's state fits one workgroup's LDS.
-    T1 = 0;
-    if (T > 0 && opts.schedule == SMP_SCHEDULE_BAMBOO && pactae::late_queue_cap(ni) > 0) {
-      int t1 = opts.late_threshold;
-      if (t1 < 0) t1 = ni == n_alloc && opts.tail_mode == SMP_TAIL_AUTO && tuned_T1 >= 0 ? tuned_T1 : 0;
-      if (t1 > T) T1 = std::min(t1, ni);
-    }
   }
 
   // Measured choice of the hand-off threshold T and the prefix width K: every
@@ -1113,7 +1098,6 @@ struct smp_engine : public pactae::AbsSmpEngine {
     }
     const int handoff = h_sc->parked_count;
     times.handoff_free = handoff;
-    times.late_chains = h_sc->late_taken;
     if (handoff > 0) {
       if (int st = host_tail()) return st;
     }
@@ -1125,18 +1109,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   // the control block (and, with a hand-off, the partner words and parked
   // chains).  ev[5] marks the end of the phase on the device.
   int propose_phase(int grid) {
-    SMP_HIP_TRY(pactae::launch_propose(propose_args(T1 > 0 ? T1 : T, resume_starts, resume_n), grid, stream));
-    if (T1 > 0) {
-      pactae::LateArgs la{};
-      la.pr = d_pr;
-      la.partner = d_partner;
-      la.parked = d_parked;
-      la.sc = d_sc;
-      la.n = n;
-      la.K = K;
-      la.T = T;
-      SMP_HIP_TRY(pactae::launch_late(la, kLateLanes, stream));
-    }
+    SMP_HIP_TRY(pactae::launch_propose(propose_args(T, resume_starts, resume_n), grid, stream));
     span_propose = {&ev[2], &ev[5]};
     state_copied = false;
     if (T == 0) {

@@ -67,7 +67,6 @@ __global__ void k_reset_scalars(Scalars* sc, int active, int* z0, size_t n0, int
     sc->women_differ = 0;
     sc->scans_done = 0;
     sc->both_master = 0;
-    sc->late_taken = 0;
     sc->deep_steps = 0;
     sc->atomics = 0;
     sc->steps = 0;
@@ -822,215 +821,6 @@ __global__ __launch_bounds__(256) void k_fold_wstats(const unsigned long long* _
   if (threadIdx.x == 0) {
     sc->atomics += red[0][0] + red[0][1] + red[0][2] + red[0][3];
     sc->steps += red[1][0] + red[1][1] + red[1][2] + red[1][3];
-  }
-}
-
-// Late phase in one workgroup.  Once few chains are left, a k_propose step is
-// three dependent round trips through memory shared by all XCDs (the PR segment,
-// the women's words, the claim performed past the L2).  Here one workgroup takes
-// the chains k_propose parked at its threshold and holds the women's state in
-// LDS: a 32-bit word (rank << 16) | man per woman, claimed with a 32-bit LDS
-// atomicMin (ranks are distinct), and each man's list position of his current
-// woman in a 16-bit array, so that a step is one global load (the PR segment)
-// and LDS work.  Groups of G lanes walk one chain each (1024 / G chains at once)
-// and take the next parked chain when theirs ends.
-//
-// A man's position is written after his claim succeeded, as a maximum (his
-// positions only grow).  A chain that displaces him before that write reads an
-// earlier position of his and walks again through women who already refused
-// him: they refuse him again, since a woman's partner only improves, so this
-// costs steps and never changes the matching.
-//
-// Once at most T chains are left (running or not yet started), every group
-// parks its chain (man, next position), and the chains not started are parked
-// too, for the host; a chain that reaches the end of the PR prefix is parked at
-// K as in k_propose.  At the end the women's words go back to `partner` in the
-// 64-bit format and parked_count is set.  With malformed-list flags, a restart,
-// more parked chains than the LDS queue holds, or at most T of them, the kernel
-// changes nothing.
-constexpr uint32_t kLateFree = 0xFFFFFFFFu;
-
-__device__ inline void lds_max16(uint32_t* words, int m, uint32_t pos) {
-  uint32_t* wd = words + (m >> 1);
-  const int sh = (m & 1) * 16;
-  uint32_t old = __hip_atomic_load(wd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  for (;;) {
-    if (((old >> sh) & 0xFFFFu) >= pos) return;
-    const uint32_t nw = (old & ~(0xFFFFu << sh)) | (pos << sh);
-    const uint32_t prev = atomicCAS(wd, old, nw);
-    if (prev == old) return;
-    old = prev;
-  }
-}
-
-template <int G>
-__global__ __launch_bounds__(1024) void k_late(LateArgs a) {
-  static_assert(G == 16 || G == 32 || G == 64, "lanes per chain");
-  extern __shared__ uint32_t lds_late[];
-  const int n = a.n, tid = threadIdx.x;
-  uint32_t* P = lds_late;                                   // women's words
-  uint32_t* mp32 = P + n;                                   // men's positions, 16 bits each
-  uint16_t* mp = reinterpret_cast<uint16_t*>(mp32);
-  uint32_t* queue = mp32 + (n + 1) / 2;                     // parked chains (k << 16) | m
-  __shared__ int s_go, s_pc, s_qhead, s_live, s_out, s_err;
-  __shared__ unsigned long long s_at, s_st;
-  if (tid == 0) {
-    const int pc = __hip_atomic_load(&a.sc->parked_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // (After a prefix overflow k_propose did nothing and parked nothing.)
-    const bool skip = __hip_atomic_load(&a.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                      __hip_atomic_load(&a.sc->restart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-    s_go = !skip && pc > a.T && pc <= a.qcap && n <= 65535;
-    a.sc->late_taken = s_go ? pc : 0;
-    s_pc = pc;
-    s_qhead = 0;
-    s_live = pc;
-    s_out = 0;
-    s_err = 0;
-    s_at = s_st = 0;
-  }
-  __syncthreads();
-  if (!s_go) return;
-  const int pc = s_pc;
-  for (int i = tid; i < (n + 1) / 2; i += 1024) mp32[i] = 0;
-  for (int i = tid; i < pc; i += 1024) {
-    const int2 c = a.parked[i];
-    queue[i] = ((uint32_t)c.y << 16) | ((uint32_t)c.x & 0xFFFFu);
-  }
-  __syncthreads();
-  for (int w = tid; w < n; w += 1024) {
-    const unsigned long long v = a.partner[w];
-    if (v == kEmpty) {
-      P[w] = kLateFree;
-    } else {
-      const uint32_t m = pw_man(v);
-      P[w] = (pw_rank(v) << 16) | (m & 0xFFFFu);
-      if (m < (uint32_t)n) mp[m] = (uint16_t)pw_pos(v);
-    }
-  }
-  // Block 0 of the PR prefix (positions 0-31 of every man, where most late
-  // steps fall: a displaced man resumes one past where he was engaged) is read
-  // once, one load per 128-byte line, so that the steps find it in this XCD's
-  // L2 (2.56 MB at n = 20,000) instead of going past it.
-  {
-    const int blk = min(a.K, kPrBlock) == kPrBlock ? n * kPrBlock : 0;
-    uint32_t acc = 0;
-    for (int i0 = tid * 32; i0 < blk; i0 += 8 * 1024 * 32) {
-      uint32_t v[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = i0 + j * 1024 * 32 < blk ? a.pr[i0 + j * 1024 * 32] : 0u;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) acc ^= v[j];
-    }
-    asm volatile("" ::"v"(acc));  // keeps the loads (no instruction)
-  }
-  __syncthreads();
-
-  const int lane = tid & 63, gl = lane & (G - 1), gbase = lane & ~(G - 1);
-  const int Klim = min(a.K, n);
-  int m = -1, k = 0, err = 0;
-  unsigned long long at = 0, st = 0;
-  auto park = [&](int pm_, int pk) {
-    const int o = atomicAdd(&s_out, 1);
-    a.parked[o] = make_int2(pm_, pk);
-  };
-  for (;;) {
-    const int live = __builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(&s_live, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
-    if (live <= a.T) {
-      if (m >= 0 && gl == 0) park(m, k);
-      break;
-    }
-    if (m < 0) {
-      int idx = 0;
-      if (gl == 0) idx = atomicAdd(&s_qhead, 1);
-      idx = __shfl(idx, gbase);
-      if (idx >= pc) break;
-      const uint32_t q = queue[idx];
-      m = (int)(q & 0xFFFFu);
-      k = (int)(q >> 16);
-      if (m >= n) {  // the parked list holds only men of this instance
-        err |= kErrEntryRange;
-        if (gl == 0) atomicSub(&s_live, 1);
-        m = -1;
-        continue;
-      }
-    }
-    if (k >= Klim) {
-      // Past the prefix: parked at the first position not examined, for the host
-      // (or, past the list's end, an error: rows are not permutations).
-      if (k >= n) err |= kErrListEnd;
-      else if (gl == 0) park(m, k);
-      if (gl == 0) atomicSub(&s_live, 1);
-      m = -1;
-      continue;
-    }
-    ++st;
-    const int kk = k + gl;
-    bool valid = kk < Klim;
-    uint32_t w = 0, mine = 0, hint = 0;
-    if (valid) {
-      const uint32_t e = a.pr[pr_index(m, kk, n)];
-      w = pr_woman(e);
-      mine = (pr_rank(e) << 16) | (uint32_t)m;
-      if (w >= (uint32_t)n) {
-        valid = false;
-        err |= kErrEntryRange;
-      } else {
-        hint = P[w];
-      }
-    }
-    const unsigned long long bal = __ballot(valid && mine < hint);
-    unsigned long long cand = G == 64 ? bal : (bal >> gbase) & ((1ull << G) - 1);
-    bool moved = false;
-    while (cand) {
-      const int j = __builtin_ctzll(cand);
-      uint32_t old = 0;
-      if (gl == j) old = atomicMin(&P[w], mine);
-      old = (uint32_t)__shfl((int)old, gbase + j);
-      const uint32_t mj = (uint32_t)__shfl((int)mine, gbase + j);
-      ++at;
-      if (old > mj) {
-        if (gl == j) lds_max16(mp32, m, (uint32_t)kk);
-        if (old == kLateFree) {
-          if (gl == 0) atomicSub(&s_live, 1);
-          m = -1;
-        } else {
-          m = (int)(old & 0xFFFFu);
-          if (m < n) {
-            k = (int)mp[m] + 1;
-          } else {  // a word naming no man of this instance: drop the chain
-            err |= kErrEntryRange;
-            if (gl == 0) atomicSub(&s_live, 1);
-            m = -1;
-          }
-        }
-        moved = true;
-        break;
-      }
-      cand &= cand - 1;
-    }
-    if (!moved) k = min(k + G, Klim);
-  }
-  if (err) atomicOr(&s_err, err);
-  if (gl == 0) {
-    atomicAdd(&s_at, at);
-    atomicAdd(&s_st, st);
-  }
-  __syncthreads();
-  const int q0 = min(s_qhead, pc), outc = s_out;
-  for (int i = q0 + tid; i < pc; i += 1024) {
-    const uint32_t q = queue[i];
-    a.parked[outc + i - q0] = make_int2((int)(q & 0xFFFFu), (int)(q >> 16));
-  }
-  for (int w = tid; w < n; w += 1024) {
-    const uint32_t v = P[w];
-    a.partner[w] = v == kLateFree ? kEmpty : pw_pack(v >> 16, v & 0xFFFFu, mp[v & 0xFFFFu]);
-  }
-  if (tid == 0) {
-    a.sc->parked_count = outc + pc - q0;
-    atomicAdd(&a.sc->atomics, s_at);
-    atomicAdd(&a.sc->steps, s_st);
-    if (s_err) atomicOr(&a.sc->error, s_err);
   }
 }
 
@@ -2278,43 +2068,6 @@ hipError_t launch_frontier(const FrontierArgs& a, int cus, hipStream_t s) {
 
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// LDS of k_late: 4 bytes per woman, 2 per man, 4 per queued chain, within the
-// 160 KB a gfx950 workgroup can hold.
-constexpr int kLateLdsMax = 160 * 1024 - 256;
-static size_t late_lds_bytes(int n, int q) {
-  return (size_t)n * 4 + (size_t)((n + 1) / 2) * 4 + (size_t)q * 4;
-}
-
-int late_queue_cap(int n) {
-  if (n <= 0 || n > 65535) return 0;
-  const long long room = ((long long)kLateLdsMax - (long long)late_lds_bytes(n, 0)) / 4;
-  if (room < 64) return 0;
-  return (int)std::min<long long>(room, n);
-}
-
-hipError_t set_late_lds_limit() {
-  const void* fns[] = {reinterpret_cast<const void*>(&k_late<16>), reinterpret_cast<const void*>(&k_late<32>),
-                       reinterpret_cast<const void*>(&k_late<64>)};
-  for (const void* f : fns) {
-    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kLateLdsMax);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-
-hipError_t launch_late(LateArgs a, int lanes, hipStream_t s) {
-  a.qcap = late_queue_cap(a.n);
-  if (a.qcap <= 0) return hipErrorInvalidValue;
-  const size_t lds = late_lds_bytes(a.n, a.qcap);
-  if (lanes == 16)
-    hipLaunchKernelGGL(k_late<16>, dim3(1), dim3(1024), lds, s, a);
-  else if (lanes == 32)
-    hipLaunchKernelGGL(k_late<32>, dim3(1), dim3(1024), lds, s, a);
-  else
-    hipLaunchKernelGGL(k_late<64>, dim3(1), dim3(1024), lds, s, a);
   return hipGetLastError();
 }
 

@@ -123,21 +123,6 @@ hipError_t launch_rank_prefix(const int32_t* pref_w, uint16_t* rank, int n, int 
                               const int* cnt, const uint32_t* bucket, int cap, uint16_t* brank, uint32_t* pr,
                               Scalars* sc, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, int grid, hipStream_t s);
-// The late phase in one workgroup (k_late): the chains k_propose parked at its
-// threshold, walked with the women's state in LDS until at most T are left,
-// which stay parked for the host.  `lanes` (16, 32 or 64) lanes walk one chain.
-struct LateArgs {
-  const uint32_t* pr;            // PR prefix (positions < K)
-  unsigned long long* partner;   // women's words, read at the start, written at the end
-  int2* parked;                  // in: k_propose's parked chains; out: those left
-  Scalars* sc;
-  int n, K, T;
-  int qcap;                      // parked chains the LDS queue holds (set by the launcher)
-};
-// Parked chains k_late's LDS queue holds for n women (0: n does not fit).
-int late_queue_cap(int n);
-hipError_t set_late_lds_limit();
-hipError_t launch_late(LateArgs a, int lanes, hipStream_t s);
 // Extension windows for the host tail: req[i].w list entries (woman, rank) from
 // position req[i].y of man req[i].x into out[req[i].z ..] (pinned host memory,
 // device address), kWinInvalid past the list's end.  The requests' entries are

@@ -80,7 +80,7 @@ struct Scalars {
   int scans_done;    // master-list path: scan workgroups finished
   int both_master;   // master-list path: every scan done, every man's list equal to
                      // man 0's and every woman's to woman 0's
-  int late_taken;    // k_late: chains it took from the proposal kernel (0 = it did not run)
+  int pad0;
   unsigned long long deep_steps; // proposal kernel: steps past the PR prefix so far
                                  // (flushed by the waves every 256 steps)
   unsigned long long atomics;    // atomicMin attempts

@@ -220,17 +220,17 @@ def test_alternative_schedules(gpu, oracle, kind, schedule):
 
 
 def test_tuner_picks_a_schedule_on_contention(gpu, oracle):
-    """All-shared lists with the precheck off at n = 5,000 take more than 5 ms per
+    """All-shared lists with the precheck off at n = 8,000 take more than 5 ms per
     solve, so the tuner measures the schedules and keeps the fastest; later
     solves use it and stay bit-exact.  A fast instance keeps the default."""
     smp = gpu
-    n = 5000
+    n = 8000
     pm, pw = smp.generate(smp.Workload.CONGESTED, n, seed=41)
     want, t = oracle.gs_seq(n, pm, pw)
     obj = smp.SmpObj.create_from_flat_lists(pm, pw, n)
     with smp.SmpEngineBamboo(obj, n, precheck=0) as e:
         r = e.tune(2)
-        assert 0 <= r["schedule"] < 6 and r["candidates"] == 0
+        assert 0 <= r["schedule"] < 6 and r["candidates"] == 0, r
         got = e.find_stable_matching()
         assert e.profile()["schedule"] == r["schedule"]
         assert np.array_equal(got, want)
@@ -623,3 +623,19 @@ def test_init_only_rank_and_pr_matrices(gpu, n):
             assert pr.shape == (n, K, 2)
             assert np.array_equal(pr[:, :, 0], M[:, :K])
             assert np.array_equal(pr[:, :, 1], rank_w[M[:, :K], np.arange(n)[:, None]])
+
+
+def test_handed_back_chains_finish_on_deep_lists(gpu, oracle):
+    """The reference's RANDOM lists at n = 2,500 with the host taking
+    every chain at the proposal kernel's first poll: the host hands hundreds of
+    deep chains back to the GPU, whose resumed walk passes the deep-list step
+    count; it must finish them (no restart) rather than park them with nothing
+    left to resume them."""
+    smp = gpu
+    n = 2500
+    pm, pw = smp.generate(smp.Workload.REF_RANDOM, n, seed=3000 + n)
+    want, t = oracle.gs_seq(n, pm, pw)
+    got, prof = solve(smp, n, pm, pw, tail_mode=2, tail_threshold=100000, poll_interval=1)
+    assert np.array_equal(got, want)
+    assert prof["proposals"] == t["proposals"]
+    assert prof["bailed"] > 0
