@@ -205,7 +205,8 @@ typedef struct smp_times {
   int64_t proposals;       /* sum over men of (final list position + 1)           */
   int64_t atomics;         /* atomicMin attempts made by the proposal kernel(s)   */
   int64_t host_proposals;  /* proposals made by the host tail                     */
-  int64_t steps;           /* 64-entry segment scans made by the proposal kernel  */
+  int64_t steps;           /* list-segment scans (16 to 32 entries) made by the
+                              proposal kernel                                      */
   int32_t is_perfect;      /* 1 if the first-choice fast path answered            */
   int32_t handoff_free;    /* free men handed to the host (0 = GPU finished)      */
   int32_t prefix_cols;     /* PR columns actually gathered                        */

@@ -605,9 +605,9 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const unsigned long long* __r
 // ---------------------------------------------------------------------------
 // Proposal kernel.  Persistent: each wave owns one free man ("chain") at a time.
 // Wave g starts the men g, g + W, g + 2W, ... (W waves in the grid) in turn, so
-// starting a man needs no shared counter.  A step loads the next 64 entries of
-// the man's list (one coalesced PR load, or pref_m + rank_w beyond the gathered
-// prefix), reads the 64 women's partner words and ballots the lanes where the
+// starting a man needs no shared counter.  A step loads the next 16 to 32
+// entries of the man's list (one coalesced PR load, or pref_m + rank_w beyond the
+// gathered prefix), reads those women's partner words and ballots the lanes where the
 // man beats the current partner.  A partner word only ever decreases, so a stale
 // read can only make the wave try an atomic that then fails; coherent_hint picks
 // whether those reads bypass the non-coherent caches.  Candidates are tried in
@@ -622,6 +622,17 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const unsigned long long* __r
 // been started, so `active` equals the number of free men.  After a sparse prefix
 // pass (no rank matrix), a chain that reaches position K is parked at K in the
 // same list, and its wave starts its next man.
+// A step examines the next `win` positions of the man's list: kProposeWin0 for a
+// man just started or displaced, doubled up to kProposeWinMax after a step that
+// found no candidate.  Each examined position costs a load of a woman's word
+// from its own cache line, and the positions past the one that accepts him are
+// wasted.  Measured against a fixed 64: the proposal phase of n = 20,000 uniform
+// 0.219 -> 0.193 ms (median of five seeds), the reference's RANDOM lists (groups
+// of 5, n = 20,000, whole PR matrix) 65.1 -> 43.7 ms per solve; a maximum of 16
+// was 1.5 ms faster still on RANDOM but slower on correlated lists.
+constexpr int kProposeWin0 = 16;
+constexpr int kProposeWinMax = 32;
+
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   const int lane = threadIdx.x & 63;
   const int n = a.n, K = a.K, T = a.T;
@@ -630,6 +641,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
   const int total = a.starts ? a.nstarts : n;  // work items: men, or resumed chains
   int next_init = gwave;
   int m = -1, k = 0;
+  int win = kProposeWin0;  // positions examined per step (lanes < win)
   // Speculation: while a candidate's atomicMin is in flight, the segment of the
   // man it would displace is loaded too; (spec_m, spec_k) says whose it is.
   int spec_m = -1, spec_k = -1;
@@ -668,6 +680,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         k = 0;
       }
       next_init += waves;
+      win = kProposeWin0;
       if ((unsigned)m >= (unsigned)n) {  // a resume list holds only men of this instance
         if (lane == 0) atomicOr(&a.sc->error, kErrEntryRange);
         m = -1;
@@ -714,7 +727,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
       break;
     }
     const int kk = k + lane;
-    bool valid = kk < n && (have_rank || kk < K);
+    bool valid = lane < win && kk < n && (have_rank || kk < K);
     uint32_t w = 0, r = 0;
     if (valid) {
       if (kk < K) {
@@ -768,13 +781,15 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
           k = (int)pw_pos(old) + 1;
           if (m == sm && k == sk) spec_m = sm, spec_k = sk;
         }
+        win = kProposeWin0;
         moved = true;
         break;
       }
       cand &= cand - 1;
     }
     if (!moved) {
-      k += 64;
+      k += win;
+      win = min(kProposeWinMax, 2 * win);
       if (!have_rank && k >= K) {
         // Positions below K are settled; the rest needs the rank matrix, which the
         // host builds before it resumes the chain at K.
