@@ -642,6 +642,16 @@ struct smp_engine : public pactae::AbsSmpEngine {
       if (int st = measure(c.k, c.t, 3 * trials, &c.med)) return fail(st);
       if (c.med < best.med) best = c;
     }
+    // The prefix width that wins is fitted to how deep this instance's chains
+    // end; other lists of the same kind end deeper (uniform n = 20,000: deepest
+    // final position 91 to 131 over six seeds), and a chain past the prefix
+    // costs a GPU round trip.  So among the finals within 3% of the fastest,
+    // the widest prefix is kept (a wider prefix costs a few microseconds).
+    {
+      const double limit = best.med * 1.03;
+      for (const Cand& c : finals)
+        if (c.med <= limit && (c.k > best.k || (c.k == best.k && c.med < best.med))) best = c;
+    }
     opts = saved;
     tuned_K = best.k;
     tuned_T = best.t;
