@@ -596,8 +596,9 @@ __device__ int wg_pool_chains(const Desc* __restrict__ desc, int count, int* nex
 //   3. for a master-list instance, Linv[L[k]] = k in LDS;
 //   4. the walk (the La3F9CoreKernel proposal, src/smp/kernel.cu:633-670): one
 //      chain per wave, the pool walk's step (a window of about twice the
-//      instance's running mean reach, first candidate from ballots, claim with a
-//      64-bit LDS atomicMin, wave-uniform values through readlane); a master-list
+//      instance's running mean reach, doubling to at most 64 positions after an
+//      empty step, first candidate from ballots, claim with a 64-bit LDS
+//      atomicMin, wave-uniform values through readlane); a master-list
 //      instance is walked from the women's side with the rank Linv[w] every man
 //      gives woman w;
 //   5. the matching (PostProc, src/smp/smp_engine_bamboo.cu:299-303) written to
@@ -619,7 +620,10 @@ __device__ int wg_solve_instance(const int32_t* __restrict__ pm, const int32_t* 
   constexpr int NW = BLOCK / 64;
   constexpr int kPer = fused_per(BLOCK);
   constexpr int kStage = BLOCK * kPer;
-  constexpr int G = 64, JMAX = 2;
+  // At most 64 positions per step: 4,096 mixed instances 6.5 -> 6.0 ms against
+  // 128 (each examined position looks up a rank on a line of its own), 7.0 ms
+  // at 32 (nearly shared lists refuse dozens of positions in a row).
+  constexpr int G = 64, JMAX = 1;
   int& ctl_cursor = ctl[0];
   int& ctl_differs = ctl[1];
   int& ctl_reach16 = ctl[2];
