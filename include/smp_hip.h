@@ -447,7 +447,11 @@ typedef struct smp_server_opts {
                                a large-request worker when one is idle and none
                                waits for the workers, else to the lanes;
                                0 = always to the lanes; -1 = the default          */
-  int32_t reserved[3];
+  int32_t fused_lanes;      /* 1 = lane launches whose largest request has 512 < n <=
+                               2048 run one kernel, a workgroup per request (rank
+                               rows, a shared-list check and chains in it); 0 = a
+                               flat rank pass and a chain kernel; -1 = the default */
+  int32_t reserved[2];
 } smp_server_opts;
 
 typedef struct smp_request_info {

@@ -179,7 +179,8 @@ class SmpServerOpts(ctypes.Structure):
         ("large_spill_us", i32),
         ("small_chain_lanes", i32),
         ("small_to_idle_worker", i32),
-        ("reserved", i32 * 3),
+        ("fused_lanes", i32),
+        ("reserved", i32 * 2),
     ]
 
 

@@ -223,6 +223,58 @@ __global__ __launch_bounds__(BLOCK, 2) void k_serve_small(const ServeReq* __rest
   }
 }
 
+// Requests of up to kFusedLaneN men in a launch whose largest is above kSmallN:
+// each request in one workgroup, by the batched path's per-instance solve
+// (wg_solve_instance: shared men's list check, women's rank rows staged in LDS,
+// Linv for a shared list, chains of one wave each with the women's words in
+// LDS, matching), in one kernel instead of a flat rank pass and a chain kernel.
+// Descriptor, results, error bits and the completion flag as in k_serve_small.
+constexpr int kFusedLaneN = 2048;
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 8) void k_serve_fused(const ServeReq* __restrict__ host_reqs, int count,
+                                                          unsigned long long* h_props, int* h_errs, int* done,
+                                                          unsigned* h_flag, unsigned seq, int max_n) {
+  static_assert(sizeof(ServeReq) % 4 == 0 && sizeof(ServeReq) / 4 <= 64, "descriptor words");
+  constexpr int kDescWords = (int)(sizeof(ServeReq) / 4);
+  constexpr int NW = BLOCK / 64;
+  extern __shared__ unsigned long long smem64[];
+  unsigned long long* partner = smem64;
+  int32_t* linv = reinterpret_cast<int32_t*>(partner + max_n);
+  uint16_t* stage = reinterpret_cast<uint16_t*>(linv + max_n);
+  __shared__ int desc[kDescWords];
+  __shared__ int ctl[3];
+  __shared__ int sbad;
+  __shared__ unsigned long long red[NW];
+  const int i = blockIdx.x;
+  if (threadIdx.x < kDescWords) desc[threadIdx.x] = reinterpret_cast<const int*>(host_reqs + i)[threadIdx.x];
+  if (threadIdx.x == 0) sbad = 0;
+  __syncthreads();
+  const ServeReq q = *reinterpret_cast<const ServeReq*>(desc);
+  int bad = 0;
+  unsigned long long p = 0;
+  if (q.n > 0 && q.n <= max_n && q.n <= BLOCK * fused_per(BLOCK)) {
+    bad = wg_solve_instance<BLOCK>(q.pm, q.pw, q.rank, q.match, q.n, partner, linv, stage, ctl, &p);
+  } else {
+    bad = kErrEntryRange;  // the host sends only 1 <= n <= max_n here
+  }
+  for (int off = 32; off > 0; off >>= 1) p += __shfl_down(p, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = p;
+  if (bad) atomicOr(&sbad, bad);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < NW; ++w) t += red[w];
+    h_props[i] = t;
+    h_errs[i] = sbad;
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(done, 1) == count - 1) {
+    *done = 0;
+    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 // First-fit arena over one device allocation, 256-byte granules.
 class Arena {
  public:
@@ -672,6 +724,17 @@ struct smp_server {
       ++stats.launches;
       return hipSuccess;
     }
+    if (bmax <= pactae::kFusedLaneN && opts.fused_lanes) {
+      // Mid-size requests: one fused kernel, one workgroup each.
+      const size_t lds = pactae::fused_lds(1024, bmax);
+      hipLaunchKernelGGL(pactae::k_serve_fused<1024>, dim3(count), dim3(1024), lds, L.stream, L.h_reqs_dev, count,
+                         L.h_props_dev, L.h_errs_dev, L.d_done, L.h_flag_dev, L.seq, bmax);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = hipEventRecord(L.done, L.stream)) != hipSuccess) return e;
+      L.busy = true;
+      ++stats.launches;
+      return hipSuccess;
+    }
     hipLaunchKernelGGL(pactae::k_serve_prep, dim3(1), dim3(256), 0, L.stream, L.h_reqs_dev, L.d_reqs, count,
                        L.d_errs);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -810,6 +873,7 @@ void smp_server_default_opts(smp_server_opts* o) {
   o->large_spill_us = 0;
   o->small_chain_lanes = 0;
   o->small_to_idle_worker = -1;
+  o->fused_lanes = -1;
 }
 
 int smp_server_create(const smp_server_opts* opts, smp_server** out) {
@@ -828,6 +892,7 @@ int smp_server_create(const smp_server_opts* opts, smp_server** out) {
   if (o.block_threads != 256 && o.block_threads != 512 && o.block_threads != 1024) o.block_threads = 1024;
   if (o.large_n <= 0) o.large_n = 512;
   if (o.small_to_idle_worker < 0) o.small_to_idle_worker = 1;
+  if (o.fused_lanes < 0) o.fused_lanes = 1;
   if (o.small_chain_lanes != 2 && o.small_chain_lanes != 4 && o.small_chain_lanes != 8 &&
       o.small_chain_lanes != 16)
     o.small_chain_lanes = 16;

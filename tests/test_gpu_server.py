@@ -130,15 +130,20 @@ def test_invalid_request_fails_alone(gpu, oracle):
 
 
 @pytest.mark.parametrize("max_batch", [1, 8])
-def test_lane_kernels_at_the_small_request_bound(gpu, oracle, max_batch):
-    """Lane launches whose requests all have n <= 512 run the fused one-kernel
-    solve; a launch holding a larger request runs the rank pass and the chain
-    kernel.  Sizes on both sides of the bound, every kind, all on the lanes."""
+@pytest.mark.parametrize("fused", [1, 0])
+def test_lane_kernels_at_the_small_request_bound(gpu, oracle, max_batch, fused):
+    """Lane launches whose requests all have n <= 512 run the small one-kernel
+    solve; a launch whose largest request has n <= 2,048 runs the fused
+    per-instance kernel (fused_lanes 1) or the rank pass and the chain kernel;
+    a larger one the rank pass and the chain kernel.  Sizes on both sides of
+    both bounds, every kind (congested lists share one men's list), all on the
+    lanes."""
     smp = gpu
-    sizes = [1, 2, 3, 64, 511, 512, 513, 700, 512, 40]
-    kinds = [0, 1, 3, 0, 3, 1, 0, 3, 0, 1]
+    sizes = [1, 2, 3, 64, 511, 512, 513, 700, 512, 40, 2047, 2048, 1500, 2049, 3000]
+    kinds = [0, 1, 3, 0, 3, 1, 0, 3, 0, 1, 0, 3, 1, 1, 0]
     reqs = [(n, smp.generate(k, n, seed=4242 + i)) for i, (n, k) in enumerate(zip(sizes, kinds))]
-    with smp.SmpServer(lanes=2, max_batch=max_batch, max_n=1024, large_n=100000, small_to_idle_worker=0) as srv:
+    with smp.SmpServer(lanes=2, max_batch=max_batch, max_n=4096, large_n=100000, small_to_idle_worker=0,
+                       fused_lanes=fused) as srv:
         tickets = [srv.submit(pm, pw, n) for n, (pm, pw) in reqs]
         out = [srv.wait(t, timeout_ms=60000) for t in tickets]
     for (n, (pm, pw)), (got, info) in zip(reqs, out):
