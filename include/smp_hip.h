@@ -440,18 +440,15 @@ typedef struct smp_server_opts {
                                waited this long (so that at low load a request runs
                                alone on the GPU); 0 = any idle worker at once
                                (default: measured best for p50 and p99)            */
-  int32_t small_chain_lanes; /* lanes per proposal chain in the one-kernel solve of
-                               lane launches whose requests all have n <= 512
-                               (2, 4, 8 or 16); 0 = auto                          */
   int32_t small_to_idle_worker; /* batched mode: 1 = a request below large_n goes to
                                a large-request worker when one is idle and none
                                waits for the workers, else to the lanes;
                                0 = always to the lanes; -1 = the default          */
-  int32_t fused_lanes;      /* 1 = lane launches whose largest request has 512 < n <=
-                               2048 run one kernel, a workgroup per request (rank
-                               rows, a shared-list check and chains in it); 0 = a
-                               flat rank pass and a chain kernel; -1 = the default */
-  int32_t reserved[2];
+  int32_t fused_lanes;      /* 1 = lane launches whose requests all have n <= 2048
+                               run one kernel, a workgroup per request (a shared-
+                               list check, rank rows and chains in it); 0 = a flat
+                               rank pass and a chain kernel; -1 = the default (1) */
+  int32_t reserved[3];
 } smp_server_opts;
 
 typedef struct smp_request_info {

@@ -177,10 +177,9 @@ class SmpServerOpts(ctypes.Structure):
         ("large_n", i32),
         ("large_workers", i32),
         ("large_spill_us", i32),
-        ("small_chain_lanes", i32),
         ("small_to_idle_worker", i32),
         ("fused_lanes", i32),
-        ("reserved", i32 * 2),
+        ("reserved", i32 * 3),
     ]
 
 

@@ -17,11 +17,12 @@
 //     together on whichever lane is free, so a request that arrives while
 //     others are on the GPU starts as soon as a lane and memory free up, and its
 //     kernels overlap with the ones already running.  A launch whose requests
-//     all have n <= kSmallN is one kernel, k_serve_small (one workgroup per
-//     request: rank rows, chains, matching); a larger one is k_serve_prep, a
-//     flat k_rank_rows16 pass over all its women's rows (one wave per row, so a
-//     large request is spread over the whole GPU) and k_serve_chains (one
-//     workgroup per request).  Results, proposal counts, error bits and a
+//     all have n <= kFusedLaneN is one kernel, k_serve_fused (one workgroup per
+//     request: shared-list check, rank rows, chains, matching); a larger one is
+//     k_serve_prep, a flat k_rank_rows16 pass over all its women's rows (one
+//     wave per row, so a large request is spread over the whole GPU) and
+//     k_serve_chains (one workgroup per request).  Results, proposal counts,
+//     error bits and a
 //     completion flag go straight into pinned host memory; the server loop
 //     polls the flag.
 // The plain mode is the baseline: one request at a time through the plain
@@ -94,141 +95,14 @@ __global__ __launch_bounds__(BLOCK, 8) void k_serve_chains(const ServeReq* __res
   }
 }
 
-// Small requests (n <= kSmallN): the whole solve in the request's own workgroup,
-// one launch per lane launch.  Split over a flat rank kernel and a chain kernel,
-// a small request paid two dependent HBM misses per chain step (its list entry,
-// then its rank, which the flat pass had left in other XCDs' L2s): ~180 us at
-// n < 512.  Here the workgroup inverts its own women's rows (rank[w][pref_w[w][r]]
-// = r, the F9RankMatrixInitKernel step, src/smp/kernel.cu:672-686) and writes the
-// int16 ranks itself, so they sit in its own XCD's L2 when its chains read them.
-constexpr int kSmallN = 512;
-constexpr int kSmallPer = 16;  // entries each thread loads per round of the rank pass
-// LDS prefix: the first kSmallP positions of every man's list with their ranks,
-// filled from the LDS rank rows during the rank pass.  Nearly all chain steps of
-// uniform lists fall inside it and then read no global memory at all.
-constexpr int kSmallP = 32;
-
-// The instance's women's rank rows, as many whole rows per round as
-// BLOCK * kSmallPer entries hold.  Each thread issues all of its loads of a
-// round before the LDS staging is cleared, so a round costs one memory round
-// trip.  `buf` holds BLOCK * kSmallPer int16 of LDS.  Returns this thread's
-// error bits.
-// `lpfx` (n * kSmallP words of LDS) must hold (w << 16) | 0xFFFF for each
-// prefix entry (kLdsPfxBad outside the list or for an entry outside [0, n)); the
-// rank of every entry whose woman's row passes through `buf` is filled in.
-template <int BLOCK>
-__device__ int wg_rank_small(const int32_t* __restrict__ pw, int16_t* __restrict__ rank, int n, int16_t* buf,
-                             uint32_t* lpfx) {
-  constexpr int kRound = BLOCK * kSmallPer;
-  const int rows_per = kRound / n;  // n <= kSmallN < kRound
-  const int npfx = n * kSmallP;
-  int bad = 0;
-  for (int r0 = 0; r0 < n; r0 += rows_per) {
-    const int cnt = min(rows_per, n - r0) * n;
-    const int32_t* src = pw + (size_t)r0 * n;
-    int v[kSmallPer];
-#pragma unroll
-    for (int j = 0; j < kSmallPer; ++j) {
-      const int idx = (int)threadIdx.x + j * BLOCK;
-      v[j] = idx < cnt ? src[idx] : -1;
-    }
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) buf[i] = -1;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kSmallPer; ++j) {
-      const int idx = (int)threadIdx.x + j * BLOCK;
-      if (idx < cnt) {
-        const int row = idx / n, col = idx - row * n;
-        if ((unsigned)v[j] < (unsigned)n) buf[row * n + v[j]] = (int16_t)col;
-        else bad |= kErrEntryRange;
-      }
-    }
-    __syncthreads();
-    const int nr = cnt / n;
-    for (int i = threadIdx.x; i < npfx; i += BLOCK) {
-      const uint32_t e = lpfx[i];
-      if (e == kLdsPfxBad) continue;
-      const int w = (int)(e >> 16);
-      if (w >= r0 && w < r0 + nr) {
-        const int16_t x = buf[(w - r0) * n + i / kSmallP];
-        if (x >= 0) lpfx[i] = (e & 0xFFFF0000u) | (uint32_t)(uint16_t)x;
-      }
-    }
-    int16_t* dst = rank + (size_t)r0 * n;
-    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
-      const int16_t x = buf[i];
-      if (x < 0) bad |= kErrNotPerm;
-      dst[i] = x;
-    }
-    __syncthreads();  // buf is cleared again by the next round
-  }
-  return bad;
-}
-
-// One workgroup per request of at most kSmallN men: descriptor from pinned host
-// memory, rank rows, chains (16 lanes per chain), matching, proposal count and
-// error bits into pinned host memory, then the completion flag as in
-// k_serve_chains.
-template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK, 2) void k_serve_small(const ServeReq* __restrict__ host_reqs, int count,
-                                                          unsigned long long* h_props, int* h_errs, int* done,
-                                                          unsigned* h_flag, unsigned seq) {
-  static_assert(sizeof(ServeReq) % 4 == 0 && sizeof(ServeReq) / 4 <= 64, "descriptor words");
-  constexpr int kDescWords = (int)(sizeof(ServeReq) / 4);
-  __shared__ unsigned long long partner[kSmallN];
-  __shared__ int16_t buf[BLOCK * kSmallPer];
-  __shared__ uint32_t lpfx[kSmallN * kSmallP];
-  __shared__ int desc[kDescWords];
-  __shared__ int cursor, sbad;
-  __shared__ unsigned long long red[BLOCK / 64];
-  const int i = blockIdx.x;
-  if (threadIdx.x < kDescWords) desc[threadIdx.x] = reinterpret_cast<const int*>(host_reqs + i)[threadIdx.x];
-  if (threadIdx.x == 0) sbad = 0;
-  __syncthreads();
-  const ServeReq q = *reinterpret_cast<const ServeReq*>(desc);
-  int bad = 0;
-  if (q.n <= kSmallN && q.n > 0) {
-    const int n = q.n;
-    // The men's list prefixes (women only; ranks come with the rank rows).
-    constexpr int kPer = kSmallN * kSmallP / BLOCK;
-    int wv[kPer];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const int idx = (int)threadIdx.x + j * BLOCK, m = idx / kSmallP, k = idx % kSmallP;
-      wv[j] = m < n && k < n ? q.pm[(size_t)m * n + k] : -1;
-    }
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const int idx = (int)threadIdx.x + j * BLOCK;
-      lpfx[idx] = (unsigned)wv[j] < (unsigned)n ? ((uint32_t)wv[j] << 16) | 0xFFFFu : kLdsPfxBad;
-    }
-    __syncthreads();
-    bad = wg_rank_small<BLOCK>(q.pw, q.rank, n, buf, lpfx);
-    InstanceRef ref{q.pm, q.pw, q.rank, q.match, n};
-    unsigned long long p = 0;
-    bad |= wg_chains_g<BLOCK, G, 8, false, kSmallP>(ref, partner, &cursor, red, &p, lpfx);
-    if (threadIdx.x == 0) h_props[i] = p;
-  } else if (threadIdx.x == 0) {
-    h_props[i] = 0;
-    bad = kErrEntryRange;  // the host sends only 1 <= n <= kSmallN here
-  }
-  if (bad) atomicOr(&sbad, bad);
-  __syncthreads();
-  if (threadIdx.x == 0) h_errs[i] = sbad;
-  __threadfence_system();
-  __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(done, 1) == count - 1) {
-    *done = 0;
-    __hip_atomic_store(h_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-}
-
-// Requests of up to kFusedLaneN men in a launch whose largest is above kSmallN:
-// each request in one workgroup, by the batched path's per-instance solve
-// (wg_solve_instance: shared men's list check, women's rank rows staged in LDS,
-// Linv for a shared list, chains of one wave each with the women's words in
-// LDS, matching), in one kernel instead of a flat rank pass and a chain kernel.
-// Descriptor, results, error bits and the completion flag as in k_serve_small.
+// Launches whose requests all have at most kFusedLaneN men: each request in one
+// workgroup, by the batched path's per-instance solve (wg_solve_instance: shared
+// men's list check, women's rank rows staged in LDS and written where the
+// workgroup's own chains read them, Linv for a shared list, chains of one wave
+// each with the women's words in LDS, matching), one kernel instead of a flat
+// rank pass and a chain kernel.  The descriptor is read from pinned host memory;
+// the matching, proposal count and error bits go into pinned host memory, and
+// the completion flag is written as in k_serve_chains.
 constexpr int kFusedLaneN = 2048;
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK, 8) void k_serve_fused(const ServeReq* __restrict__ host_reqs, int count,
@@ -707,25 +581,8 @@ struct smp_server {
     }
     ++L.seq;
     hipError_t e = hipSuccess;
-    if (bmax <= pactae::kSmallN) {
-      // Small requests only: one fused kernel, one workgroup each.
-      auto kern = pactae::k_serve_small<1024, 16>;
-      switch (opts.small_chain_lanes) {
-        case 2: kern = pactae::k_serve_small<1024, 2>; break;
-        case 4: kern = pactae::k_serve_small<1024, 4>; break;
-        case 8: kern = pactae::k_serve_small<1024, 8>; break;
-        default: break;
-      }
-      hipLaunchKernelGGL(kern, dim3(count), dim3(1024), 0, L.stream, L.h_reqs_dev, count, L.h_props_dev,
-                         L.h_errs_dev, L.d_done, L.h_flag_dev, L.seq);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-      if ((e = hipEventRecord(L.done, L.stream)) != hipSuccess) return e;
-      L.busy = true;
-      ++stats.launches;
-      return hipSuccess;
-    }
     if (bmax <= pactae::kFusedLaneN && opts.fused_lanes) {
-      // Mid-size requests: one fused kernel, one workgroup each.
+      // One fused kernel, one workgroup per request.
       const size_t lds = pactae::fused_lds(1024, bmax);
       hipLaunchKernelGGL(pactae::k_serve_fused<1024>, dim3(count), dim3(1024), lds, L.stream, L.h_reqs_dev, count,
                          L.h_props_dev, L.h_errs_dev, L.d_done, L.h_flag_dev, L.seq, bmax);
@@ -871,7 +728,6 @@ void smp_server_default_opts(smp_server_opts* o) {
   o->large_n = 512;
   o->large_workers = 2;
   o->large_spill_us = 0;
-  o->small_chain_lanes = 0;
   o->small_to_idle_worker = -1;
   o->fused_lanes = -1;
 }
@@ -893,9 +749,6 @@ int smp_server_create(const smp_server_opts* opts, smp_server** out) {
   if (o.large_n <= 0) o.large_n = 512;
   if (o.small_to_idle_worker < 0) o.small_to_idle_worker = 1;
   if (o.fused_lanes < 0) o.fused_lanes = 1;
-  if (o.small_chain_lanes != 2 && o.small_chain_lanes != 4 && o.small_chain_lanes != 8 &&
-      o.small_chain_lanes != 16)
-    o.small_chain_lanes = 16;
   if (o.large_workers <= 0) o.large_workers = 2;
   if (o.large_workers > 8) {
     pactae::set_error("server: large_workers must be <= 8");

@@ -6,10 +6,15 @@
 //                   packed (rank, man, position) word; displaced men continue in
 //                   the same group of lanes; then the matching (PostProc,
 //                   src/smp/smp_engine_bamboo.cu:299-303) and the proposal count
-//                   (the serving lanes);
+//                   (the serving lanes' chain kernel, for launches above
+//                   2,048 men);
 //   wg_pool_chains  the same walk over a pool of instances open in LDS at once
 //                   (the batched pool mode), with a window per step that follows
 //                   each instance's running mean reach;
+//   wg_solve_instance  one workgroup solves one instance alone: shared men's
+//                   list check, women's rank rows staged in LDS, the walk, the
+//                   matching (the batched fused mode and the serving lanes up to
+//                   2,048 men);
 //   k_rank_rows16   the women's rank rows (rank[w][pref_w[w][r]] = r, the
 //                   F9RankMatrixInitKernel step, src/smp/kernel.cu:672-686) of many
 //                   instances in one flat pass, one wave per row;
@@ -57,15 +62,9 @@ __device__ inline uint32_t rank_of(const int16_t* rk, int n, uint32_t w, uint32_
 // `partner` holds n words of LDS, `cursor` one int of LDS, `red` BLOCK/64 words.
 // The rank matrix must be complete and visible to the workgroup.  Thread 0
 // receives the proposal count in *props.  Returns this thread's error bits.
-// With P > 0, `lpfx` is an LDS prefix of every man's list: lpfx[m * P + k] =
-// (w << 16) | rank of m in w's list for positions k < P (kLdsPfxBad marks an
-// entry outside [0, n)); those positions are then read from LDS instead of from
-// the list and the rank matrix.
-constexpr uint32_t kLdsPfxBad = 0xFFFFFFFFu;
-template <int BLOCK, int G, int JMAX, bool kTransposed, int P = 0>
+template <int BLOCK, int G, int JMAX, bool kTransposed>
 __device__ int wg_chains_g(const InstanceRef& d, unsigned long long* partner, int* cursor,
-                           unsigned long long* red, unsigned long long* props,
-                           const uint32_t* lpfx = nullptr) {
+                           unsigned long long* red, unsigned long long* props) {
   static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "group size");
   static_assert(JMAX >= 1 && JMAX <= 8, "window");
   constexpr int NW = BLOCK / 64;
@@ -108,22 +107,13 @@ __device__ int wg_chains_g(const InstanceRef& d, unsigned long long* partner, in
     if (!done) {
       int wv[JMAX];
       uint32_t rv[JMAX];
-      bool lds_rank[JMAX];
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) {
         const int pos = k + gl + G * j;
         wv[j] = -1;
         rv[j] = 0;
-        lds_rank[j] = false;
         if (j < J && pos < n) {
-          if (P > 0 && pos < P) {
-            const uint32_t e = lpfx[m * P + pos];
-            wv[j] = e == kLdsPfxBad ? n : (int)(e >> 16);
-            rv[j] = e & 0xFFFFu;
-            lds_rank[j] = true;
-          } else {
-            wv[j] = pm[(size_t)m * n + pos];
-          }
+          wv[j] = pm[(size_t)m * n + pos];
           if ((unsigned)wv[j] >= (unsigned)n) {
             bad |= kErrEntryRange;
             wv[j] = -1;
@@ -132,7 +122,7 @@ __device__ int wg_chains_g(const InstanceRef& d, unsigned long long* partner, in
       }
 #pragma unroll
       for (int j = 0; j < JMAX; ++j)
-        if (wv[j] >= 0 && !lds_rank[j]) rv[j] = rank_of<kTransposed>(rk, n, (uint32_t)wv[j], (uint32_t)m);
+        if (wv[j] >= 0) rv[j] = rank_of<kTransposed>(rk, n, (uint32_t)wv[j], (uint32_t)m);
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) {
         if (wv[j] >= 0 && jf == 8) {
