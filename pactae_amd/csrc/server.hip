@@ -300,7 +300,7 @@ struct smp_server {
   // single-instance pipelines of the large-request workers and of plain mode
   // have their own) and of the list copies (host-submitted requests).
   size_t rank_bytes(const Request& r) const {
-    return opts.mode != SMP_SERVE_BATCHED || is_large(r) || r.to_worker
+    return opts.mode != SMP_SERVE_BATCHED || r.to_worker
                ? 0
                : (((size_t)r.n * r.n * 2 + 255) & ~size_t(255));
   }
@@ -781,6 +781,9 @@ int smp_server_submit(smp_server* s, const int32_t* pm, const int32_t* pw, int32
     r->host_pm.assign(pm, pm + (size_t)n * n);
     r->host_pw.assign(pw, pw + (size_t)n * n);
   }
+  // (Admission decides where the request goes; a large one that the workers
+  // take needs no rank block.)
+  r->to_worker = s->is_large(*r) && !s->large_workers.empty();
   if (s->footprint(*r) > s->arena.size()) {
     pactae::set_error("server submit: request footprint exceeds the HBM budget");
     return SMP_ERR_NOMEM;
