@@ -1097,6 +1097,13 @@ struct smp_engine : public pactae::AbsSmpEngine {
         // faster; below n = 1,024 the wider host copy costs more than it saves).
         // The engine keeps the width for its later solves of the same lists.
         if (opts.prefix_cols <= 0 && n >= 1024) K = std::min(K_alloc, std::max(K, 256));
+        // Their tail is also long: the host walks each of its proposals, while
+        // the GPU's many chains still find candidates fast.  With the automatic
+        // threshold the host takes over at 16 free men instead of 64 (correlated
+        // lists, alternating runs on one box: n = 20,000 three seeds 4.3-4.4 ->
+        // 3.1, 1.8-1.9 -> 1.5-1.6 and 4.0 -> 2.9 ms, n = 8,000 0.79-0.88 -> 0.71
+        // ms, n = 2,000 to 5,000 the same or a little faster; 8 was slower).
+        if (T > 16 && opts.tail_threshold <= 0 && !(opts.tail_mode == SMP_TAIL_AUTO && tuned_T >= 0)) T = 16;
         take_span(span_rank, &times.rank_kernel_ms);
         SMP_HIP_TRY(hipEventRecord(ev[1], stream));
         SMP_HIP_TRY(pactae::launch_gather_pr(d_pref_m, d_rank_w, d_pr, n, K, d_sc, stream));
