@@ -544,12 +544,14 @@ struct smp_engine : public pactae::AbsSmpEngine {
     n = ni;
     K = prefix_for(ni);
     full_pr_next = false;
-    // Default from the measured sweep at n = 20,000 (uniform): the host takes
-    // over at 64 free men; smp_tune replaces this with a measurement.
+    // Default from measured sweeps (uniform lists): the host takes over at 48
+    // free men, which the tuner picks at n = 20,000 (untuned solves at 64 were
+    // 0.04-0.10 ms slower there); at n = 2,000 to 8,000, 48 and 64 are within
+    // 2%.  smp_tune replaces this with a measurement.
     if (opts.tail_mode == SMP_TAIL_GPU)
       T = 0;
     else
-      T = opts.tail_threshold > 0 ? std::min(opts.tail_threshold, ni) : std::min(ni, 64);
+      T = opts.tail_threshold > 0 ? std::min(opts.tail_threshold, ni) : std::min(ni, 48);
     if (ni == n_alloc && opts.tail_mode == SMP_TAIL_AUTO && tuned_T >= 0) {
       if (opts.prefix_cols <= 0) K = tuned_K;
       if (opts.tail_threshold <= 0) T = tuned_T;
@@ -632,7 +634,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
       opts = saved;
       opts.tail_mode = SMP_TAIL_AUTO;
       opts.prefix_cols = opts.tail_threshold = 0;
-      const int def_k = prefix_for(n), def_t = std::min(n, 64);
+      const int def_k = prefix_for(n), def_t = std::min(n, 48);
       bool have = false;
       for (const Cand& c : finals) have = have || (c.k == def_k && c.t == def_t);
       if (!have) finals.push_back({def_k, def_t, 0});
