@@ -152,15 +152,28 @@ def single_instance(smp, args):
         it["dm"].free()
         it["dw"].free()
         del it["eng"], it["dm"], it["dw"]
+    # Per seed, the median of its timed solves: a host that stalls for one solve
+    # (the host tail runs on a shared CPU) does not move it, as it moves a mean.
+    seed_medians = [statistics.median(ws) for ws in per_seed if ws]
     seed_means = [statistics.mean(ws) for ws in per_seed if ws]
-    log(f"timed {args.steps} solves over {len(seeds)} seeds: per-seed means "
-        f"{[round(x, 3) for x in seed_means]} ms, median {statistics.median(seed_means):.3f} ms")
+    log(f"timed {args.steps} solves over {len(seeds)} seeds: per-seed medians "
+        f"{[round(x, 3) for x in seed_medians]} ms (means {[round(x, 3) for x in seed_means]}), "
+        f"median {statistics.median(seed_medians):.3f} ms")
     headline = {
         "tuned_on_seed": args.seed, "chosen": {k: int(v) for k, v in chosen.items()},
         "timing_seeds": seeds,
-        "per_seed_ms": [round(statistics.mean(ws), 4) if ws else None for ws in per_seed],
+        "per_seed_ms": [round(statistics.median(ws), 4) if ws else None for ws in per_seed],
+        "per_seed_mean_ms": [round(statistics.mean(ws), 4) if ws else None for ws in per_seed],
         "per_seed_solves": [len(ws) for ws in per_seed],
-        "median_of_seeds_ms": round(statistics.median(seed_means), 4),
+        # Where each seed's time went (medians over its timed solves): the GPU
+        # phases repeat within microseconds from run to run, the host tail does not.
+        "per_seed_phase_ms": [
+            {ph: round(statistics.median(profs[i][ph] for i in range(args.steps) if owner[i] == si), 4)
+             for ph in ("rank_kernel_ms", "propose_kernel_ms", "host_tail_ms")}
+            if any(owner[i] == si for i in range(args.steps)) else None
+            for si in range(len(seeds))],
+        "median_of_seeds_ms": round(statistics.median(seed_medians), 4),
+        "median_of_seed_means_ms": round(statistics.median(seed_means), 4),
         "warmup_solves_per_engine": warm,
         "untuned_defaults_ms_on_tuning_seed": {"median": round(statistics.median(untuned), 4), "runs": len(untuned)},
         "tuned_ms_on_tuning_seed": {"median": round(statistics.median(tuned_same_seed), 4),
