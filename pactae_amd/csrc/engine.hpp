@@ -527,6 +527,7 @@ struct smp_engine : public pactae::AbsSmpEngine {
   hipError_t wait_flag(int f, unsigned seq) {
     for (unsigned i = 1;; ++i) {
       if (flag_seen(f, seq)) return hipSuccess;
+      __builtin_ia32_pause();  // leaves the core's other hardware thread its resources
       if ((i & 4095) == 0) {
         const hipError_t q = hipStreamQuery(stream);
         if (q == hipErrorNotReady) continue;
